@@ -1,0 +1,176 @@
+"""XC Fock-build benchmark on one MI355X.
+
+Workload: C20H42 (all-anti, seeded jitter) / def2-SVP (490 AOs) / PBE on the
+default SSF accuracy-4 grid, with the synthetic density P = 2 C_occ C_occ^T
+(81 occupied orbitals, C_occ = S^-1/2 Q).  A step is one XC Fock build
+P -> (F, E_xc) with the grid, basis and AO tiles resident in HBM and P already
+on the device.  Both contraction precisions are timed on the same P: the
+FP32-MFMA path (the headline value) and the all-FP64 path; dE_xc and max|dF|
+of each against the other and against the CPU oracle are reported.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_TFLOPS = {"FP64": 78.6, "FP32": 157.3}  # MI355X dense matrix peaks
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def time_path(ctx, pot, dP, dF, steps, warmup):
+    for _ in range(warmup):
+        pot.build_device(dP, dF, sync=False)
+    ctx.synchronize()
+    ctx.profile_collect()
+    ctx.profile(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pot.build_device(dP, dF, sync=False)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    stages, nb = ctx.profile_collect()
+    assert nb == steps, (nb, steps)
+    return (t1 - t0) * 1e3 / steps, {k: v / steps for k, v in stages.items()}
+
+
+def cpu_baseline(sysd, min_seconds):
+    from oracle import oracle as O
+    b, g, P, func = sysd["basis"], sysd["grid"], sysd["P"], sysd["cfg"]["functional"]
+    reps, t0 = 0, time.perf_counter()
+    F = E = None
+    while True:
+        F, E, _, _ = O.fock_build(b, g.points, g.weights, P, func)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds or reps >= 50:
+            break
+    return dict(seconds=el, reps=reps, F=F, E=E, threads=O.max_threads())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the XC Fock build runs on one MI355X here "
+              "(multi-GPU block sharding is a design note, not part of this build).", file=sys.stderr)
+        sys.exit(2)
+
+    from serenity_amd import xc, workloads as W
+
+    t0 = time.perf_counter()
+    sysd = W.build_system(args.config)
+    b, g, P, cfg = sysd["basis"], sysd["grid"], sysd["P"], sysd["cfg"]
+    N, n = g.getNGridPoints(), b.nao
+    log("workload %s: %d grid points, %d AOs, host setup %.2fs" % (cfg["name"], N, n, time.perf_counter() - t0))
+
+    ctx = xc.Context(0)
+    t0 = time.perf_counter()
+    bf = xc.BasisFunctionOnGridController(ctx, b, g, 128, 1e-9, 1, keep_fp32=True)
+    ctx.synchronize()
+    t_ao = time.perf_counter() - t0
+    sum_nb, sum_Bnb2, tile_bytes = bf.stats()
+    nblocks = bf.getNBlocks()
+    log("AO tiles resident: %d blocks, mean n_b %.1f, %.2f GB, setup %.2fs" %
+        (nblocks, sum_nb / nblocks, tile_bytes / 1e9, t_ao))
+
+    dP, dF = ctx.alloc(8 * n * n), ctx.alloc(8 * n * n)
+    ctx.upload(dP, np.asfortranarray(P).ravel(order="F"))
+    results = {}
+    for prec in ("FP64", "FP32"):
+        pot = xc.FuncPotential(bf, cfg["functional"], prec)
+        ms, stages = time_path(ctx, pot, dP, dF, args.steps, args.warmup)
+        E = pot.build_device(dP, dF, sync=True)
+        F = np.zeros(n * n)
+        ctx.download(F, dF)
+        results[prec] = dict(ms=ms, stages=stages, E=E, F=F.reshape(n, n, order="F"))
+        log("%s: %.3f ms/build  (%.3e pts/s)  stages(ms) %s" %
+            (prec, ms, N / (ms * 1e-3), {k: round(v, 4) for k, v in stages.items()}))
+    ctx.free(dP)
+    ctx.free(dF)
+
+    # algorithmic work: 2 sum_b B_b n_b^2 flops per contraction (phi.P and phi^T.G)
+    flops_per_kernel = 2.0 * sum_Bnb2
+
+    def roofline(prec):
+        st = results[prec]["stages"]
+        kern = max(("density", "fock"), key=lambda k: st[k])
+        achieved = flops_per_kernel / (st[kern] * 1e-3) / 1e12
+        return dict(bound="mfma", kernel=kern, achieved=round(achieved, 3), peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
+                    frac=round(achieved / PEAK_TFLOPS[prec], 5), traffic=None,
+                    flops_per_launch=flops_per_kernel, avg_launch_ms=round(st[kern], 5))
+
+    r64, r32 = results["FP64"], results["FP32"]
+    out_extra = dict(
+        dE_xc_fp32_vs_fp64=abs(r32["E"] - r64["E"]),
+        max_dF_fp32_vs_fp64=float(np.abs(r32["F"] - r64["F"]).max()),
+        E_xc_fp64=r64["E"],
+        fp64=dict(ms_per_build=round(r64["ms"], 5), grid_pts_per_s=N / (r64["ms"] * 1e-3),
+                  builds_per_s=1e3 / r64["ms"], stages_ms=r64["stages"], roofline=roofline("FP64")),
+        fp32_mfma=dict(ms_per_build=round(r32["ms"], 5), grid_pts_per_s=N / (r32["ms"] * 1e-3),
+                       builds_per_s=1e3 / r32["ms"], stages_ms=r32["stages"]),
+        speedup_fp32_over_fp64=r64["ms"] / r32["ms"],
+        ao_setup_s=round(t_ao, 3),
+    )
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline: oracle (restatement of Serenity's per-block path, 3 AO passes per build)")
+        cb = cpu_baseline(sysd, args.cpu_seconds)
+        pts_s = N * cb["reps"] / cb["seconds"]
+        cpu = dict(value=pts_s, unit="grid pts/s", cores=cb["threads"], kind="port",
+                   sample="%d full XC Fock builds of the same workload (%.1f s, OpenMP, FP64, built-in loops, "
+                          "no BLAS)" % (cb["reps"], cb["seconds"]),
+                   builds_per_s=cb["reps"] / cb["seconds"])
+        out_extra["dE_xc_fp64_vs_oracle"] = abs(r64["E"] - cb["E"])
+        out_extra["max_dF_fp64_vs_oracle"] = float(np.abs(r64["F"] - cb["F"]).max())
+        out_extra["dE_xc_fp32_vs_oracle"] = abs(r32["E"] - cb["E"])
+        out_extra["max_dF_fp32_vs_oracle"] = float(np.abs(r32["F"] - cb["F"]).max())
+        log("oracle: %.3f builds/s on %d threads; FP64 |dE| %.2e max|dF| %.2e" %
+            (cpu["builds_per_s"], cb["threads"], out_extra["dE_xc_fp64_vs_oracle"], out_extra["max_dF_fp64_vs_oracle"]))
+
+    line = dict(
+        metric="XC Fock-build grid pts/s (FP32-MFMA contractions, FP64 functional; FP64 path and dE stated)",
+        value=N / (r32["ms"] * 1e-3),
+        unit="grid pts/s",
+        n_gpus=1,
+        steps=args.steps,
+        warmup=args.warmup,
+        ms_per_step=round(r32["ms"], 5),
+        higher_is_better=True,
+        scaling="weak",
+        vs_baseline=None,
+        dtype="f32",
+        data="synthetic",
+        config=dict(workload=cfg["name"] + ", SSF/Ahlrichs/Lebedev accuracy-4 grid, seeded geometry, "
+                    "synthetic P (P = 2 C C^T, C = S^-1/2 Q)", npoints=N, nao=n, nblocks=nblocks,
+                    mean_nb=round(sum_nb / nblocks, 2), seed=W.SEED, parallelism="single GPU"),
+        roofline=roofline("FP32"),
+        cpu_baseline=cpu,
+    )
+    line.update(out_extra)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

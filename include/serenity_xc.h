@@ -1,0 +1,195 @@
+/*
+ * serenity_xc.h -- C ABI of the MI355X-native XC Fock build.
+ *
+ * The library replaces the hot path of Serenity's FuncPotential
+ * (src/potentials/FuncPotential.cpp:73-111): AO values/gradients on grid
+ * blocks, density and density gradient on the grid, the XC functional, and the
+ * grid-to-matrix Fock assembly.  Every entry point returns an int status
+ * (XC_OK = 0); on failure xc_last_error() returns a thread-local message.  Host
+ * pointers are borrowed for the duration of the call; handles own their device
+ * memory.  Matrices are n x n column-major doubles (Eigen's default layout),
+ * grid points are 3 x N column-major doubles (x, y, z contiguous per point,
+ * Eigen::Matrix3Xd), all lengths in bohr and energies in hartree.
+ *
+ * Threading: calls on one xc_ctx are serialised internally; different
+ * contexts are independent.
+ */
+#ifndef SERENITY_XC_H
+#define SERENITY_XC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  XC_OK = 0,
+  XC_ERR_ARG = 1,      /* invalid argument (null handle, size mismatch, ...) */
+  XC_ERR_DEVICE = 2,   /* HIP runtime error */
+  XC_ERR_MEMORY = 3,   /* device footprint above the admitted budget */
+  XC_ERR_STATE = 4,    /* call order / missing data */
+  XC_ERR_INPUT = 5     /* bad input data (parse error, duplicated atoms, ...) */
+};
+
+/* Functionals available on the device.  LDA = Slater + VWN5, PBE = PBE
+ * exchange + PBE correlation (PW92 LDA part), TF = Thomas-Fermi kinetic.
+ * Serenity: functional_definitions.dat:44,47,80. */
+enum { XC_FUNC_LDA = 0, XC_FUNC_PBE = 1, XC_FUNC_TF = 2, XC_FUNC_SLATER = 3, XC_FUNC_VWN5 = 4,
+       XC_FUNC_PBEX = 5, XC_FUNC_PBEC = 6 };
+
+/* Precision of the two dense contractions (phi.P and phi^T.(w v phi)).  The
+ * functional, the E_xc reduction and the F accumulation are FP64 in every mode. */
+enum { XC_PREC_FP64 = 0, XC_PREC_FP32 = 1 };
+
+/* Grid partition and radial quadrature (Serenity Options::GRID_TYPES / RADIAL_GRID_TYPES). */
+enum { XC_GRID_BECKE = 0, XC_GRID_VORONOI = 1, XC_GRID_SSF = 2 };
+enum { XC_RAD_BECKE = 0, XC_RAD_HANDY = 1, XC_RAD_AHLRICHS = 2, XC_RAD_KNOWLES = 3, XC_RAD_EQUI = 4 };
+
+typedef struct xc_ctx xc_ctx;       /* one device + one stream */
+typedef struct xc_basis xc_basis;   /* contracted shells, resident on the device */
+typedef struct xc_grid xc_grid;     /* grid points and weights, resident on the device */
+typedef struct xc_bfogc xc_bfogc;   /* basis functions on grid blocks: screening lists + resident AO tiles */
+
+/* ---- errors, version ---------------------------------------------------- */
+const char* xc_last_error(void);
+int xc_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates a context on HIP device `device` with its own (default-priority) stream. */
+int xc_ctx_create(int device, xc_ctx** out);
+int xc_ctx_destroy(xc_ctx* ctx);
+int xc_ctx_synchronize(xc_ctx* ctx);
+/* The hipStream_t every kernel of this context is launched on. */
+void* xc_ctx_stream(xc_ctx* ctx);
+/* Device bytes currently held by handles of this context. */
+int xc_ctx_memory(xc_ctx* ctx, int64_t* bytes_in_use);
+
+/* ---- host-side helpers (no device work) ------------------------------- */
+/* libint2-style renormalisation of one contraction, in place
+ * (Serenity Shell ctor, src/basis/Shell.cpp:30-47). */
+int xc_normalize_contraction(int l, int nprim, const double* alpha, double* coef);
+
+/* Parses a Turbomole-format basis file for every atom of a geometry and
+ * returns the shell list in Serenity's AO order (atoms in input order, shells
+ * in file order; AtomCenteredBasisController.cpp:60-75,
+ * BasisFunctionProvider.cpp:60-130).  Output arrays are allocated by the
+ * library and released with xc_free_host.  coef is renormalised. */
+int xc_load_basis(const char* path, int natoms, const int* Z, const double* coords_bohr, int pure,
+                  int* nshell, int** shell_l, int** shell_nprim, int** shell_atom, double** shell_center,
+                  double** alpha, double** coef);
+void xc_free_host(void* p);
+
+/* Molecular grid (Serenity GridFactory::produce + AtomCenteredGridController
+ * sort).  points: 3 x N (xyz per point), weights: N; released with xc_free_host. */
+int xc_molecular_grid(int natoms, const int* Z, const double* coords_bohr, int grid_type, int radial_type,
+                      int accuracy, int smoothing, double weight_threshold, int sort, int64_t* npoints,
+                      double** points, double** weights, int64_t* npoints_before_cut);
+
+/* Hilbert-curve ordering of npoints points (3 x N, xyz per point) and their
+ * weights, in place (HilbertRTreeSorting::sort, src/grid/HilbertRTreeSorting.cpp
+ * :27-216; equal keys keep their input order). */
+int xc_hilbert_sort(int64_t npoints, double* points, double* weights);
+
+/* Lebedev-Laikov rule with `npoints` points: x,y,z,w arrays of length npoints
+ * (weights sum to 1). */
+int xc_lebedev(int npoints, double* x, double* y, double* z, double* w);
+
+/* Analytic overlap matrix S (n x n column-major) of a shell list. */
+int xc_overlap(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
+               const double* shell_center, const double* alpha, const double* coef, double* S);
+
+/* ---- resident objects --------------------------------------------------- */
+/* Shells in AO order.  shell_center: 3 x nshell; alpha/coef: concatenated
+ * primitives (coef already renormalised).  Replaces BasisController's shell
+ * list as seen by BasisFunctionOnGridController (BasisController.h:157-193). */
+int xc_basis_create(xc_ctx* ctx, int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
+                    const double* shell_center, const double* alpha, const double* coef, xc_basis** out);
+int xc_basis_destroy(xc_basis* basis);
+int xc_basis_nao(const xc_basis* basis, int* nao);
+
+/* Grid of npoints points; replaces GridController::getGridPoints/getWeights
+ * (src/grid/GridController.cpp:30-50). */
+int xc_grid_create(xc_ctx* ctx, int64_t npoints, const double* points, const double* weights, xc_grid** out);
+int xc_grid_destroy(xc_grid* grid);
+
+/* BasisFunctionOnGridController(basis, grid, maxBlockSize, radialThreshold,
+ * highestDerivative) (BasisFunctionOnGridController.h:105-107).  Runs the
+ * block screening and evaluates the AO values (and gradients if
+ * highest_derivative >= 1) of every block into resident HBM tiles.
+ * highest_derivative must be 0 or 1 (the Fock build needs at most 1).
+ * keep_fp32 additionally keeps an FP32 copy of the tiles for XC_PREC_FP32. */
+int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double radial_threshold,
+                    int highest_derivative, int keep_fp32, xc_bfogc** out);
+int xc_bfogc_destroy(xc_bfogc* bfogc);
+/* getNBlocks / getNGridPoints / getNBasisFunctions / getFirstIndexOfBlock. */
+int xc_bfogc_info(const xc_bfogc* bfogc, int* nblocks, int64_t* npoints, int* nao, int* max_block_size);
+/* Sum over blocks of the number of non-negligible AOs, and of B_b * n_b^2
+ * (the algorithmic work of both contractions is 4 * sum B_b n_b^2 flops). */
+int xc_bfogc_stats(const xc_bfogc* bfogc, int64_t* sum_nb, double* sum_B_nb2, int64_t* tile_bytes);
+/* getBlockOnGridData(block) (BasisFunctionOnGridController.cpp:122-131):
+ * copies the block's dense B x nao column-major value matrix (and gradient
+ * components if requested and available; pass NULL to skip) and the
+ * negligible flags (1 = screened out) to the host. */
+int xc_bfogc_block_data(const xc_bfogc* bfogc, int block, double* values, double* grad_x, double* grad_y,
+                        double* grad_z, int* negligible, int* block_size);
+
+/* ---- the path ----------------------------------------------------------- */
+/* DensityOnGridCalculator::calcDensityAndGradientOnGrid(P, rho, grad)
+ * (DensityOnGridCalculator.cpp:54-65, MatrixOperatorToGridTransformer.cpp
+ * :37-198).  P: host n x n.  rho: host N; grad: host 3 x N SoA (x[N], y[N],
+ * z[N]) or NULL for values only.  nonnegligible_blocks: host int[nblocks]
+ * (1 where the block has at least one non-negligible AO) or NULL. */
+int xc_density(xc_bfogc* bfogc, const double* P, double* rho, double* grad, int* nonnegligible_blocks);
+
+/* FunctionalLibrary::calcData(GRADIENTS, functional, density)
+ * (FunctionalLibrary.cpp:38-71, XCFun.cpp:38-158) on host arrays of npoints:
+ * eps = energy density per volume, v = d eps/d rho, g = d eps/d grad rho
+ * (3 x N SoA, may be NULL for LDA-type functionals).  grad may be NULL for
+ * LDA-type functionals.  Blocks of block_size points whose sum |rho| is below
+ * block_size * 1e-12 are skipped (outputs zero).  energy = sum_p w_p eps_p. */
+int xc_functional(xc_ctx* ctx, int functional, int64_t npoints, int block_size, const double* rho,
+                  const double* grad, const double* weights, double* eps, double* v, double* g, double* energy);
+
+/* ScalarOperatorToMatrixAdder::addScalarOperatorToMatrix(F, v[, g])
+ * (ScalarOperatorToMatrixAdder.cpp:51-113, addBlock :179-303): F += integral
+ * of the scalar potential v (and gradient potential g, 3 x N SoA, or NULL).
+ * F: host n x n, updated in place. */
+int xc_fock_add(xc_bfogc* bfogc, double block_ave_threshold, const double* v, const double* g, double* F);
+
+/* FuncPotential::getMatrix + getEnergy (FuncPotential.cpp:66-111) fused:
+ * F = V_xc[P] (overwritten), energy = E_xc[P].  Host arrays. */
+int xc_fock_build(xc_bfogc* bfogc, int functional, int precision, double block_ave_threshold, const double* P,
+                  double* F, double* energy);
+
+/* Same on device-resident buffers (P, F: device n x n; energy: host),
+ * enqueued on the context stream; synchronises before returning the energy
+ * unless energy is NULL (then the value stays in the context and can be read
+ * with xc_last_energy after xc_ctx_synchronize). */
+int xc_fock_build_device(xc_bfogc* bfogc, int functional, int precision, double block_ave_threshold,
+                         const double* dP, double* dF, double* energy);
+int xc_last_energy(xc_ctx* ctx, double* energy);
+
+/* Stage timing with HIP events on the context stream.  While enabled, every
+ * build records an event between its stages; xc_profile_collect waits for
+ * them, returns the summed milliseconds of each stage over the builds since
+ * the last collect (index 0 P to FP32, 1 density, 2 functional, 3 clear of
+ * the Fock accumulator, 4 fock, 5 finalize + energy sum) and resets. */
+enum { XC_STAGE_CONVERT = 0, XC_STAGE_DENSITY = 1, XC_STAGE_FUNCTIONAL = 2, XC_STAGE_CLEAR = 3, XC_STAGE_FOCK = 4,
+       XC_STAGE_FINALIZE = 5, XC_NSTAGES = 6 };
+int xc_profile(xc_ctx* ctx, int enable);
+int xc_profile_collect(xc_ctx* ctx, double* stage_ms, int* nbuilds);
+
+/* Device buffer helpers so that callers without their own allocator (ctypes,
+ * tests, benchmarks) can hold resident P and F. */
+int xc_device_alloc(xc_ctx* ctx, size_t bytes, void** dptr);
+int xc_device_free(xc_ctx* ctx, void* dptr);
+int xc_memcpy_h2d(xc_ctx* ctx, void* dst, const void* src, size_t bytes);
+int xc_memcpy_d2h(xc_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SERENITY_XC_H */

@@ -1,0 +1,618 @@
+// TEST INFRASTRUCTURE -- the CPU oracle of the XC Fock build.
+//
+// A plain C++17/OpenMP FP64 restatement of Serenity's FuncPotential path,
+// block by block as the reference does it.  Only tests/, the smoke check and
+// the CPU-baseline leg of bench.py load it (as libxc_oracle.so, through
+// oracle/oracle.py); the library under serenity_amd/ never calls it.
+//
+// Functions and the reference code each follows:
+//   oracle_ao_block      BasisFunctionOnGridController::calculateBasisFunctionData
+//                        (src/data/grid/BasisFunctionOnGridController.cpp:150-1105)
+//   oracle_density       MatrixOperatorToGridTransformer::transform
+//                        (src/data/grid/MatrixOperatorToGridTransformer.cpp:37-198)
+//   oracle_functional    XCFun::calcData + calcEnergy (src/dft/functionals/wrappers/XCFun.cpp:38-158,
+//                        751-765) for LDA (slaterx + vwn5c), PBE (pbex + pbec), TF (tfk)
+//   oracle_fock_add      ScalarOperatorToMatrixAdder::addScalarOperatorToMatrix / addBlock
+//                        (src/data/grid/ScalarOperatorToMatrixAdder.cpp:51-113, 179-303)
+//   oracle_fock_build    FuncPotential::getMatrix / getEnergy (src/potentials/FuncPotential.cpp:66-111),
+//                        with the reference's three AO evaluations per build
+//
+// Parity pins (tests/test_oracle.py): the reference's Mathematica known-answer
+// values for AO values/gradients/Hessians, density/gradient/Hessian and the
+// Fock assembly; the spherical normalisation invariant; finite-difference
+// checks of every functional derivative and of the AO gradients; the stored
+// E_xc of Serenity's He and water PBE test systems and the H2 LDA Fock matrix.
+// The solid harmonics here use the real-solid-harmonic recurrences of
+// Helgaker, Jorgensen, Olsen (eqs. 6.4.70-6.4.73), evaluated per point, not
+// the coefficient tables the product uses.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+struct OShell {
+  double A[3];
+  int l, pure, nprim;
+  const double* alpha;
+  const double* coef;
+  int first;  // first AO
+  int nfunc;
+};
+
+struct Basis {
+  std::vector<OShell> sh;
+  int nao = 0;
+  Basis(int nshell, const int* l, const int* pure, const int* nprim, const double* center, const double* alpha,
+        const double* coef) {
+    int off = 0, ao = 0;
+    for (int s = 0; s < nshell; ++s) {
+      OShell o;
+      for (int d = 0; d < 3; ++d) o.A[d] = center[3 * s + d];
+      o.l = l[s];
+      o.pure = pure[s];
+      o.nprim = nprim[s];
+      o.alpha = alpha + off;
+      o.coef = coef + off;
+      o.first = ao;
+      o.nfunc = o.pure ? 2 * o.l + 1 : (o.l + 1) * (o.l + 2) / 2;
+      off += o.nprim;
+      ao += o.nfunc;
+      sh.push_back(o);
+    }
+    nao = ao;
+  }
+};
+
+double dfact(int k) {
+  double r = 1.0;
+  for (int i = k; i > 1; i -= 2) r *= i;
+  return r;
+}
+
+// Real solid harmonics S_lm (Racah normalisation) and their gradients by the
+// HJO recurrences.  out[(l)][m+l] for one point; value and d/dx,d/dy,d/dz.
+struct Harm {
+  double v[8][15], x[8][15], y[8][15], z[8][15];
+};
+
+void solid_harmonics(int lmax, double X, double Y, double Z, Harm& H) {
+  const double r2 = X * X + Y * Y + Z * Z;
+  for (int l = 0; l <= lmax; ++l)
+    for (int k = 0; k < 2 * l + 1; ++k) H.v[l][k] = H.x[l][k] = H.y[l][k] = H.z[l][k] = 0.0;
+  H.v[0][0] = 1.0;
+  for (int l = 0; l < lmax; ++l) {
+    // diagonal terms m = +-(l+1)
+    const double c = std::sqrt((l == 0 ? 2.0 : 1.0) * (2.0 * l + 1.0) / (2.0 * l + 2.0));
+    const double sll = H.v[l][2 * l], slm = H.v[l][0];  // S_{l,l}, S_{l,-l}
+    const double f = (l == 0) ? 0.0 : 1.0;
+    const int P = 2 * (l + 1), M0 = 0;
+    H.v[l + 1][P] = c * (X * sll - f * Y * slm);
+    H.x[l + 1][P] = c * (sll + X * H.x[l][2 * l] - f * Y * H.x[l][0]);
+    H.y[l + 1][P] = c * (X * H.y[l][2 * l] - f * (slm + Y * H.y[l][0]));
+    H.z[l + 1][P] = c * (X * H.z[l][2 * l] - f * Y * H.z[l][0]);
+    H.v[l + 1][M0] = c * (Y * sll + f * X * slm);
+    H.x[l + 1][M0] = c * (Y * H.x[l][2 * l] + f * (slm + X * H.x[l][0]));
+    H.y[l + 1][M0] = c * (sll + Y * H.y[l][2 * l] + f * X * H.y[l][0]);
+    H.z[l + 1][M0] = c * (Y * H.z[l][2 * l] + f * X * H.z[l][0]);
+    // |m| <= l by the z recurrence
+    for (int m = -l; m <= l; ++m) {
+      const double a = (2.0 * l + 1.0);
+      const double b = (std::abs(m) <= l - 1) ? std::sqrt((double)(l + m) * (l - m)) : 0.0;
+      const double d = std::sqrt((double)(l + m + 1) * (l - m + 1));
+      const int k1 = m + l + 1;  // index in l+1
+      const int k0 = m + l;      // index in l
+      double sm1 = 0.0, sm1x = 0.0, sm1y = 0.0, sm1z = 0.0;
+      if (b != 0.0) {
+        const int km = m + l - 1;
+        sm1 = H.v[l - 1][km];
+        sm1x = H.x[l - 1][km];
+        sm1y = H.y[l - 1][km];
+        sm1z = H.z[l - 1][km];
+      }
+      H.v[l + 1][k1] = (a * Z * H.v[l][k0] - b * r2 * sm1) / d;
+      H.x[l + 1][k1] = (a * Z * H.x[l][k0] - b * (2.0 * X * sm1 + r2 * sm1x)) / d;
+      H.y[l + 1][k1] = (a * Z * H.y[l][k0] - b * (2.0 * Y * sm1 + r2 * sm1y)) / d;
+      H.z[l + 1][k1] = (a * (H.v[l][k0] + Z * H.z[l][k0]) - b * (2.0 * Z * sm1 + r2 * sm1z)) / d;
+    }
+  }
+}
+
+struct BlockAO {
+  int B = 0, n = 0, deriv = 0;
+  std::vector<double> val, gx, gy, gz;                // B x n column-major
+  std::vector<double> hxx, hxy, hxz, hyy, hyz, hzz;   // B x n (Cartesian shells only)
+  std::vector<int> neg;                               // n flags, 1 = negligible
+};
+
+// AO values of one block, dense over all AOs, with the reference's screening:
+// shell negligible for the whole block when (|centre - A| - spread) >= 1 and
+// |sum_k c_k exp(-a_k d^2)| < thr; per point, primitives with a r^2 >= -ln thr
+// are skipped and the shell is zero when |R| < thr.
+void ao_block(const Basis& bas, const double* pts, int B, double thr, int deriv, BlockAO& out) {
+  const int n = bas.nao;
+  out.B = B;
+  out.n = n;
+  out.deriv = deriv;
+  out.val.assign((size_t)B * n, 0.0);
+  if (deriv >= 1) {
+    out.gx.assign((size_t)B * n, 0.0);
+    out.gy.assign((size_t)B * n, 0.0);
+    out.gz.assign((size_t)B * n, 0.0);
+  }
+  if (deriv >= 2) {
+    for (auto* h : {&out.hxx, &out.hxy, &out.hxz, &out.hyy, &out.hyz, &out.hzz}) h->assign((size_t)B * n, 0.0);
+  }
+  out.neg.assign(n, 0);
+  double c[3] = {0, 0, 0};
+  for (int p = 0; p < B; ++p)
+    for (int d = 0; d < 3; ++d) c[d] += pts[3 * p + d];
+  for (int d = 0; d < 3; ++d) c[d] /= B;
+  double spread = 0.0;
+  for (int p = 0; p < B; ++p) {
+    const double dx = pts[3 * p] - c[0], dy = pts[3 * p + 1] - c[1], dz = pts[3 * p + 2] - c[2];
+    spread = std::max(spread, std::sqrt(dx * dx + dy * dy + dz * dz));
+  }
+  std::vector<char> skip(bas.sh.size(), 0);
+  for (size_t s = 0; s < bas.sh.size(); ++s) {
+    const OShell& o = bas.sh[s];
+    const double dx = c[0] - o.A[0], dy = c[1] - o.A[1], dz = c[2] - o.A[2];
+    double d = std::sqrt(dx * dx + dy * dy + dz * dz) - spread;
+    if (d < 1.0) continue;
+    d *= d;
+    double radial = 0.0;
+    for (int k = 0; k < o.nprim; ++k) radial += o.coef[k] * std::exp(-o.alpha[k] * d);
+    if (std::fabs(radial) < thr) {
+      skip[s] = 1;
+      for (int f = 0; f < o.nfunc; ++f) out.neg[o.first + f] = 1;
+    }
+  }
+  const double expthr = -std::log(thr);
+  Harm H;
+  for (int p = 0; p < B; ++p) {
+    for (size_t s = 0; s < bas.sh.size(); ++s) {
+      if (skip[s]) continue;
+      const OShell& o = bas.sh[s];
+      const double X = pts[3 * p] - o.A[0], Y = pts[3 * p + 1] - o.A[1], Z = pts[3 * p + 2] - o.A[2];
+      const double r2 = X * X + Y * Y + Z * Z;
+      double R = 0.0, dR = 0.0, ddR = 0.0;
+      for (int k = 0; k < o.nprim; ++k) {
+        const double t = o.alpha[k] * r2;
+        if (t < expthr) {
+          const double e = std::exp(-t);
+          R += o.coef[k] * e;
+          dR -= 2.0 * o.alpha[k] * o.coef[k] * e;
+          ddR += 4.0 * o.alpha[k] * o.alpha[k] * o.coef[k] * e;
+        }
+      }
+      if (std::fabs(R) < thr) continue;  // stays zero
+      const int l = o.l;
+      if (!o.pure) {
+        int f = 0;
+        const double top = dfact(2 * l - 1);
+        for (int a = l; a >= 0; --a)
+          for (int b = l - a; b >= 0; --b, ++f) {
+            const int cc = l - a - b;
+            const double N = std::sqrt(top / (dfact(2 * a - 1) * dfact(2 * b - 1) * dfact(2 * cc - 1)));
+            auto pw = [](double x, int e) { return e < 0 ? 0.0 : std::pow(x, e); };
+            const double mono = pw(X, a) * pw(Y, b) * pw(Z, cc);
+            const size_t o_ = (size_t)(o.first + f) * B + p;
+            out.val[o_] = N * R * mono;
+            if (deriv >= 1) {
+              const double mx = a * pw(X, a - 1) * pw(Y, b) * pw(Z, cc);
+              const double my = b * pw(X, a) * pw(Y, b - 1) * pw(Z, cc);
+              const double mz = cc * pw(X, a) * pw(Y, b) * pw(Z, cc - 1);
+              out.gx[o_] = N * (R * mx + dR * X * mono);
+              out.gy[o_] = N * (R * my + dR * Y * mono);
+              out.gz[o_] = N * (R * mz + dR * Z * mono);
+              if (deriv >= 2) {
+                // second derivatives of R(r) m(x,y,z): R'' x_i x_j m + R' (d_ij m + x_i m_j + x_j m_i) + R m_ij
+                const double mxx = a * (a - 1) * pw(X, a - 2) * pw(Y, b) * pw(Z, cc);
+                const double myy = b * (b - 1) * pw(X, a) * pw(Y, b - 2) * pw(Z, cc);
+                const double mzz = cc * (cc - 1) * pw(X, a) * pw(Y, b) * pw(Z, cc - 2);
+                const double mxy = a * b * pw(X, a - 1) * pw(Y, b - 1) * pw(Z, cc);
+                const double mxz = a * cc * pw(X, a - 1) * pw(Y, b) * pw(Z, cc - 1);
+                const double myz = b * cc * pw(X, a) * pw(Y, b - 1) * pw(Z, cc - 1);
+                out.hxx[o_] = N * (ddR * X * X * mono + dR * (mono + 2 * X * mx) + R * mxx);
+                out.hyy[o_] = N * (ddR * Y * Y * mono + dR * (mono + 2 * Y * my) + R * myy);
+                out.hzz[o_] = N * (ddR * Z * Z * mono + dR * (mono + 2 * Z * mz) + R * mzz);
+                out.hxy[o_] = N * (ddR * X * Y * mono + dR * (X * my + Y * mx) + R * mxy);
+                out.hxz[o_] = N * (ddR * X * Z * mono + dR * (X * mz + Z * mx) + R * mxz);
+                out.hyz[o_] = N * (ddR * Y * Z * mono + dR * (Y * mz + Z * my) + R * myz);
+              }
+            }
+          }
+      } else {
+        solid_harmonics(l, X, Y, Z, H);
+        for (int m = 0; m < 2 * l + 1; ++m) {
+          const size_t o_ = (size_t)(o.first + m) * B + p;
+          out.val[o_] = R * H.v[l][m];
+          if (deriv >= 1) {
+            out.gx[o_] = R * H.x[l][m] + dR * X * H.v[l][m];
+            out.gy[o_] = R * H.y[l][m] + dR * Y * H.v[l][m];
+            out.gz[o_] = R * H.z[l][m] + dR * Z * H.v[l][m];
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---- functionals: closed forms with hand-derived first derivatives -------
+struct FParams {
+  double vwn_A = 0.0310907, vwn_b = 3.72744, vwn_c = 12.9352, vwn_x0 = -0.10498;
+  double pw_A = 0.031091, pw_a1 = 0.21370, pw_b1 = 7.5957, pw_b2 = 3.5876, pw_b3 = 1.6382, pw_b4 = 0.49294;
+  double kappa = 0.804, mu = 0.2195149727645171, beta = 0.06672455060314922;
+};
+
+// e, de/drho, de/dsigma
+struct EV {
+  double e, dr, ds;
+};
+
+EV slater(double r) {
+  const double cx = -0.75 * std::cbrt(3.0 / M_PI);
+  const double r13 = std::cbrt(r);
+  return {cx * r * r13, (4.0 / 3.0) * cx * r13, 0.0};
+}
+
+EV vwn5(double r, const FParams& P) {
+  const double rs = std::cbrt(3.0 / (4.0 * M_PI * r));
+  const double x = std::sqrt(rs);
+  const double b = P.vwn_b, c = P.vwn_c, x0 = P.vwn_x0, A = P.vwn_A;
+  const double Q = std::sqrt(4.0 * c - b * b);
+  const double X = x * x + b * x + c, X0 = x0 * x0 + b * x0 + c;
+  const double at = std::atan(Q / (2.0 * x + b));
+  const double eps = A * (std::log(x * x / X) + 2.0 * b / Q * at -
+                          b * x0 / X0 * (std::log((x - x0) * (x - x0) / X) + 2.0 * (b + 2.0 * x0) / Q * at));
+  const double deps_dx = A * (2.0 / x - (2.0 * x + b) / X - b / X -
+                              b * x0 / X0 * (2.0 / (x - x0) - (2.0 * x + b) / X - (b + 2.0 * x0) / X));
+  return {r * eps, eps - x / 6.0 * deps_dx, 0.0};
+}
+
+void pw92(double rs, const FParams& P, double& eps, double& deps_drs) {
+  const double A = P.pw_A;
+  const double s = std::sqrt(rs);
+  const double Q1 = 2.0 * A * (P.pw_b1 * s + P.pw_b2 * rs + P.pw_b3 * rs * s + P.pw_b4 * rs * rs);
+  const double Q1p = 2.0 * A * (0.5 * P.pw_b1 / s + P.pw_b2 + 1.5 * P.pw_b3 * s + 2.0 * P.pw_b4 * rs);
+  const double lg = std::log(1.0 + 1.0 / Q1);
+  eps = -2.0 * A * (1.0 + P.pw_a1 * rs) * lg;
+  deps_drs = -2.0 * A * P.pw_a1 * lg + 2.0 * A * (1.0 + P.pw_a1 * rs) * Q1p / (Q1 * Q1 + Q1);
+}
+
+EV pbex(double r, double sig, const FParams& P) {
+  const double k = 4.0 * std::pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+  const double r83 = std::pow(r, 8.0 / 3.0);
+  const double s2 = sig / (k * r83);
+  const double den = 1.0 + P.mu * s2 / P.kappa;
+  const double F = 1.0 + P.kappa - P.kappa / den;
+  const double dF = P.mu / (den * den);
+  const EV lda = slater(r);
+  return {lda.e * F, lda.dr * F + lda.e * dF * (-8.0 / 3.0) * s2 / r, lda.e * dF / (k * r83)};
+}
+
+EV pbec(double r, double sig, const FParams& P) {
+  const double gamma = (1.0 - std::log(2.0)) / (M_PI * M_PI);
+  const double bg = P.beta / gamma;
+  const double rs = std::cbrt(3.0 / (4.0 * M_PI * r));
+  double eps, deps_drs;
+  pw92(rs, P, eps, deps_drs);
+  const double deps = deps_drs * (-rs / (3.0 * r));
+  const double kt = M_PI / (16.0 * std::cbrt(3.0 * M_PI * M_PI));
+  const double r73 = std::pow(r, 7.0 / 3.0);
+  const double y = kt * sig / r73;
+  const double ex = std::exp(-eps / gamma);
+  const double A = bg / (ex - 1.0);
+  const double dA_deps = bg * (ex / gamma) / ((ex - 1.0) * (ex - 1.0));
+  const double N = 1.0 + A * y, D = 1.0 + A * y + A * A * y * y;
+  const double Phi = bg * y * N / D;
+  const double dPhi_dy = bg * (N / D + y * (A * D - N * (A + 2.0 * A * A * y)) / (D * D));
+  const double dPhi_dA = bg * y * (y * D - N * (y + 2.0 * A * y * y)) / (D * D);
+  const double H = gamma * std::log(1.0 + Phi);
+  const double dH_dPhi = gamma / (1.0 + Phi);
+  const double dH_dr = dH_dPhi * (dPhi_dy * (-7.0 / 3.0) * y / r + dPhi_dA * dA_deps * deps);
+  const double dH_ds = dH_dPhi * dPhi_dy * kt / r73;
+  return {r * (eps + H), eps + H + r * (deps + dH_dr), r * dH_ds};
+}
+
+EV tf(double r) {
+  const double cf = 0.3 * std::pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+  const double r23 = std::pow(r, 2.0 / 3.0);
+  return {cf * r * r23, (5.0 / 3.0) * cf * r23, 0.0};
+}
+
+bool is_gga(int f) { return f == 1 || f == 5 || f == 6; }
+
+EV eval(int f, double r, double sig, const FParams& P) {
+  EV a{0, 0, 0}, b{0, 0, 0};
+  switch (f) {
+    case 0: a = slater(r); b = vwn5(r, P); break;
+    case 1: a = pbex(r, sig, P); b = pbec(r, sig, P); break;
+    case 2: a = tf(r); break;
+    case 3: a = slater(r); break;
+    case 4: a = vwn5(r, P); break;
+    case 5: a = pbex(r, sig, P); break;
+    case 6: a = pbec(r, sig, P); break;
+    default: break;
+  }
+  return {a.e + b.e, a.dr + b.dr, a.ds + b.ds};
+}
+
+FParams params_from(const double* p) {
+  FParams P;
+  if (p) {
+    P.vwn_A = p[0];
+    P.pw_A = p[1];
+    P.kappa = p[2];
+    P.mu = p[3];
+    P.beta = p[4];
+  }
+  return P;
+}
+
+}  // namespace
+
+extern "C" {
+
+// AO data of points[0..B) as one block: values (B x n col-major), gradients,
+// Hessian (Cartesian shells only; deriv = 2), negligible flags.
+int oracle_ao_block(int nshell, const int* l, const int* pure, const int* nprim, const double* center,
+                    const double* alpha, const double* coef, const double* pts, int B, double thr, int deriv,
+                    double* val, double* gx, double* gy, double* gz, double* hess /*6 x B x n*/, int* neg) {
+  Basis bas(nshell, l, pure, nprim, center, alpha, coef);
+  BlockAO ao;
+  ao_block(bas, pts, B, thr, deriv, ao);
+  const size_t sz = (size_t)B * bas.nao;
+  std::memcpy(val, ao.val.data(), sz * sizeof(double));
+  if (deriv >= 1 && gx) {
+    std::memcpy(gx, ao.gx.data(), sz * sizeof(double));
+    std::memcpy(gy, ao.gy.data(), sz * sizeof(double));
+    std::memcpy(gz, ao.gz.data(), sz * sizeof(double));
+  }
+  if (deriv >= 2 && hess) {
+    const std::vector<double>* h[6] = {&ao.hxx, &ao.hxy, &ao.hxz, &ao.hyy, &ao.hyz, &ao.hzz};
+    for (int c = 0; c < 6; ++c) std::memcpy(hess + c * sz, h[c]->data(), sz * sizeof(double));
+  }
+  if (neg) std::memcpy(neg, ao.neg.data(), bas.nao * sizeof(int));
+  return 0;
+}
+
+// rho (N), grad (3 x N SoA, nullable), hess (6 x N SoA, nullable; needs deriv 2
+// and Cartesian shells), nonneg (nblocks, nullable).
+int oracle_density(int nshell, const int* l, const int* pure, const int* nprim, const double* center,
+                   const double* alpha, const double* coef, const double* pts, int64_t N, int bs, double thr,
+                   const double* P, double* rho, double* grad, double* hess, int* nonneg) {
+  Basis bas(nshell, l, pure, nprim, center, alpha, coef);
+  const int n = bas.nao;
+  const int64_t nb = (N + bs - 1) / bs;
+  const int deriv = hess ? 2 : (grad ? 1 : 0);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t first = b * bs;
+    const int B = (int)std::min<int64_t>(bs, N - first);
+    BlockAO ao;
+    ao_block(bas, pts + 3 * first, B, thr, deriv, ao);
+    std::vector<int> sig;
+    for (int i = 0; i < n; ++i)
+      if (!ao.neg[i]) sig.push_back(i);
+    for (int p = 0; p < B; ++p) {
+      rho[first + p] = 0.0;
+      if (grad)
+        for (int d = 0; d < 3; ++d) grad[d * N + first + p] = 0.0;
+      if (hess)
+        for (int d = 0; d < 6; ++d) hess[d * N + first + p] = 0.0;
+    }
+    if (nonneg) nonneg[b] = sig.empty() ? 0 : 1;
+    if (sig.empty()) continue;
+    const int m = (int)sig.size();
+    // Psub = P[sig, sig];  BP = phi_sub Psub  (B x m)
+    std::vector<double> Ps((size_t)m * m), BP((size_t)B * m, 0.0);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) Ps[i + (size_t)m * j] = P[sig[i] + (size_t)n * sig[j]];
+    for (int j = 0; j < m; ++j)
+      for (int k = 0; k < m; ++k) {
+        const double pkj = Ps[k + (size_t)m * j];
+        const double* phik = &ao.val[(size_t)sig[k] * B];
+        double* bpj = &BP[(size_t)j * B];
+        for (int p = 0; p < B; ++p) bpj[p] += phik[p] * pkj;
+      }
+    std::vector<double> GX, GY, GZ;  // grad phi . Psub for the Hessian
+    if (hess) {
+      GX.assign((size_t)B * m, 0.0);
+      GY.assign((size_t)B * m, 0.0);
+      GZ.assign((size_t)B * m, 0.0);
+      for (int j = 0; j < m; ++j)
+        for (int k = 0; k < m; ++k) {
+          const double pkj = Ps[k + (size_t)m * j];
+          for (int p = 0; p < B; ++p) {
+            GX[(size_t)j * B + p] += ao.gx[(size_t)sig[k] * B + p] * pkj;
+            GY[(size_t)j * B + p] += ao.gy[(size_t)sig[k] * B + p] * pkj;
+            GZ[(size_t)j * B + p] += ao.gz[(size_t)sig[k] * B + p] * pkj;
+          }
+        }
+    }
+    for (int p = 0; p < B; ++p) {
+      double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
+      double hxx = 0, hxy = 0, hxz = 0, hyy = 0, hyz = 0, hzz = 0;
+      for (int j = 0; j < m; ++j) {
+        const size_t o = (size_t)sig[j] * B + p;
+        const double t = BP[(size_t)j * B + p];
+        r += t * ao.val[o];
+        if (grad || hess) {
+          x += t * ao.gx[o];
+          y += t * ao.gy[o];
+          z += t * ao.gz[o];
+        }
+        if (hess) {
+          const size_t q = (size_t)j * B + p;
+          hxx += t * ao.hxx[o] + GX[q] * ao.gx[o];
+          hxy += t * ao.hxy[o] + GX[q] * ao.gy[o];
+          hxz += t * ao.hxz[o] + GX[q] * ao.gz[o];
+          hyy += t * ao.hyy[o] + GY[q] * ao.gy[o];
+          hyz += t * ao.hyz[o] + GY[q] * ao.gz[o];
+          hzz += t * ao.hzz[o] + GZ[q] * ao.gz[o];
+        }
+      }
+      rho[first + p] = r;
+      if (grad) {
+        grad[first + p] = 2.0 * x;
+        grad[N + first + p] = 2.0 * y;
+        grad[2 * N + first + p] = 2.0 * z;
+      }
+      if (hess) {
+        const double hv[6] = {hxx, hxy, hxz, hyy, hyz, hzz};
+        for (int d = 0; d < 6; ++d) hess[d * N + first + p] = 2.0 * hv[d];
+      }
+    }
+  }
+  return 0;
+}
+
+// Functional on N points in blocks of bs: eps, v, g (3 x N, nullable), energy.
+// params: NULL or {vwn_A, pw_A, kappa, mu, beta}.
+int oracle_functional(int func, int64_t N, int bs, const double* rho, const double* grad, const double* w,
+                      const double* params, double* eps, double* v, double* g, double* energy) {
+  const FParams P = params_from(params);
+  const bool gga = is_gga(func);
+  const int64_t nb = (N + bs - 1) / bs;
+  std::vector<double> eb(nb, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t first = b * bs;
+    const int B = (int)std::min<int64_t>(bs, N - first);
+    double s = 0.0;
+    for (int p = 0; p < B; ++p) s += std::fabs(rho[first + p]);
+    const bool skip = s < B * 1e-12;
+    double e_acc = 0.0;
+    for (int p = 0; p < B; ++p) {
+      const int64_t q = first + p;
+      double e = 0.0, vv = 0.0, gg[3] = {0, 0, 0};
+      const double r = rho[q];
+      if (!skip && r >= 1e-14) {
+        double gx = 0, gy = 0, gz = 0;
+        if (gga) {
+          gx = grad[q];
+          gy = grad[N + q];
+          gz = grad[2 * N + q];
+        }
+        const EV ev = eval(func, r, gx * gx + gy * gy + gz * gz, P);
+        e = ev.e;
+        vv = ev.dr;
+        if (gga) {
+          gg[0] = 2.0 * ev.ds * gx;
+          gg[1] = 2.0 * ev.ds * gy;
+          gg[2] = 2.0 * ev.ds * gz;
+        }
+      }
+      if (eps) eps[q] = e;
+      if (v) v[q] = vv;
+      if (g) {
+        g[q] = gg[0];
+        g[N + q] = gg[1];
+        g[2 * N + q] = gg[2];
+      }
+      e_acc += w[q] * e;
+    }
+    eb[b] = e_acc;
+  }
+  double E = 0.0;
+  for (int64_t b = 0; b < nb; ++b) E += eb[b];
+  if (energy) *energy = E;
+  return 0;
+}
+
+// F += sum_b phi_b^T G_b + G_b^T phi_b (addBlock semantics); v: N, g: 3 x N or NULL.
+// Blocks [block_begin, block_end) only (block_end < 0: all).
+int oracle_fock_add(int nshell, const int* l, const int* pure, const int* nprim, const double* center,
+                    const double* alpha, const double* coef, const double* pts, const double* w, int64_t N, int bs,
+                    double thr, double block_ave_thr, const double* v, const double* g, double* F,
+                    int64_t block_begin, int64_t block_end) {
+  Basis bas(nshell, l, pure, nprim, center, alpha, coef);
+  const int n = bas.nao;
+  const int64_t nb = (N + bs - 1) / bs;
+  if (block_end < 0 || block_end > nb) block_end = nb;
+  const int nthreads = omp_get_max_threads();
+  std::vector<std::vector<double>> Ft(nthreads, std::vector<double>((size_t)n * n, 0.0));
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t b = block_begin; b < block_end; ++b) {
+    auto& Fm = Ft[omp_get_thread_num()];
+    const int64_t first = b * bs;
+    const int B = (int)std::min<int64_t>(bs, N - first);
+    std::vector<double> s(B), gw[3];
+    double avg = 0.0;
+    for (int p = 0; p < B; ++p) {
+      s[p] = v[first + p] * w[first + p];
+      avg += std::fabs(s[p]);
+    }
+    if (g) {
+      for (int d = 0; d < 3; ++d) {
+        gw[d].resize(B);
+        for (int p = 0; p < B; ++p) {
+          gw[d][p] = w[first + p] * g[d * N + first + p];
+          avg += std::fabs(gw[d][p]);
+        }
+      }
+    }
+    if (avg / B < block_ave_thr) continue;
+    // the reference fetches the block twice (controllers A and B), recomputing the AOs
+    BlockAO ao, aoB;
+    ao_block(bas, pts + 3 * first, B, thr, g ? 1 : 0, ao);
+    ao_block(bas, pts + 3 * first, B, thr, g ? 1 : 0, aoB);
+    std::vector<int> sig;
+    for (int i = 0; i < n; ++i)
+      if (!ao.neg[i]) sig.push_back(i);
+    const int m = (int)sig.size();
+    if (m == 0) continue;
+    // G (B x m)
+    std::vector<double> G((size_t)B * m);
+    for (int j = 0; j < m; ++j)
+      for (int p = 0; p < B; ++p) {
+        const size_t o = (size_t)sig[j] * B + p;
+        double gv = 0.5 * s[p] * ao.val[o];
+        if (g) gv += gw[0][p] * ao.gx[o] + gw[1][p] * ao.gy[o] + gw[2][p] * ao.gz[o];
+        G[(size_t)j * B + p] = gv;
+      }
+    // T = phi^T G (m x m); F[sig,sig] += T + T^T
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) {
+        const double* pi = &aoB.val[(size_t)sig[i] * B];
+        const double* gj = &G[(size_t)j * B];
+        double t = 0.0;
+        for (int p = 0; p < B; ++p) t += pi[p] * gj[p];
+        Fm[sig[i] + (size_t)n * sig[j]] += t;
+        Fm[sig[j] + (size_t)n * sig[i]] += t;
+      }
+  }
+  for (int t = 0; t < nthreads; ++t)
+    for (size_t e = 0; e < (size_t)n * n; ++e) F[e] += Ft[t][e];
+  return 0;
+}
+
+// FuncPotential::getMatrix/getEnergy: F = V_xc (overwritten), energy.
+// Optional outputs rho/grad (N, 3N) for inspection.
+int oracle_fock_build(int nshell, const int* l, const int* pure, const int* nprim, const double* center,
+                      const double* alpha, const double* coef, const double* pts, const double* w, int64_t N, int bs,
+                      double thr, double block_ave_thr, int func, const double* params, const double* P, double* F,
+                      double* energy, double* rho_out, double* grad_out) {
+  const bool gga = is_gga(func);
+  std::vector<double> rho(N), grad(gga ? 3 * N : 0), v(N), g(gga ? 3 * N : 0);
+  oracle_density(nshell, l, pure, nprim, center, alpha, coef, pts, N, bs, thr, P, rho.data(),
+                 gga ? grad.data() : nullptr, nullptr, nullptr);
+  oracle_functional(func, N, 128, rho.data(), gga ? grad.data() : nullptr, w, params, nullptr, v.data(),
+                    gga ? g.data() : nullptr, energy);
+  Basis bas(nshell, l, pure, nprim, center, alpha, coef);
+  std::fill(F, F + (size_t)bas.nao * bas.nao, 0.0);
+  oracle_fock_add(nshell, l, pure, nprim, center, alpha, coef, pts, w, N, bs, thr, block_ave_thr, v.data(),
+                  gga ? g.data() : nullptr, F, 0, -1);
+  if (rho_out) std::copy(rho.begin(), rho.end(), rho_out);
+  if (grad_out && gga) std::copy(grad.begin(), grad.end(), grad_out);
+  return 0;
+}
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

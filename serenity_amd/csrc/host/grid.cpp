@@ -1,0 +1,454 @@
+// Molecular integration grid: Lebedev-Laikov angular rules, radial
+// quadratures, fuzzy-cell (Becke / SSF / Voronoi) partitioning, weight cut and
+// Hilbert-curve ordering.  A restatement of Serenity's grid construction
+// (src/grid/construction/AtomGridFactory.cpp:78-290, GridFactory.cpp:51-347,
+// src/grid/HilbertRTreeSorting.cpp:27-216) written for this library.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+
+#include "host.h"
+
+namespace sxc {
+
+namespace {
+
+struct LebedevOrbit {
+  int code;
+  double a, b, v;
+};
+#include "lebedev_generators.inc"
+
+// Per-element parameters, index = nuclear charge (entry 0 is a dummy).
+// Ahlrichs radial scaling (AtomGridFactory.cpp:39-52, Treutler & Ahlrichs,
+// JCP 102 (1995) 346).
+const double kAhlrichsAlpha[] = {1.1, 0.8, 0.9, 1.8, 1.4, 1.3, 1.1, 0.9, 0.9, 0.9, 0.9, 1.4, 1.3,
+                                 1.3, 1.2, 1.1, 1.0, 1.0, 1.0, 1.5, 1.4, 1.3, 1.2, 1.2, 1.2, 1.2,
+                                 1.2, 1.2, 1.1, 1.1, 1.1, 1.1, 1.0, 0.9, 0.9, 0.9, 0.9};
+// Clementi radii used for the angular pruning zones (AtomGridFactory.cpp:53-66).
+const double kClementiRadii[] = {1.00, 1.00, 0.59, 3.16, 2.12, 1.64, 1.27, 1.06, 0.91, 0.79, 0.72, 3.59, 2.74,
+                                 2.23, 2.10, 1.85, 1.66, 1.49, 1.34, 4.59, 3.67, 3.48, 3.33, 3.23, 3.14, 3.04,
+                                 2.95, 2.87, 2.82, 2.74, 2.68, 2.57, 2.36, 2.15, 1.95, 1.78, 1.66};
+// Bragg-Slater radii in Angstrom (src/parameters/AtomicParameters.h:454-470;
+// hydrogen 0.25 as in src/geometry/AtomTypeFactory.cpp:105-110).
+const double kBraggSlaterAngstrom[] = {-1.0, 0.25, 1.20, 1.45, 1.05, 0.85, 0.70, 0.65, 0.60, 0.50, 0.38, 1.80, 1.50,
+                                       1.25, 1.10, 1.00, 1.00, 1.00, 0.71, 2.20, 1.80, 1.60, 1.40, 1.35, 1.40, 1.40,
+                                       1.40, 1.35, 1.35, 1.35, 1.35, 1.30, 1.25, 1.15, 1.15, 1.15, 0.88};
+constexpr int kMaxZ = 36;
+
+// Bohr radius in Angstrom as Serenity defines it (src/parameters/Constants.h:63,79).
+constexpr double kBohrToAngstrom = 5.29177210544E-011 * 1.0E+010;
+constexpr double kAngstromToBohr = 1.0 / kBohrToAngstrom;
+
+int row_of(int Z) {
+  if (Z < 3) return 1;
+  if (Z < 11) return 2;
+  if (Z < 19) return 3;
+  if (Z < 37) return 4;
+  if (Z < 55) return 5;
+  if (Z < 87) return 6;
+  return 7;
+}
+
+void check_Z(int Z) {
+  if (Z < 1 || Z > kMaxZ) throw std::runtime_error("grid: element Z=" + std::to_string(Z) + " not tabulated");
+}
+
+// Expand one octahedral orbit (code 1..6) of a Lebedev rule.
+void expand_orbit(const LebedevOrbit& o, std::vector<double>& x, std::vector<double>& y, std::vector<double>& z,
+                  std::vector<double>& w) {
+  auto put = [&](double px, double py, double pz) {
+    x.push_back(px);
+    y.push_back(py);
+    z.push_back(pz);
+    w.push_back(o.v);
+  };
+  auto signs3 = [&](double p, double q, double r) {
+    // all sign combinations of the non-zero components of (p, q, r); the sign
+    // of the first component alternates fastest (the published generator's order)
+    const double sp[2] = {p, -p}, sq[2] = {q, -q}, sr[2] = {r, -r};
+    const int np = (p == 0.0) ? 1 : 2, nq = (q == 0.0) ? 1 : 2, nr = (r == 0.0) ? 1 : 2;
+    for (int k = 0; k < nr; ++k)
+      for (int j = 0; j < nq; ++j)
+        for (int i = 0; i < np; ++i) put(sp[i], sq[j], sr[k]);
+  };
+  switch (o.code) {
+    case 1:  // (1,0,0) and permutations: 6 points
+      signs3(1.0, 0.0, 0.0);
+      signs3(0.0, 1.0, 0.0);
+      signs3(0.0, 0.0, 1.0);
+      break;
+    case 2: {  // (0,a,a), a = 1/sqrt(2): 12 points
+      const double a = std::sqrt(0.5);
+      signs3(0.0, a, a);
+      signs3(a, 0.0, a);
+      signs3(a, a, 0.0);
+    } break;
+    case 3: {  // (a,a,a), a = 1/sqrt(3): 8 points
+      const double a = std::sqrt(1.0 / 3.0);
+      signs3(a, a, a);
+    } break;
+    case 4: {  // (a,a,b), b = sqrt(1-2a^2): 24 points
+      const double a = o.a, b = std::sqrt(1.0 - 2.0 * a * a);
+      signs3(a, a, b);
+      signs3(a, b, a);
+      signs3(b, a, a);
+    } break;
+    case 5: {  // (a,b,0), b = sqrt(1-a^2): 24 points
+      const double a = o.a, b = std::sqrt(1.0 - a * a);
+      signs3(a, b, 0.0);
+      signs3(b, a, 0.0);
+      signs3(a, 0.0, b);
+      signs3(b, 0.0, a);
+      signs3(0.0, a, b);
+      signs3(0.0, b, a);
+    } break;
+    case 6: {  // (a,b,c), c = sqrt(1-a^2-b^2): 48 points
+      const double a = o.a, b = o.b, c = std::sqrt(1.0 - a * a - b * b);
+      signs3(a, b, c);
+      signs3(a, c, b);
+      signs3(b, a, c);
+      signs3(b, c, a);
+      signs3(c, a, b);
+      signs3(c, b, a);
+    } break;
+    default:
+      throw std::runtime_error("lebedev: bad orbit code");
+  }
+}
+
+void radial_grid(RadialType type, double alpha, int n, std::vector<double>& r, std::vector<double>& w) {
+  r.assign(n, 0.0);
+  w.assign(n, 0.0);
+  switch (type) {
+    case RadialType::BECKE:
+      for (int i = 1; i <= n; ++i) {
+        const double xi = std::cos((double)i * M_PI / (n + 1));
+        w[n - i] = std::sqrt(std::pow(1.0 + xi, 5) / std::pow(1.0 - xi, 7)) * (2.0 * M_PI) * std::pow(alpha, 3) / (n + 1);
+        r[n - i] = alpha * (1.0 + xi) / (1.0 - xi);
+      }
+      break;
+    case RadialType::HANDY:
+      for (int i = 1; i <= n; ++i) {
+        const double xi = i / (n + 1.0);
+        r[i - 1] = alpha * xi * xi / ((1 - xi) * (1 - xi));
+        w[i - 1] = 2 * std::pow(alpha, 3) * std::pow(xi, 5) / (n + 1) / std::pow((1 - xi), 7);
+      }
+      break;
+    case RadialType::AHLRICHS:
+      // Treutler-Ahlrichs M3 mapping with exponent 0.6 on Chebyshev nodes of the
+      // second kind; stored with increasing radius.
+      for (int i = 1; i <= n; ++i) {
+        const double log2 = std::log(2.0);
+        const double tmp = alpha / log2;
+        const double xi = std::cos(i * M_PI / (n + 1.0));
+        const double ri = tmp * std::pow((xi + 1.0), 0.6) * std::log(2.0 / (1.0 - xi));
+        const double ln = std::log((1.0 - xi) / 2.0);
+        const double sq = std::sqrt((1.0 + xi) / (1.0 - xi));
+        const double wi = (M_PI / (n + 1.0)) * std::pow((1.0 + xi), (3 * 0.6)) * tmp * tmp * tmp *
+                          (sq * ln * ln - 0.6 * ln * ln * ln / sq);
+        w[n - i] = wi;
+        r[n - i] = ri;
+      }
+      break;
+    case RadialType::KNOWLES:
+      for (int i = 1; i <= n; ++i) {
+        const double xi = i / (n + 1.0);
+        const double tmp = 1 - xi * xi * xi;
+        const double ln = std::log(tmp);
+        r[n - i] = -alpha * ln;
+        w[n - i] = 3 * xi * xi * ln * ln * alpha * alpha * alpha / ((n + 1) * tmp);
+      }
+      break;
+    case RadialType::EQUI:
+      for (int i = 1; i <= n; ++i) {
+        r[i - 1] = i * 5.0 / n;
+        w[i - 1] = 1.0 / n;
+      }
+      break;
+  }
+}
+
+// Reference grid of one element (not shifted to the atom), AtomGridFactory.cpp:78-203.
+void atom_grid(int Z, const GridSettings& s, std::vector<double>& pts, std::vector<double>& wts) {
+  check_Z(Z);
+  if (s.accuracy < 1 || s.accuracy > 7) throw std::runtime_error("grid accuracy must be in [1,7]");
+  static const int radAcc[7] = {13, 13, 13, 14, 15, 16, 17};
+  const int row = row_of(Z);
+  const int nRadial = (int)(5.0 * (radAcc[s.accuracy - 1] + row - 8));
+  const double bsr = kBraggSlaterAngstrom[Z] * kAngstromToBohr;
+  double alpha = 0.0;
+  switch (s.radial) {
+    case RadialType::BECKE:
+    case RadialType::HANDY:
+      alpha = (Z == 1) ? bsr : 0.5 * bsr;
+      break;
+    case RadialType::AHLRICHS:
+      alpha = kAhlrichsAlpha[Z];
+      break;
+    case RadialType::KNOWLES:
+      alpha = (Z == 1 || Z == 2 || Z == 3 || Z == 4 || Z == 11 || Z == 12 || Z == 19 || Z == 20) ? 5.0 : 7.0;
+      break;
+    case RadialType::EQUI:
+      alpha = 0.0;
+      break;
+  }
+  std::vector<double> rr, rw;
+  radial_grid(s.radial, alpha, nRadial, rr, rw);
+
+  // Angular pruning zones (ORCA-style, ldval/ranges tables of AtomGridFactory.cpp:156-180).
+  static const int ldval[7][5] = {{4, 4, 4, 4, 4},     {4, 4, 4, 7, 4},      {4, 4, 7, 10, 7},     {4, 7, 10, 13, 10},
+                                  {7, 10, 13, 15, 13}, {10, 13, 15, 16, 15}, {13, 15, 16, 17, 16}};
+  static const double ranges[3][4] = {{0.25, 0.5, 1.0, 4.5}, {0.1667, 0.5, 0.9, 3.5}, {0.1, 0.4, 0.8, 2.5}};
+  const int rrow = (row > 3) ? 2 : row - 1;
+  const int redp1 = (row == 1 && s.accuracy > 1) ? 2 : 1;
+  double zone[4];
+  for (int i = 0; i < 4; ++i) zone[i] = ranges[rrow][i] * kClementiRadii[Z];
+  int sphAcc = 0;
+  pts.clear();
+  wts.clear();
+  std::vector<double> sx, sy, sz, sw;
+  for (int i = 0; i < nRadial; ++i) {
+    if (sphAcc < 4)
+      if (rr[i] > zone[sphAcc]) ++sphAcc;
+    lebedev_rule(lebedev_points_of_index(ldval[s.accuracy - redp1][sphAcc]), sx, sy, sz, sw);
+    for (size_t k = 0; k < sx.size(); ++k) {
+      wts.push_back(sw[k] * rw[i] * 4.0 * M_PI);
+      pts.push_back(sx[k] * rr[i]);
+      pts.push_back(sy[k] * rr[i]);
+      pts.push_back(sz[k] * rr[i]);
+    }
+  }
+}
+
+constexpr double becke_p(double nu, unsigned k) {
+  return (k > 1) ? becke_p((3.0 - nu * nu) * nu / 2.0, k - 1) : (3.0 - nu * nu) * nu / 2.0;
+}
+constexpr double becke_s(double nu, unsigned k) { return 0.5 * (1 - becke_p(nu, k)); }
+
+double dist3(const double* a, const double* b) {
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+}  // namespace
+
+int lebedev_points_of_index(int index) {
+  if (index < 0 || index >= kLebedevNumRules) throw std::runtime_error("lebedev index out of range");
+  return kLebedevRulePoints[index];
+}
+
+void lebedev_rule(int npoints, std::vector<double>& x, std::vector<double>& y, std::vector<double>& z,
+                  std::vector<double>& w) {
+  x.clear();
+  y.clear();
+  z.clear();
+  w.clear();
+  int first = 0;
+  for (int r = 0; r < kLebedevNumRules; ++r) {
+    if (kLebedevRulePoints[r] == npoints) {
+      for (int o = 0; o < kLebedevRuleOrbits[r]; ++o) expand_orbit(kLebedevOrbits[first + o], x, y, z, w);
+      if ((int)x.size() != npoints) throw std::runtime_error("lebedev: orbit expansion size mismatch");
+      return;
+    }
+    first += kLebedevRuleOrbits[r];
+  }
+  throw std::runtime_error("lebedev: no rule with " + std::to_string(npoints) + " points");
+}
+
+void hilbert_sort(std::vector<double>& points, std::vector<double>& weights) {
+  const int64_t n = (int64_t)weights.size();
+  if (n < 2) return;
+  int64_t length = 2, nvert = 8;
+  while (n * 8 > nvert) {
+    length *= 2;
+    nvert *= 8;
+  }
+  double mn[3], spread[3];
+  for (int d = 0; d < 3; ++d) {
+    double lo = std::numeric_limits<double>::infinity(), hi = -lo;
+    for (int64_t i = 0; i < n; ++i) {
+      lo = std::min(lo, points[3 * i + d]);
+      hi = std::max(hi, points[3 * i + d]);
+    }
+    mn[d] = lo;
+    spread[d] = double(length) / (hi - lo);
+  }
+  // Child-cube numbering and the transition table between refinement levels.
+  static const int trans[8][8] = {{0, 7, 6, 1, 2, 5, 4, 3}, {0, 3, 4, 6, 7, 5, 2, 1}, {0, 3, 4, 6, 7, 5, 2, 1},
+                                  {2, 3, 0, 1, 6, 7, 4, 5}, {2, 3, 0, 1, 6, 7, 4, 5}, {6, 5, 2, 1, 0, 3, 4, 7},
+                                  {6, 5, 2, 1, 0, 3, 4, 7}, {4, 3, 2, 5, 6, 1, 0, 7}};
+  static const int val[2][2][2] = {{{5, 6}, {4, 7}}, {{2, 1}, {3, 0}}};
+  std::vector<int64_t> key(n);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t pt[3];
+    for (int d = 0; d < 3; ++d) pt[d] = (int64_t)(int)((points[3 * i + d] - mn[d]) * spread[d]);
+    int64_t l = length / 2;
+    int gx = pt[0] > l, gy = pt[1] > l, gz = pt[2] > l;
+    int v = val[gx][gy][gz];
+    int64_t idx = v;
+    while (l > 1) {
+      idx *= 8;
+      if (gx) pt[0] -= l;
+      if (gy) pt[1] -= l;
+      if (gz) pt[2] -= l;
+      l /= 2;
+      gx = pt[0] > l;
+      gy = pt[1] > l;
+      gz = pt[2] > l;
+      v = trans[v][val[gx][gy][gz]];
+      idx += v;
+    }
+    key[i] = idx;
+  }
+  // Descending key order; equal keys keep their input order (the order a
+  // single-threaded run of the reference's selection/merge sort produces).
+  std::vector<int64_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] > key[b]; });
+  std::vector<double> p2(points.size()), w2(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t s = order[i];
+    p2[3 * i + 0] = points[3 * s + 0];
+    p2[3 * i + 1] = points[3 * s + 1];
+    p2[3 * i + 2] = points[3 * s + 2];
+    w2[i] = weights[s];
+  }
+  points.swap(p2);
+  weights.swap(w2);
+}
+
+MolecularGrid build_molecular_grid(const std::vector<int>& Z, const std::vector<double>& xyz, const GridSettings& s) {
+  const int nAtoms = (int)Z.size();
+  if (nAtoms == 0) throw std::runtime_error("grid: empty geometry");
+  if ((int)xyz.size() != 3 * nAtoms) throw std::runtime_error("grid: coordinate array size mismatch");
+  for (int i = 0; i < nAtoms; ++i) check_Z(Z[i]);
+  for (int i = 0; i < nAtoms; ++i)
+    for (int j = 0; j < i; ++j)
+      if (xyz[3 * i] == xyz[3 * j] && xyz[3 * i + 1] == xyz[3 * j + 1] && xyz[3 * i + 2] == xyz[3 * j + 2])
+        throw std::runtime_error("grid construction with duplicated atoms");
+
+  std::vector<double> dist((size_t)nAtoms * nAtoms);
+  for (int i = 0; i < nAtoms; ++i)
+    for (int j = 0; j <= i; ++j) dist[i + (size_t)nAtoms * j] = dist[j + (size_t)nAtoms * i] = dist3(&xyz[3 * i], &xyz[3 * j]);
+
+  // Becke atomic size adjustment (Becke, JCP 88 (1988) 2547, appendix).
+  std::vector<double> aPair((size_t)nAtoms * nAtoms, 0.0);
+  if (s.type == GridType::BECKE) {
+    for (int i = 0; i < nAtoms; ++i)
+      for (int j = 0; j < nAtoms; ++j) {
+        const double q = std::sqrt(kBraggSlaterAngstrom[Z[j]] * kAngstromToBohr / (kBraggSlaterAngstrom[Z[i]] * kAngstromToBohr));
+        const double u = (q - 1) / (q + 1);
+        double a = u / (u * u - 1);
+        a = std::min(0.5, std::max(-0.5, a));
+        aPair[j + (size_t)nAtoms * i] = a;
+      }
+  }
+
+  // Reference grid per element.
+  std::vector<std::vector<double>> refPts(kMaxZ + 1), refW(kMaxZ + 1);
+  for (int k = 0; k < nAtoms; ++k)
+    if (refW[Z[k]].empty()) atom_grid(Z[k], s, refPts[Z[k]], refW[Z[k]]);
+
+  std::vector<std::vector<double>> atomPts(nAtoms), atomW(nAtoms);
+  int64_t nBefore = 0;
+  for (int k = 0; k < nAtoms; ++k) nBefore += (int64_t)refW[Z[k]].size();
+
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nAtoms; ++k) {
+    std::vector<int> significant;
+    double minAtomDist = 999999999.9;
+    for (int l = 0; l < nAtoms; ++l) {
+      if (dist[l + (size_t)nAtoms * k] < 40.0) significant.push_back(l);
+      if (l != k) minAtomDist = std::min(minAtomDist, dist[l + (size_t)nAtoms * k]);
+    }
+    const auto& rp = refPts[Z[k]];
+    const auto& rw = refW[Z[k]];
+    auto& outP = atomPts[k];
+    auto& outW = atomW[k];
+    outP.reserve(rp.size());
+    outW.reserve(rw.size());
+    std::vector<double> rdist(nAtoms);
+    for (size_t i = 0; i < rw.size(); ++i) {
+      const double gp[3] = {rp[3 * i] + xyz[3 * k], rp[3 * i + 1] + xyz[3 * k + 1], rp[3 * i + 2] + xyz[3 * k + 2]};
+      double weight = rw[i];
+      double cellSum = 0.0;
+      for (int a = 0; a < nAtoms; ++a) rdist[a] = dist3(&xyz[3 * a], gp);
+      if (s.type == GridType::BECKE) {
+        for (int l : significant) {
+          double cell = 1.0;
+          for (int j : significant) {
+            if (l == j) continue;
+            const double mu = (rdist[l] - rdist[j]) / dist[l + (size_t)nAtoms * j];
+            const double nu = mu + aPair[j + (size_t)nAtoms * l] * (1.0 - mu * mu);
+            cell *= becke_s(nu, s.smoothing);
+          }
+          if (l == k) weight *= cell;
+          cellSum += cell;
+        }
+        weight /= cellSum;
+      } else if (s.type == GridType::VORONOI) {
+        for (int l : significant) {
+          const double mu = (rdist[l] - rdist[k]) / dist[l + (size_t)nAtoms * k];
+          const double nu = mu + aPair[k + (size_t)nAtoms * l] * (1.0 - mu * mu);
+          if (nu < 0.0) {
+            weight = 0.0;
+            break;
+          }
+        }
+      } else {
+        // Stratmann-Scuseria-Frisch partition (CPL 257 (1996) 213), a = 0.64.
+        if (rdist[k] >= 0.5 * (1.0 - 0.64) * minAtomDist) {
+          bool zero = false;
+          for (int j = 0; j < nAtoms; ++j) {
+            const double nu = (rdist[k] - rdist[j]) / dist[k + (size_t)nAtoms * j];
+            if (nu >= 0.64) {
+              zero = true;
+              break;
+            }
+          }
+          if (zero) continue;
+          for (int a = 0; a < nAtoms; ++a) {
+            double cell = 1.0;
+            for (int j = 0; j < nAtoms; ++j) {
+              if (a == j) continue;
+              double nu = (rdist[a] - rdist[j]) / dist[a + (size_t)nAtoms * j];
+              if (nu <= -0.64) continue;
+              if (nu >= 0.64) {
+                cell = 0.0;
+                break;
+              }
+              nu /= 0.64;
+              const double n3 = nu * nu * nu;
+              const double poly = (-5.0 * n3 * n3 * nu + 21.0 * n3 * nu * nu - 35.0 * n3 + 35.0 * nu) / 16.0;
+              cell *= 0.5 * (1 - poly);
+            }
+            if (a == k) weight *= cell;
+            cellSum += cell;
+          }
+          weight /= cellSum;
+        }
+      }
+      if (weight > s.weight_threshold) {
+        outW.push_back(weight);
+        outP.push_back(gp[0]);
+        outP.push_back(gp[1]);
+        outP.push_back(gp[2]);
+      }
+    }
+  }
+
+  MolecularGrid g;
+  g.n_before_cut = nBefore;
+  for (int k = 0; k < nAtoms; ++k) {
+    g.points.insert(g.points.end(), atomPts[k].begin(), atomPts[k].end());
+    g.weights.insert(g.weights.end(), atomW[k].begin(), atomW[k].end());
+  }
+  if (s.sort) hilbert_sort(g.points, g.weights);
+  return g;
+}
+
+}  // namespace sxc

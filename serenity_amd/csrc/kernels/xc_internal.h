@@ -1,0 +1,71 @@
+// Device-side data layout of the XC Fock build and the kernel launchers shared
+// by the C ABI (xc_api.hip) and the kernels (xc_kernels.hip).
+//
+// HBM layout of the basis-functions-on-grid tiles: the grid is cut into
+// blocks of at most 128 consecutive (Hilbert-ordered) points.  Block b keeps
+// the n_b AOs that survive Serenity's shell screening, padded to npad (a
+// multiple of 32) and stored as ncomp component planes
+//   tiles[tile_off + (c * npad + i) * 128 + p],  c = value, d/dx, d/dy, d/dz,
+// i = compact AO index (ao_idx maps it to the global AO, -1 for padding),
+// p = point within the block (rows beyond the block's size are zero).  Point
+// is the fastest index so a wavefront reads 64 consecutive values.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "functional.h"
+
+namespace sxc {
+
+constexpr int kTile = 128;      // points per tile row (max block size)
+constexpr int kAoPad = 32;      // AO padding granularity of a block
+constexpr int kMaxL = 6;        // highest angular momentum the AO kernel evaluates
+
+struct DevShell {
+  double cx, cy, cz;
+  int l, pure, nprim, prim_off;
+  int ao_off, nfunc, ang_off, ncart;
+};
+
+struct DevBlock {
+  int64_t tile_off;   // element offset of the block's tiles
+  int64_t first;      // first grid point
+  int npts;           // points in the block (<= 128)
+  int nb;             // non-negligible AOs
+  int npad;           // nb rounded up to kAoPad
+  int idx_off;        // offset into ao_idx
+  int shell_off;      // offset into the significant-shell list
+  int nshell;         // significant shells
+};
+
+// ---- launchers (xc_kernels.hip) -------------------------------------------
+void launch_screen(hipStream_t s, const double* pts, int64_t npts, int bs, int nblocks, const DevShell* shells,
+                   int nshell, const double* alpha, const double* coef, double radial_thr, unsigned char* flags);
+
+void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const int* sig_shell, const int* sig_pos,
+                    const DevShell* shells, const double* alpha, const double* coef, const double* ang,
+                    const double* pts, double radial_thr, double exp_thr, int ncomp, double* tiles64, float* tiles32);
+
+template <typename T>
+void launch_density(hipStream_t s, const DevBlock* blocks, int nblocks, const int* ao_idx, const T* tiles,
+                    int ncomp, const T* P, int n, int want_grad, double* rho, double* grad, int64_t npts);
+
+void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* rho,
+                       const double* grad, const double* w, double* eps, double* v, double* g, double* vs,
+                       double* gw, double* e_blk);
+
+template <typename T>
+void launch_fock(hipStream_t s, const DevBlock* blocks, int nblocks, const int* ao_idx, const T* tiles, int ncomp,
+                 const double* vs, const double* gw, int gga, double block_ave_thr, double* U, int n, int64_t npts);
+
+void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
+                             double* vs, double* gw);
+void launch_fill(hipStream_t s, double* p, int64_t n, double value);
+void launch_to_float(hipStream_t s, const double* src, float* dst, int64_t n);
+// F[i + n j] (+)= U[min(i,j) + n max(i,j)]
+void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accumulate);
+void launch_sum(hipStream_t s, const double* x, int64_t n, double* out);
+
+}  // namespace sxc

@@ -1,0 +1,827 @@
+// C ABI of the XC Fock build (include/serenity_xc.h): resident handles, the
+// screening/AO setup of a basis-functions-on-grid controller, and the
+// density -> functional -> Fock pipeline on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/serenity_xc.h"
+#include "host/host.h"
+#include "kernels/xc_internal.h"
+
+using namespace sxc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct XcError : std::runtime_error {
+  int code;
+  XcError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPX(expr)                                                                                    \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess)                                                                             \
+      throw XcError(XC_ERR_DEVICE, std::string(#expr) + " failed: " + hipGetErrorString(e_));         \
+  } while (0)
+
+void require(bool cond, int code, const std::string& msg) {
+  if (!cond) throw XcError(code, msg);
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return XC_OK;
+  } catch (const XcError& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return XC_ERR_INPUT;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return XC_ERR_STATE;
+  }
+}
+
+}  // namespace
+
+struct xc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::recursive_mutex mu;
+  int64_t bytes = 0;
+  double last_energy = 0.0;
+  double* d_energy = nullptr;
+  XcParams prm;
+  std::map<void*, size_t> user_allocs;  // buffers handed out by xc_device_alloc
+  // stage timing: XC_NSTAGES + 1 events per profiled build
+  bool profile = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      HIPX(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  void mark() {
+    if (profile) HIPX(hipEventRecord(next_event(), stream));
+  }
+
+  void* alloc(size_t bytes_) {
+    if (bytes_ == 0) return nullptr;
+    void* p = nullptr;
+    HIPX(hipSetDevice(device));
+    hipError_t e = hipMalloc(&p, bytes_);
+    if (e != hipSuccess) throw XcError(XC_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes_) + " bytes failed");
+    bytes += (int64_t)bytes_;
+    return p;
+  }
+  void release(void* p, size_t bytes_) {
+    if (!p) return;
+    (void)hipFree(p);
+    bytes -= (int64_t)bytes_;
+  }
+};
+
+namespace {
+
+// Device allocation owned by a handle.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  xc_ctx* ctx = nullptr;
+  void alloc(xc_ctx* c, size_t count) {
+    free_();
+    ctx = c;
+    n = count;
+    p = (T*)c->alloc(count * sizeof(T));
+  }
+  void upload(xc_ctx* c, const T* host, size_t count) {
+    alloc(c, count);
+    if (count) HIPX(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  }
+  void free_() {
+    if (p && ctx) ctx->release(p, n * sizeof(T));
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { free_(); }
+};
+
+}  // namespace
+
+struct xc_basis {
+  xc_ctx* ctx = nullptr;
+  int nshell = 0, nao = 0, maxl = 0;
+  std::vector<DevShell> shells;
+  std::vector<double> alpha, coef, ang;
+  DBuf<DevShell> d_shells;
+  DBuf<double> d_alpha, d_coef, d_ang;
+};
+
+struct xc_grid {
+  xc_ctx* ctx = nullptr;
+  int64_t npts = 0;
+  DBuf<double> d_pts, d_w;
+};
+
+struct xc_bfogc {
+  xc_ctx* ctx = nullptr;
+  xc_basis* basis = nullptr;
+  xc_grid* grid = nullptr;
+  int bs = 128;
+  double radial_thr = 1e-9;
+  int deriv = 1, ncomp = 4, nblocks = 0;
+  std::vector<DevBlock> blocks;
+  std::vector<int> ao_idx;
+  int64_t tile_elems = 0;
+  int64_t sum_nb = 0;
+  double sum_B_nb2 = 0.0;
+  DBuf<DevBlock> d_blocks;
+  DBuf<int> d_ao_idx, d_sig_shell, d_sig_pos;
+  DBuf<double> d_t64;
+  DBuf<float> d_t32;
+  // per-build work buffers
+  DBuf<double> d_rho, d_grad, d_vs, d_gw, d_eblk, d_U, d_P, d_F, d_v, d_g, d_E;
+  DBuf<float> d_P32;
+};
+
+extern "C" {
+
+const char* xc_last_error(void) { return g_last_error.c_str(); }
+
+int xc_version(void) { return 1; }
+
+int xc_ctx_create(int device, xc_ctx** out) {
+  return guarded([&] {
+    require(out != nullptr, XC_ERR_ARG, "xc_ctx_create: null output");
+    int ndev = 0;
+    HIPX(hipGetDeviceCount(&ndev));
+    require(device >= 0 && device < ndev, XC_ERR_ARG, "xc_ctx_create: no HIP device " + std::to_string(device));
+    auto* c = new xc_ctx();
+    c->device = device;
+    HIPX(hipSetDevice(device));
+    HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->d_energy = (double*)c->alloc(sizeof(double));
+    *out = c;
+  });
+}
+
+int xc_ctx_destroy(xc_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    ctx->release(ctx->d_energy, sizeof(double));
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int xc_ctx_synchronize(xc_ctx* ctx) {
+  return guarded([&] {
+    require(ctx, XC_ERR_ARG, "null context");
+    HIPX(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+void* xc_ctx_stream(xc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int xc_ctx_memory(xc_ctx* ctx, int64_t* bytes_in_use) {
+  return guarded([&] {
+    require(ctx && bytes_in_use, XC_ERR_ARG, "null argument");
+    *bytes_in_use = ctx->bytes;
+  });
+}
+
+void xc_free_host(void* p) { std::free(p); }
+
+int xc_normalize_contraction(int l, int nprim, const double* alpha, double* coef) {
+  return guarded([&] {
+    require(l >= 0 && nprim > 0 && alpha && coef, XC_ERR_ARG, "xc_normalize_contraction: bad arguments");
+    std::vector<double> a(alpha, alpha + nprim), c(coef, coef + nprim);
+    renormalize_contraction(l, a, c);
+    std::copy(c.begin(), c.end(), coef);
+  });
+}
+
+int xc_load_basis(const char* path, int natoms, const int* Z, const double* coords, int pure, int* nshell,
+                  int** shell_l, int** shell_nprim, int** shell_atom, double** shell_center, double** alpha,
+                  double** coef) {
+  return guarded([&] {
+    require(path && natoms > 0 && Z && coords && nshell && shell_l && shell_nprim && shell_atom && shell_center &&
+                alpha && coef,
+            XC_ERR_ARG, "xc_load_basis: bad arguments");
+    static const char* symbols[] = {"",  "h",  "he", "li", "be", "b",  "c",  "n",  "o",  "f",  "ne", "na", "mg",
+                                    "al", "si", "p",  "s",  "cl", "ar", "k",  "ca", "sc", "ti", "v",  "cr", "mn",
+                                    "fe", "co", "ni", "cu", "zn", "ga", "ge", "as", "se", "br", "kr"};
+    std::vector<Shell> all;
+    for (int a = 0; a < natoms; ++a) {
+      require(Z[a] >= 1 && Z[a] <= 36, XC_ERR_INPUT, "xc_load_basis: element out of range");
+      auto sh = parse_turbomole_basis(path, symbols[Z[a]], pure != 0);
+      for (auto& s : sh) {
+        s.center = {coords[3 * a], coords[3 * a + 1], coords[3 * a + 2]};
+        s.atom = a;
+        all.push_back(s);
+      }
+    }
+    const int ns = (int)all.size();
+    int np = 0;
+    for (auto& s : all) np += (int)s.alpha.size();
+    *nshell = ns;
+    *shell_l = (int*)std::malloc(sizeof(int) * ns);
+    *shell_nprim = (int*)std::malloc(sizeof(int) * ns);
+    *shell_atom = (int*)std::malloc(sizeof(int) * ns);
+    *shell_center = (double*)std::malloc(sizeof(double) * 3 * ns);
+    *alpha = (double*)std::malloc(sizeof(double) * np);
+    *coef = (double*)std::malloc(sizeof(double) * np);
+    int k = 0;
+    for (int s = 0; s < ns; ++s) {
+      (*shell_l)[s] = all[s].l;
+      (*shell_nprim)[s] = (int)all[s].alpha.size();
+      (*shell_atom)[s] = all[s].atom;
+      for (int d = 0; d < 3; ++d) (*shell_center)[3 * s + d] = all[s].center[d];
+      for (size_t q = 0; q < all[s].alpha.size(); ++q, ++k) {
+        (*alpha)[k] = all[s].alpha[q];
+        (*coef)[k] = all[s].coef[q];
+      }
+    }
+  });
+}
+
+int xc_molecular_grid(int natoms, const int* Z, const double* coords, int grid_type, int radial_type, int accuracy,
+                      int smoothing, double weight_threshold, int sort, int64_t* npoints, double** points,
+                      double** weights, int64_t* npoints_before_cut) {
+  return guarded([&] {
+    require(natoms > 0 && Z && coords && npoints && points && weights, XC_ERR_ARG, "xc_molecular_grid: bad arguments");
+    require(grid_type >= 0 && grid_type <= 2 && radial_type >= 0 && radial_type <= 4, XC_ERR_ARG,
+            "xc_molecular_grid: unknown grid or radial type");
+    GridSettings s;
+    s.type = (GridType)grid_type;
+    s.radial = (RadialType)radial_type;
+    s.accuracy = accuracy;
+    s.smoothing = (unsigned)smoothing;
+    s.weight_threshold = weight_threshold;
+    s.sort = sort != 0;
+    MolecularGrid g = build_molecular_grid(std::vector<int>(Z, Z + natoms), std::vector<double>(coords, coords + 3 * natoms), s);
+    const int64_t n = (int64_t)g.weights.size();
+    *npoints = n;
+    *points = (double*)std::malloc(sizeof(double) * 3 * std::max<int64_t>(n, 1));
+    *weights = (double*)std::malloc(sizeof(double) * std::max<int64_t>(n, 1));
+    std::memcpy(*points, g.points.data(), sizeof(double) * 3 * n);
+    std::memcpy(*weights, g.weights.data(), sizeof(double) * n);
+    if (npoints_before_cut) *npoints_before_cut = g.n_before_cut;
+  });
+}
+
+int xc_hilbert_sort(int64_t npoints, double* points, double* weights) {
+  return guarded([&] {
+    require(npoints >= 0 && (npoints == 0 || (points && weights)), XC_ERR_ARG, "xc_hilbert_sort: bad arguments");
+    std::vector<double> p(points, points + 3 * npoints), w(weights, weights + npoints);
+    hilbert_sort(p, w);
+    std::copy(p.begin(), p.end(), points);
+    std::copy(w.begin(), w.end(), weights);
+  });
+}
+
+int xc_lebedev(int npoints, double* x, double* y, double* z, double* w) {
+  return guarded([&] {
+    require(x && y && z && w, XC_ERR_ARG, "xc_lebedev: null output");
+    std::vector<double> vx, vy, vz, vw;
+    lebedev_rule(npoints, vx, vy, vz, vw);
+    std::copy(vx.begin(), vx.end(), x);
+    std::copy(vy.begin(), vy.end(), y);
+    std::copy(vz.begin(), vz.end(), z);
+    std::copy(vw.begin(), vw.end(), w);
+  });
+}
+
+int xc_overlap(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim, const double* shell_center,
+               const double* alpha, const double* coef, double* S) {
+  return guarded([&] {
+    require(nshell > 0 && shell_l && shell_pure && shell_nprim && shell_center && alpha && coef && S, XC_ERR_ARG,
+            "xc_overlap: bad arguments");
+    std::vector<Shell> sh(nshell);
+    int k = 0;
+    for (int s = 0; s < nshell; ++s) {
+      sh[s].l = shell_l[s];
+      sh[s].pure = shell_pure[s] != 0;
+      require(sh[s].l >= 0 && sh[s].l <= 6, XC_ERR_INPUT, "xc_overlap: angular momentum above 6");
+      sh[s].center = {shell_center[3 * s], shell_center[3 * s + 1], shell_center[3 * s + 2]};
+      for (int q = 0; q < shell_nprim[s]; ++q, ++k) {
+        sh[s].alpha.push_back(alpha[k]);
+        sh[s].coef.push_back(coef[k]);
+      }
+    }
+    std::vector<double> out;
+    overlap_matrix(sh, out);
+    std::copy(out.begin(), out.end(), S);
+  });
+}
+
+int xc_basis_create(xc_ctx* ctx, int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
+                    const double* shell_center, const double* alpha, const double* coef, xc_basis** out) {
+  return guarded([&] {
+    require(ctx && out && nshell > 0 && shell_l && shell_pure && shell_nprim && shell_center && alpha && coef,
+            XC_ERR_ARG, "xc_basis_create: bad arguments");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    auto* b = new xc_basis();
+    b->ctx = ctx;
+    b->nshell = nshell;
+    int prim = 0, ao = 0;
+    for (int s = 0; s < nshell; ++s) {
+      const int l = shell_l[s];
+      if (l < 0 || l > kMaxL) {
+        delete b;
+        throw XcError(XC_ERR_INPUT, "xc_basis_create: angular momentum above " + std::to_string(kMaxL));
+      }
+      DevShell d{};
+      d.cx = shell_center[3 * s];
+      d.cy = shell_center[3 * s + 1];
+      d.cz = shell_center[3 * s + 2];
+      d.l = l;
+      d.pure = shell_pure[s] != 0;
+      d.nprim = shell_nprim[s];
+      d.prim_off = prim;
+      d.ao_off = ao;
+      d.ncart = (l + 1) * (l + 2) / 2;
+      d.nfunc = d.pure ? 2 * l + 1 : d.ncart;
+      d.ang_off = (int)b->ang.size();
+      if (d.pure) {
+        auto T = solid_harmonic_coefficients(l);
+        b->ang.insert(b->ang.end(), T.begin(), T.end());
+      } else {
+        auto f = cartesian_norm_factors(l);
+        for (int i = 0; i < d.ncart; ++i)
+          for (int j = 0; j < d.ncart; ++j) b->ang.push_back(i == j ? f[i] : 0.0);
+      }
+      for (int q = 0; q < d.nprim; ++q) {
+        b->alpha.push_back(alpha[prim + q]);
+        b->coef.push_back(coef[prim + q]);
+      }
+      prim += d.nprim;
+      ao += d.nfunc;
+      b->maxl = std::max(b->maxl, l);
+      b->shells.push_back(d);
+    }
+    b->nao = ao;
+    b->d_shells.upload(ctx, b->shells.data(), b->shells.size());
+    b->d_alpha.upload(ctx, b->alpha.data(), b->alpha.size());
+    b->d_coef.upload(ctx, b->coef.data(), b->coef.size());
+    b->d_ang.upload(ctx, b->ang.data(), b->ang.size());
+    HIPX(hipStreamSynchronize(ctx->stream));
+    *out = b;
+  });
+}
+
+int xc_basis_destroy(xc_basis* basis) {
+  return guarded([&] { delete basis; });
+}
+
+int xc_basis_nao(const xc_basis* basis, int* nao) {
+  return guarded([&] {
+    require(basis && nao, XC_ERR_ARG, "null argument");
+    *nao = basis->nao;
+  });
+}
+
+int xc_grid_create(xc_ctx* ctx, int64_t npoints, const double* points, const double* weights, xc_grid** out) {
+  return guarded([&] {
+    require(ctx && out && npoints > 0 && points && weights, XC_ERR_ARG, "xc_grid_create: bad arguments");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    auto* g = new xc_grid();
+    g->ctx = ctx;
+    g->npts = npoints;
+    g->d_pts.upload(ctx, points, (size_t)(3 * npoints));
+    g->d_w.upload(ctx, weights, (size_t)npoints);
+    HIPX(hipStreamSynchronize(ctx->stream));
+    *out = g;
+  });
+}
+
+int xc_grid_destroy(xc_grid* grid) {
+  return guarded([&] { delete grid; });
+}
+
+int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double radial_threshold,
+                    int highest_derivative, int keep_fp32, xc_bfogc** out) {
+  return guarded([&] {
+    require(basis && grid && out, XC_ERR_ARG, "xc_bfogc_create: null argument");
+    require(basis->ctx == grid->ctx, XC_ERR_ARG, "xc_bfogc_create: basis and grid live on different contexts");
+    require(max_block_size >= 1 && max_block_size <= kTile, XC_ERR_ARG,
+            "xc_bfogc_create: max block size must be in [1, 128]");
+    require(highest_derivative == 0 || highest_derivative == 1, XC_ERR_ARG,
+            "xc_bfogc_create: highest derivative must be 0 or 1");
+    require(radial_threshold >= 0.0, XC_ERR_ARG, "xc_bfogc_create: radial threshold must not be negative");
+    xc_ctx* ctx = basis->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipSetDevice(ctx->device));
+    auto* bf = new xc_bfogc();
+    try {
+      bf->ctx = ctx;
+      bf->basis = basis;
+      bf->grid = grid;
+      bf->bs = max_block_size;
+      bf->radial_thr = radial_threshold;
+      bf->deriv = highest_derivative;
+      bf->ncomp = highest_derivative >= 1 ? 4 : 1;
+      const int64_t N = grid->npts;
+      const int64_t nb64 = (N + max_block_size - 1) / max_block_size;
+      require(nb64 < (1LL << 31), XC_ERR_ARG, "xc_bfogc_create: too many blocks");
+      const int nb = (int)nb64;
+      bf->nblocks = nb;
+      const int ns = basis->nshell;
+      // 1. shell screening per block on the device
+      DBuf<unsigned char> d_flags;
+      d_flags.alloc(ctx, (size_t)nb * ns);
+      launch_screen(ctx->stream, grid->d_pts.p, N, max_block_size, nb, basis->d_shells.p, ns, basis->d_alpha.p,
+                    basis->d_coef.p, radial_threshold, d_flags.p);
+      HIPX(hipGetLastError());
+      std::vector<unsigned char> flags((size_t)nb * ns);
+      HIPX(hipMemcpyAsync(flags.data(), d_flags.p, flags.size(), hipMemcpyDeviceToHost, ctx->stream));
+      HIPX(hipStreamSynchronize(ctx->stream));
+      // 2. block lists (compact AO index lists padded to kAoPad) and tile offsets
+      std::vector<int> sig_shell, sig_pos;
+      bf->blocks.resize(nb);
+      int64_t tile_off = 0;
+      for (int b = 0; b < nb; ++b) {
+        DevBlock& bk = bf->blocks[b];
+        bk.first = (int64_t)b * max_block_size;
+        bk.npts = (int)std::min<int64_t>(max_block_size, N - bk.first);
+        bk.idx_off = (int)bf->ao_idx.size();
+        bk.shell_off = (int)sig_shell.size();
+        int cnt = 0, nsh = 0;
+        for (int s = 0; s < ns; ++s) {
+          if (!flags[(size_t)b * ns + s]) continue;
+          const DevShell& sh = basis->shells[s];
+          sig_shell.push_back(s);
+          sig_pos.push_back(cnt);
+          for (int f = 0; f < sh.nfunc; ++f) bf->ao_idx.push_back(sh.ao_off + f);
+          cnt += sh.nfunc;
+          ++nsh;
+        }
+        bk.nb = cnt;
+        bk.nshell = nsh;
+        bk.npad = (cnt + kAoPad - 1) / kAoPad * kAoPad;
+        for (int i = cnt; i < bk.npad; ++i) bf->ao_idx.push_back(-1);
+        bk.tile_off = tile_off;
+        tile_off += (int64_t)bk.npad * kTile * bf->ncomp;
+        bf->sum_nb += cnt;
+        bf->sum_B_nb2 += (double)bk.npts * cnt * cnt;
+      }
+      bf->tile_elems = tile_off;
+      // 3. admission by footprint
+      size_t free_b = 0, total_b = 0;
+      HIPX(hipMemGetInfo(&free_b, &total_b));
+      const double need = (double)tile_off * (sizeof(double) + (keep_fp32 ? sizeof(float) : 0));
+      require(need < 0.9 * (double)free_b, XC_ERR_MEMORY,
+              "xc_bfogc_create: AO tiles need " + std::to_string(need / 1e9) + " GB, " +
+                  std::to_string(free_b / 1e9) + " GB free");
+      // 4. upload lists, evaluate AO tiles
+      bf->d_blocks.upload(ctx, bf->blocks.data(), bf->blocks.size());
+      bf->d_ao_idx.upload(ctx, bf->ao_idx.data(), bf->ao_idx.size());
+      bf->d_sig_shell.upload(ctx, sig_shell.data(), sig_shell.size());
+      bf->d_sig_pos.upload(ctx, sig_pos.data(), sig_pos.size());
+      bf->d_t64.alloc(ctx, (size_t)std::max<int64_t>(tile_off, 1));
+      HIPX(hipMemsetAsync(bf->d_t64.p, 0, sizeof(double) * bf->d_t64.n, ctx->stream));
+      if (keep_fp32) {
+        bf->d_t32.alloc(ctx, (size_t)std::max<int64_t>(tile_off, 1));
+        HIPX(hipMemsetAsync(bf->d_t32.p, 0, sizeof(float) * bf->d_t32.n, ctx->stream));
+      }
+      launch_ao_eval(ctx->stream, bf->d_blocks.p, nb, bf->d_sig_shell.p, bf->d_sig_pos.p, basis->d_shells.p,
+                     basis->d_alpha.p, basis->d_coef.p, basis->d_ang.p, grid->d_pts.p, radial_threshold,
+                     radial_threshold > 0.0 ? -std::log(radial_threshold) : HUGE_VAL, bf->ncomp, bf->d_t64.p, keep_fp32 ? bf->d_t32.p : nullptr);
+      HIPX(hipGetLastError());
+      // 5. work buffers of a build
+      const int n = basis->nao;
+      const int nxc = (int)((N + kTile - 1) / kTile);
+      bf->d_rho.alloc(ctx, N);
+      bf->d_grad.alloc(ctx, 3 * N);
+      bf->d_vs.alloc(ctx, N);
+      bf->d_gw.alloc(ctx, 3 * N);
+      bf->d_v.alloc(ctx, N);
+      bf->d_g.alloc(ctx, 3 * N);
+      bf->d_eblk.alloc(ctx, std::max(nxc, 1));
+      bf->d_U.alloc(ctx, (size_t)n * n);
+      bf->d_P.alloc(ctx, (size_t)n * n);
+      bf->d_F.alloc(ctx, (size_t)n * n);
+      bf->d_E.alloc(ctx, 1);
+      if (keep_fp32) bf->d_P32.alloc(ctx, (size_t)n * n);
+      HIPX(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      delete bf;
+      throw;
+    }
+    *out = bf;
+  });
+}
+
+int xc_bfogc_destroy(xc_bfogc* bfogc) {
+  return guarded([&] {
+    if (!bfogc) return;
+    (void)hipStreamSynchronize(bfogc->ctx->stream);
+    delete bfogc;
+  });
+}
+
+int xc_bfogc_info(const xc_bfogc* bf, int* nblocks, int64_t* npoints, int* nao, int* max_block_size) {
+  return guarded([&] {
+    require(bf, XC_ERR_ARG, "null bfogc");
+    if (nblocks) *nblocks = bf->nblocks;
+    if (npoints) *npoints = bf->grid->npts;
+    if (nao) *nao = bf->basis->nao;
+    if (max_block_size) *max_block_size = bf->bs;
+  });
+}
+
+int xc_bfogc_stats(const xc_bfogc* bf, int64_t* sum_nb, double* sum_B_nb2, int64_t* tile_bytes) {
+  return guarded([&] {
+    require(bf, XC_ERR_ARG, "null bfogc");
+    if (sum_nb) *sum_nb = bf->sum_nb;
+    if (sum_B_nb2) *sum_B_nb2 = bf->sum_B_nb2;
+    if (tile_bytes) *tile_bytes = bf->tile_elems * (int64_t)(sizeof(double) + (bf->d_t32.p ? sizeof(float) : 0));
+  });
+}
+
+int xc_bfogc_block_data(const xc_bfogc* bf, int block, double* values, double* gx, double* gy, double* gz,
+                        int* negligible, int* block_size) {
+  return guarded([&] {
+    require(bf, XC_ERR_ARG, "null bfogc");
+    require(block >= 0 && block < bf->nblocks, XC_ERR_ARG, "block index out of range");
+    require(!(gx || gy || gz) || bf->deriv >= 1, XC_ERR_STATE, "gradients were not evaluated (highest derivative 0)");
+    std::lock_guard<std::recursive_mutex> lk(bf->ctx->mu);
+    const DevBlock& bk = bf->blocks[block];
+    const int n = bf->basis->nao;
+    const int64_t cnt = (int64_t)bk.npad * kTile * bf->ncomp;
+    std::vector<double> t((size_t)std::max<int64_t>(cnt, 1));
+    if (cnt) {
+      HIPX(hipMemcpyAsync(t.data(), bf->d_t64.p + bk.tile_off, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                          bf->ctx->stream));
+      HIPX(hipStreamSynchronize(bf->ctx->stream));
+    }
+    const int B = bk.npts;
+    if (block_size) *block_size = B;
+    double* outs[4] = {values, gx, gy, gz};
+    for (int c = 0; c < 4; ++c)
+      if (outs[c]) std::fill(outs[c], outs[c] + (size_t)B * n, 0.0);
+    if (negligible) std::fill(negligible, negligible + n, 1);
+    for (int i = 0; i < bk.nb; ++i) {
+      const int mu = bf->ao_idx[bk.idx_off + i];
+      if (negligible) negligible[mu] = 0;
+      for (int c = 0; c < bf->ncomp; ++c) {
+        if (!outs[c]) continue;
+        for (int p = 0; p < B; ++p)
+          outs[c][(size_t)mu * B + p] = t[((size_t)c * bk.npad + i) * kTile + p];
+      }
+    }
+  });
+}
+
+int xc_density(xc_bfogc* bf, const double* P, double* rho, double* grad, int* nonneg) {
+  return guarded([&] {
+    require(bf && P && rho, XC_ERR_ARG, "xc_density: null argument");
+    require(!grad || bf->deriv >= 1, XC_ERR_STATE, "xc_density: gradient requested but AO gradients not evaluated");
+    xc_ctx* ctx = bf->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const int n = bf->basis->nao;
+    const int64_t N = bf->grid->npts;
+    HIPX(hipMemcpyAsync(bf->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
+    launch_density<double>(ctx->stream, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t64.p, bf->ncomp,
+                           bf->d_P.p, n, grad ? 1 : 0, bf->d_rho.p, bf->d_grad.p, N);
+    HIPX(hipGetLastError());
+    HIPX(hipMemcpyAsync(rho, bf->d_rho.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    if (grad) HIPX(hipMemcpyAsync(grad, bf->d_grad.p, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+    if (nonneg)
+      for (int b = 0; b < bf->nblocks; ++b) nonneg[b] = bf->blocks[b].nb > 0 ? 1 : 0;
+  });
+}
+
+int xc_functional(xc_ctx* ctx, int functional, int64_t npts, int block_size, const double* rho, const double* grad,
+                  const double* weights, double* eps, double* v, double* g, double* energy) {
+  return guarded([&] {
+    require(ctx && rho && weights && npts > 0, XC_ERR_ARG, "xc_functional: bad arguments");
+    require(functional >= 0 && functional <= 6, XC_ERR_ARG, "xc_functional: unknown functional");
+    require(block_size >= 1 && block_size <= kTile, XC_ERR_ARG, "xc_functional: block size must be in [1,128]");
+    const bool gga = functional_is_gga(functional);
+    require(!gga || grad, XC_ERR_ARG, "xc_functional: GGA functional needs the density gradient");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DBuf<double> d_rho, d_grad, d_w, d_eps, d_v, d_g, d_eblk, d_E;
+    const int nxc = (int)((npts + block_size - 1) / block_size);
+    d_rho.upload(ctx, rho, npts);
+    if (gga) d_grad.upload(ctx, grad, 3 * npts);
+    d_w.upload(ctx, weights, npts);
+    d_eps.alloc(ctx, npts);
+    d_v.alloc(ctx, npts);
+    if (gga) d_g.alloc(ctx, 3 * npts);
+    d_eblk.alloc(ctx, nxc);
+    d_E.alloc(ctx, 1);
+    launch_functional(ctx->stream, functional, ctx->prm, npts, block_size, d_rho.p, d_grad.p, d_w.p, d_eps.p, d_v.p,
+                      d_g.p, nullptr, nullptr, d_eblk.p);
+    launch_sum(ctx->stream, d_eblk.p, nxc, d_E.p);
+    HIPX(hipGetLastError());
+    if (eps) HIPX(hipMemcpyAsync(eps, d_eps.p, sizeof(double) * npts, hipMemcpyDeviceToHost, ctx->stream));
+    if (v) HIPX(hipMemcpyAsync(v, d_v.p, sizeof(double) * npts, hipMemcpyDeviceToHost, ctx->stream));
+    if (g) {
+      if (gga)
+        HIPX(hipMemcpyAsync(g, d_g.p, sizeof(double) * 3 * npts, hipMemcpyDeviceToHost, ctx->stream));
+      else
+        std::fill(g, g + 3 * npts, 0.0);
+    }
+    double E = 0.0;
+    HIPX(hipMemcpyAsync(&E, d_E.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+    if (energy) *energy = E;
+  });
+}
+
+int xc_fock_add(xc_bfogc* bf, double thr, const double* v, const double* g, double* F) {
+  return guarded([&] {
+    require(bf && v && F, XC_ERR_ARG, "xc_fock_add: null argument");
+    require(!g || bf->deriv >= 1, XC_ERR_STATE, "xc_fock_add: gradient potential needs AO gradients");
+    xc_ctx* ctx = bf->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const int n = bf->basis->nao;
+    const int64_t N = bf->grid->npts;
+    HIPX(hipMemcpyAsync(bf->d_v.p, v, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    if (g) HIPX(hipMemcpyAsync(bf->d_g.p, g, sizeof(double) * 3 * N, hipMemcpyHostToDevice, ctx->stream));
+    launch_weight_potential(ctx->stream, N, bf->grid->d_w.p, bf->d_v.p, g ? bf->d_g.p : nullptr, bf->d_vs.p,
+                            bf->d_gw.p);
+    launch_fill(ctx->stream, bf->d_U.p, (int64_t)n * n, 0.0);
+    launch_fock<double>(ctx->stream, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t64.p, bf->ncomp, bf->d_vs.p,
+                        bf->d_gw.p, g ? 1 : 0, thr, bf->d_U.p, n, N);
+    HIPX(hipMemcpyAsync(bf->d_F.p, F, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
+    launch_finalize(ctx->stream, bf->d_U.p, bf->d_F.p, n, 1);
+    HIPX(hipGetLastError());
+    HIPX(hipMemcpyAsync(F, bf->d_F.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+static void enqueue_build(xc_bfogc* bf, int functional, int precision, double thr, const double* dP, double* dF) {
+  xc_ctx* ctx = bf->ctx;
+  require(functional >= 0 && functional <= 6, XC_ERR_ARG, "unknown functional");
+  require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+  const bool gga = functional_is_gga(functional);
+  require(!gga || bf->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
+  require(precision == XC_PREC_FP64 || bf->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+  const int n = bf->basis->nao;
+  const int64_t N = bf->grid->npts;
+  const int nxc = (int)((N + kTile - 1) / kTile);
+  hipStream_t s = ctx->stream;
+  ctx->mark();
+  if (precision == XC_PREC_FP32) launch_to_float(s, dP, bf->d_P32.p, (int64_t)n * n);
+  ctx->mark();
+  if (precision == XC_PREC_FP64)
+    launch_density<double>(s, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t64.p, bf->ncomp, dP, n, gga ? 1 : 0,
+                           bf->d_rho.p, bf->d_grad.p, N);
+  else
+    launch_density<float>(s, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t32.p, bf->ncomp, bf->d_P32.p, n,
+                          gga ? 1 : 0, bf->d_rho.p, bf->d_grad.p, N);
+  ctx->mark();
+  launch_functional(s, functional, ctx->prm, N, kTile, bf->d_rho.p, gga ? bf->d_grad.p : nullptr, bf->grid->d_w.p,
+                    nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
+  ctx->mark();
+  launch_fill(s, bf->d_U.p, (int64_t)n * n, 0.0);
+  ctx->mark();
+  if (precision == XC_PREC_FP64)
+    launch_fock<double>(s, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t64.p, bf->ncomp, bf->d_vs.p, bf->d_gw.p,
+                        gga ? 1 : 0, thr, bf->d_U.p, n, N);
+  else
+    launch_fock<float>(s, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t32.p, bf->ncomp, bf->d_vs.p, bf->d_gw.p,
+                       gga ? 1 : 0, thr, bf->d_U.p, n, N);
+  ctx->mark();
+  launch_finalize(s, bf->d_U.p, dF, n, 0);
+  launch_sum(s, bf->d_eblk.p, nxc, ctx->d_energy);
+  ctx->mark();
+  HIPX(hipGetLastError());
+}
+
+int xc_fock_build_device(xc_bfogc* bf, int functional, int precision, double thr, const double* dP, double* dF,
+                         double* energy) {
+  return guarded([&] {
+    require(bf && dP && dF, XC_ERR_ARG, "xc_fock_build_device: null argument");
+    xc_ctx* ctx = bf->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    enqueue_build(bf, functional, precision, thr, dP, dF);
+    if (energy) {
+      HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIPX(hipStreamSynchronize(ctx->stream));
+      *energy = ctx->last_energy;
+    }
+  });
+}
+
+int xc_last_energy(xc_ctx* ctx, double* energy) {
+  return guarded([&] {
+    require(ctx && energy, XC_ERR_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+    *energy = ctx->last_energy;
+  });
+}
+
+int xc_fock_build(xc_bfogc* bf, int functional, int precision, double thr, const double* P, double* F,
+                  double* energy) {
+  return guarded([&] {
+    require(bf && P && F, XC_ERR_ARG, "xc_fock_build: null argument");
+    xc_ctx* ctx = bf->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const int n = bf->basis->nao;
+    HIPX(hipMemcpyAsync(bf->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
+    enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p);
+    HIPX(hipMemcpyAsync(F, bf->d_F.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+    if (energy) *energy = ctx->last_energy;
+  });
+}
+
+int xc_profile(xc_ctx* ctx, int enable) {
+  return guarded([&] {
+    require(ctx, XC_ERR_ARG, "null context");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->profile = enable != 0;
+  });
+}
+
+int xc_profile_collect(xc_ctx* ctx, double* stage_ms, int* nbuilds) {
+  return guarded([&] {
+    require(ctx && stage_ms, XC_ERR_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipStreamSynchronize(ctx->stream));
+    const size_t per = XC_NSTAGES + 1;
+    for (int k = 0; k < XC_NSTAGES; ++k) stage_ms[k] = 0.0;
+    const size_t nb = ctx->ev_used / per;
+    for (size_t b = 0; b < nb; ++b)
+      for (int k = 0; k < XC_NSTAGES; ++k) {
+        float ms = 0.0f;
+        HIPX(hipEventElapsedTime(&ms, ctx->ev_pool[b * per + k], ctx->ev_pool[b * per + k + 1]));
+        stage_ms[k] += ms;
+      }
+    if (nbuilds) *nbuilds = (int)nb;
+    ctx->ev_used = 0;
+  });
+}
+
+int xc_device_alloc(xc_ctx* ctx, size_t bytes, void** dptr) {
+  return guarded([&] {
+    require(ctx && dptr, XC_ERR_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    *dptr = ctx->alloc(bytes);
+    if (*dptr) ctx->user_allocs[*dptr] = bytes;
+  });
+}
+
+int xc_device_free(xc_ctx* ctx, void* dptr) {
+  return guarded([&] {
+    require(ctx, XC_ERR_ARG, "null context");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipStreamSynchronize(ctx->stream));
+    if (dptr) {
+      auto it = ctx->user_allocs.find(dptr);
+      require(it != ctx->user_allocs.end(), XC_ERR_ARG, "xc_device_free: pointer not from xc_device_alloc");
+      ctx->release(dptr, it->second);
+      ctx->user_allocs.erase(it);
+    }
+  });
+}
+
+int xc_memcpy_h2d(xc_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    require(ctx && dst && src, XC_ERR_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int xc_memcpy_d2h(xc_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    require(ctx && dst && src, XC_ERR_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,241 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the
+reference's golden vectors.  FP64 bar: |dE_xc| <= 1e-9 Eh and max|dF| <= 1e-9 Eh
+against the oracle on the same geometry, basis, grid and density matrix."""
+import sys
+
+import numpy as np
+import pytest
+
+import common
+from oracle import oracle as O
+from serenity_amd import xc
+from serenity_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HESS_SKIP = ("hessian",)
+TOL_E = 1e-9
+TOL_F = 1e-9
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def test_ao_known_answers(ctx):
+    kat = common.load_json("kat_ao.json")
+    pts, w = common.tiny_grid()
+    basis = common.small_mixed_basis()
+    say("AO KAT: SMALL_MIXED on TINY")
+    bf = xc.BasisFunctionOnGridController(ctx, basis, xc.GridController(pts, w), kat["block_size"],
+                                          kat["radial_threshold"], 1)
+    assert bf.getNBlocks() == 1 and bf.getNGridPoints() == 4
+    d = bf.getBlockOnGridData(0)
+    for e in kat["entries"]:
+        k = e["kind"]
+        if k.startswith(HESS_SKIP):
+            continue
+        v = d["functionValues"][e["point"], e["ao"]] if k == "values" else \
+            d["derivativeValues"]["xyz".index(k[-1])][e["point"], e["ao"]]
+        assert abs(v - e["expected"]) < kat["tolerance"], e
+    ref = O.ao_block(basis, pts, thr=kat["radial_threshold"], deriv=1)
+    assert np.abs(d["functionValues"] - ref["values"]).max() < 1e-14
+    for c in range(3):
+        assert np.abs(d["derivativeValues"][c] - ref["grad"][c]).max() < 1e-14
+
+
+def test_density_known_answers(ctx):
+    kat = common.load_json("kat_density.json")
+    pts, w = common.tiny_grid()
+    basis = common.small_mixed_basis()
+    say("density KAT: block size 3, no screening")
+    bf = xc.BasisFunctionOnGridController(ctx, basis, xc.GridController(pts, w), kat["block_size"], 0.0, 1)
+    calc = xc.DensityOnGridCalculator(bf, 0.0)
+    P = np.array(kat["P"])
+    rho, grad = calc.calcDensityAndGradientOnGrid(P)
+    assert np.array_equal(calc.calcDensityOnGrid(P), rho)
+    for e in kat["entries"]:
+        k = e["kind"]
+        if k.startswith("hess"):
+            continue
+        v = rho[e["point"]] if k == "densOnGrid" else grad["xyz".index(k[-1])][e["point"]]
+        assert abs(v - e["expected"]) < kat["tolerance"], e
+    ref = O.density(basis, pts, P, block_size=3, thr=0.0, grad=True)
+    assert np.abs(rho - ref["rho"]).max() < 1e-14
+    assert np.abs(grad - ref["grad"]).max() < 1e-14
+    assert list(calc.getNonNegligibleBlocks()) == list(ref["nonneg"])
+
+
+def test_fock_known_answers(ctx):
+    kat = common.load_json("kat_fock.json")
+    pts, w = common.very_small_grid()
+    basis = common.small_mixed_basis()
+    say("Fock KAT: VERY_SMALL, block size 3 (short last block)")
+    bf = xc.BasisFunctionOnGridController(ctx, basis, xc.GridController(pts, w), kat["block_size"],
+                                          kat["radial_threshold"], 1)
+    adder = xc.ScalarOperatorToMatrixAdder(bf, kat["block_ave_threshold"])
+    F = np.zeros((basis.nao, basis.nao))
+    adder.addScalarOperatorToMatrix(F, np.array(kat["potential"]), np.array(kat["gradient"]))
+    for e in kat["entries"]:
+        assert abs(F[e["row"], e["col"]] - e["expected"]) < kat["tolerance"], e
+    ref = O.fock_add(basis, pts, w, np.array(kat["potential"]), np.array(kat["gradient"]), block_size=3,
+                     thr=kat["radial_threshold"], block_ave_thr=kat["block_ave_threshold"])
+    assert np.abs(F - ref).max() < 1e-15
+    # += semantics and the scalar-only (LDA) form
+    F2 = np.ones((basis.nao, basis.nao))
+    adder.addScalarOperatorToMatrix(F2, np.array(kat["potential"]))
+    ref2 = O.fock_add(basis, pts, w, np.array(kat["potential"]), None, F=np.ones((basis.nao, basis.nao)),
+                      block_size=3, thr=kat["radial_threshold"], block_ave_thr=kat["block_ave_threshold"])
+    assert np.abs(F2 - ref2).max() < 1e-15
+
+
+@pytest.mark.parametrize("name", ["LDA", "PBE", "TF", "SLATER", "VWN5", "PBEX", "PBEC"])
+def test_functional_matches_oracle(ctx, name):
+    rng = np.random.default_rng(7)
+    N = 128 * 37 + 5
+    rho = 10.0 ** rng.uniform(-9, 2, N)
+    rho[:128] = 1e-16                      # a skipped block
+    rho[200:210] = 1e-15                   # tiny densities inside a kept block
+    grad = rng.standard_normal((3, N)) * np.sqrt(rho) * rng.uniform(0, 3, N)
+    w = rng.uniform(0, 1, N)
+    say("functional", name, "N", N)
+    fl = xc.FunctionalLibrary(ctx, 128)
+    d = fl.calcData(name, rho, grad, w)
+    eps, v, g, E = O.functional(name, rho, grad if name in O.GGA else None, w)
+    assert np.allclose(d.epuv, eps, rtol=1e-12, atol=1e-300)
+    assert np.allclose(d.dFdRho, v, rtol=1e-11, atol=1e-300)
+    if g is not None:
+        assert np.allclose(d.dFdGradRho, g, rtol=1e-10, atol=1e-14 * np.abs(g).max())
+    assert abs(d.energy - E) < 1e-12 * max(1.0, abs(E))
+    assert np.all(d.epuv[:128] == 0) and np.all(d.dFdRho[:128] == 0)
+
+
+def _system(kind):
+    if kind == "h2o_fixture_pbe":
+        fx = common.fixtures()
+        Z, X = fx["water_Z"], fx["water_xyz_angstrom"] * xc.ANGSTROM_TO_BOHR
+        b = xc.BasisController.from_file("DEF2-TZVP", Z, X)
+        return b, xc.molecular_grid(Z, X), fx["water_P"], "PBE"
+    if kind == "h2o2_svp_lda":
+        Z, X = W.water_cluster(1, 1, 2)
+        func = "LDA"
+        basis = "DEF2-SVP"
+    elif kind == "butane_svp_pbe":
+        Z, X = W.alkane(4)
+        func, basis = "PBE", "DEF2-SVP"
+    elif kind == "h2o4_tzvp_pbe":
+        Z, X = W.water_cluster(2, 2, 1)
+        func, basis = "PBE", "DEF2-TZVP"
+    elif kind == "config1_h2o8_svp_lda":
+        Z, X = W.water_cluster(2, 2, 2)
+        func, basis = "LDA", "DEF2-SVP"
+    elif kind == "config2_c20h42_svp_pbe":
+        Z, X = W.alkane(20)
+        func, basis = "PBE", "DEF2-SVP"
+    else:
+        raise ValueError(kind)
+    b = xc.BasisController.from_file(basis, Z, X)
+    g = xc.molecular_grid(Z, X)
+    P = W.synthetic_density(b.overlap(), W.n_electrons(Z) // 2)
+    return b, g, P, func
+
+
+SYSTEMS = ["h2o_fixture_pbe", "h2o2_svp_lda", "butane_svp_pbe", "h2o4_tzvp_pbe", "config1_h2o8_svp_lda",
+           "config2_c20h42_svp_pbe"]
+
+
+@pytest.mark.parametrize("kind", SYSTEMS)
+def test_fock_build_fp64_matches_oracle(ctx, kind):
+    b, g, P, func = _system(kind)
+    say("FP64 build", kind, "nao", b.nao, "npts", g.getNGridPoints())
+    bf = xc.BasisFunctionOnGridController(ctx, b, g, 128, 1e-9, 1, keep_fp32=True)
+    pot = xc.FuncPotential(bf, func, "FP64")
+    F = pot.getMatrix(P)
+    E = pot.getEnergy()
+    Fo, Eo, rho_o, _ = O.fock_build(b, g.points, g.weights, P, func)
+    dE, dF = abs(E - Eo), np.abs(F - Fo).max()
+    say("  E_xc %.12f  |dE| %.2e  max|dF| %.2e" % (E, dE, dF))
+    assert dE <= TOL_E
+    assert dF <= TOL_F
+    assert np.array_equal(F, F.T)
+    # the same build again agrees to rounding (atomic accumulation order may differ)
+    F2 = pot.getMatrix(P)
+    assert np.abs(F2 - F).max() < 1e-12 and abs(pot.getEnergy() - E) < 1e-12
+    # mixed-precision contractions on the same P: error stated, FP64 functional and reductions
+    Fm = xc.FuncPotential(bf, func, "FP32").getMatrix(P)
+    Em = xc.FuncPotential(bf, func, "FP32")
+    Em.getMatrix(P)
+    say("  FP32-MFMA |dE| %.2e  max|dF| %.2e" % (abs(Em.getEnergy() - Eo), np.abs(Fm - Fo).max()))
+    assert abs(Em.getEnergy() - Eo) < 1e-5 * max(1.0, abs(Eo))
+    assert np.abs(Fm - Fo).max() < 1e-4
+
+
+def test_h2_lda_fock_matrix(ctx):
+    """FuncPotential_test.cpp:59-100 through the HIP path (Becke grid, Cartesian def2-TZVP)."""
+    fx = common.fixtures()
+    Z, X = fx["h2_Z"], fx["h2_xyz_angstrom"] * xc.ANGSTROM_TO_BOHR
+    b = xc.BasisController.from_file("DEF2-TZVP", Z, X, spherical=False)
+    g = xc.molecular_grid(Z, X, grid_type="BECKE", accuracy=4, weight_threshold=1e-10, smoothing=3)
+    bf = xc.BasisFunctionOnGridController(ctx, b, g, 128, 1e-9, 1)
+    F = xc.FuncPotential(bf, "LDA").getMatrix(fx["h2_P"])
+    Fref = np.array(common.load_json("kat_funcpot_h2.json")["F"])
+    assert np.abs(F - Fref).max() < 1e-8
+
+
+def test_pbe_energy_fixtures(ctx):
+    fx = common.fixtures()
+    for tag, exc, tol in (("he", -1.0490748073, 1e-6), ("water", -9.2488924049, 5e-6)):
+        Z, X = fx[tag + "_Z"], fx[tag + "_xyz_angstrom"] * xc.ANGSTROM_TO_BOHR
+        b = xc.BasisController.from_file("DEF2-TZVP", Z, X)
+        g = xc.molecular_grid(Z, X)
+        bf = xc.BasisFunctionOnGridController(ctx, b, g)
+        pot = xc.FuncPotential(bf, "PBE")
+        pot.getMatrix(fx[tag + "_P"])
+        assert abs(pot.getEnergy() - exc) < tol
+
+
+def test_device_resident_build_and_linearity(ctx):
+    b, g, P, func = _system("butane_svp_pbe")
+    bf = xc.BasisFunctionOnGridController(ctx, b, g)
+    pot = xc.FuncPotential(bf, func)
+    F = pot.getMatrix(P)
+    E = pot.getEnergy()
+    n = b.nao
+    dP, dF = ctx.alloc(8 * n * n), ctx.alloc(8 * n * n)
+    try:
+        ctx.upload(dP, np.asfortranarray(P).ravel(order="F"))
+        E2 = pot.build_device(dP, dF)
+        F2 = np.zeros(n * n)
+        ctx.download(F2, dF)
+    finally:
+        ctx.free(dP)
+        ctx.free(dF)
+    assert abs(E2 - E) < 1e-12 and np.abs(F2.reshape(n, n, order="F") - F).max() < 1e-12
+    # the grid-to-matrix step is linear in the potential
+    rng = np.random.default_rng(3)
+    N = g.getNGridPoints()
+    v1, v2 = rng.standard_normal(N), rng.standard_normal(N)
+    ad = xc.ScalarOperatorToMatrixAdder(bf, 0.0)
+    A = ad.addScalarOperatorToMatrix(np.zeros((n, n)), v1)
+    B = ad.addScalarOperatorToMatrix(np.zeros((n, n)), v2)
+    AB = ad.addScalarOperatorToMatrix(np.zeros((n, n)), v1 + 2 * v2)
+    assert np.abs(AB - (A + 2 * B)).max() < 1e-11 * max(1.0, np.abs(AB).max())
+    # with v = 1 the matrix is the grid overlap, close to the analytic one
+    Sg = ad.addScalarOperatorToMatrix(np.zeros((n, n)), np.ones(N))
+    assert np.abs(Sg - b.overlap()).max() < 1e-3
+
+
+def test_error_paths(ctx):
+    b, g, P, func = _system("h2o2_svp_lda")
+    bf = xc.BasisFunctionOnGridController(ctx, b, g, highest_derivative=0)
+    with pytest.raises(xc.XCError, match="AO gradients"):
+        xc.FuncPotential(bf, "PBE").getMatrix(P)
+    with pytest.raises(xc.XCError, match="keep_fp32"):
+        xc.FuncPotential(bf, "LDA", "FP32").getMatrix(P)
+    with pytest.raises(xc.XCError, match="density matrix"):
+        xc.FuncPotential(bf, "LDA").getMatrix(P[:-1, :-1])
+    with pytest.raises(xc.XCError, match="max block size"):
+        xc.BasisFunctionOnGridController(ctx, b, g, max_block_size=256)
+    # a failed call leaves the context usable
+    F = xc.FuncPotential(bf, "LDA").getMatrix(P)
+    assert np.isfinite(F).all()
