@@ -59,6 +59,8 @@ int xc_version(void);
 /* ---- context ------------------------------------------------------------ */
 /* Creates a context on HIP device `device` with its own (default-priority) stream. */
 int xc_ctx_create(int device, xc_ctx** out);
+/* Fails with XC_ERR_STATE while basis, grid or controller handles of the
+ * context are alive: destroy those first. */
 int xc_ctx_destroy(xc_ctx* ctx);
 int xc_ctx_synchronize(xc_ctx* ctx);
 /* The hipStream_t every kernel of this context is launched on. */

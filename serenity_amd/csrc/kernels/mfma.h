@@ -2,10 +2,12 @@
 // K = 4 step, one A and one B element per lane and four accumulator elements
 // per lane, in FP64 (v_mfma_f64_16x16x4_f64) and FP32 (v_mfma_f32_16x16x4_f32).
 //
-// Lane layout of one 16x16x4 step for a 64-wide wavefront:
+// Lane layout of one 16x16x4 step for a 64-wide wavefront (measured on gfx950):
 //   A (16 x 4):  lane l holds A[l % 16][l / 16]
 //   B (4 x 16):  lane l holds B[l / 16][l % 16]
-//   D (16 x 16): lane l, element r holds D[4 * (l / 16) + r][l % 16]
+//   D (16 x 16): lane l, element r holds D[row(l, r)][l % 16] with
+//                row = l / 16 + 4 r        for the FP64 instruction,
+//                row = 4 (l / 16) + r      for the FP32 instruction.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,6 +27,7 @@ struct Mfma<double> {
   __device__ static inline acc_t step(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
 template <>
@@ -34,6 +37,7 @@ struct Mfma<float> {
   __device__ static inline acc_t step(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 }  // namespace sxc

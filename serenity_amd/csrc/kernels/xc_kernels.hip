@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void density_kernel(const DevBlock* __restrict
     }
     for (int r = 0; r < 2; ++r)
       for (int c = 0; c < 2; ++c)
-        for (int i = 0; i < 4; ++i) buf[16 * c + (lane & 15)][32 * w + 16 * r + 4 * (lane >> 4) + i] = acc[r][c][i];
+        for (int i = 0; i < 4; ++i) buf[16 * c + (lane & 15)][32 * w + 16 * r + M::row(lane, i)] = acc[r][c][i];
     __syncthreads();
     for (int j = 0; j < 16; ++j) {
       const int nu = 16 * h + j;
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void fock_kernel(const DevBlock* __restrict__ 
       for (int r = 0; r < 2; ++r)
         for (int c = 0; c < 2; ++c)
           for (int i = 0; i < 4; ++i) {
-            const int row = 64 * I + rowbase + 16 * r + 4 * (lane >> 4) + i;
+            const int row = 64 * I + rowbase + 16 * r + M::row(lane, i);
             const int col = 64 * J + colbase + 16 * c + (lane & 15);
             if (row >= npad || col >= npad) continue;
             const int gr = ao_idx[bk.idx_off + row], gc = ao_idx[bk.idx_off + col];

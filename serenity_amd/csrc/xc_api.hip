@@ -67,6 +67,7 @@ struct xc_ctx {
   double* d_energy = nullptr;
   XcParams prm;
   std::map<void*, size_t> user_allocs;  // buffers handed out by xc_device_alloc
+  int live_handles = 0;                 // bases, grids and controllers not yet destroyed
   // stage timing: XC_NSTAGES + 1 events per profiled build
   bool profile = false;
   std::vector<hipEvent_t> ev_pool;
@@ -187,6 +188,8 @@ int xc_ctx_create(int device, xc_ctx** out) {
 int xc_ctx_destroy(xc_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
+    require(ctx->live_handles == 0, XC_ERR_STATE,
+            "xc_ctx_destroy: " + std::to_string(ctx->live_handles) + " basis/grid/controller handles still alive");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -388,12 +391,17 @@ int xc_basis_create(xc_ctx* ctx, int nshell, const int* shell_l, const int* shel
     b->d_coef.upload(ctx, b->coef.data(), b->coef.size());
     b->d_ang.upload(ctx, b->ang.data(), b->ang.size());
     HIPX(hipStreamSynchronize(ctx->stream));
+    ++ctx->live_handles;
     *out = b;
   });
 }
 
 int xc_basis_destroy(xc_basis* basis) {
-  return guarded([&] { delete basis; });
+  return guarded([&] {
+    if (!basis) return;
+    --basis->ctx->live_handles;
+    delete basis;
+  });
 }
 
 int xc_basis_nao(const xc_basis* basis, int* nao) {
@@ -413,12 +421,17 @@ int xc_grid_create(xc_ctx* ctx, int64_t npoints, const double* points, const dou
     g->d_pts.upload(ctx, points, (size_t)(3 * npoints));
     g->d_w.upload(ctx, weights, (size_t)npoints);
     HIPX(hipStreamSynchronize(ctx->stream));
+    ++ctx->live_handles;
     *out = g;
   });
 }
 
 int xc_grid_destroy(xc_grid* grid) {
-  return guarded([&] { delete grid; });
+  return guarded([&] {
+    if (!grid) return;
+    --grid->ctx->live_handles;
+    delete grid;
+  });
 }
 
 int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double radial_threshold,
@@ -530,6 +543,7 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       delete bf;
       throw;
     }
+    ++ctx->live_handles;
     *out = bf;
   });
 }
@@ -538,6 +552,7 @@ int xc_bfogc_destroy(xc_bfogc* bfogc) {
   return guarded([&] {
     if (!bfogc) return;
     (void)hipStreamSynchronize(bfogc->ctx->stream);
+    --bfogc->ctx->live_handles;
     delete bfogc;
   });
 }

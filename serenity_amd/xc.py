@@ -20,6 +20,7 @@ arrays, addressed in column-major (Fortran) order like Eigen's.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -57,6 +58,10 @@ class Context:
         check(L.xc_ctx_create(int(device), C.byref(h)))
         self.handle = h
         self.device = device
+        self._children = weakref.WeakSet()   # device handles created on this context
+
+    def _register(self, obj):
+        self._children.add(obj)
 
     @property
     def stream(self):
@@ -98,9 +103,14 @@ class Context:
         return dict(zip(_lib.STAGES, ms.tolist())), nb.value
 
     def close(self):
-        if self.handle:
-            _lib.lib().xc_ctx_destroy(self.handle)
-            self.handle = None
+        """Release every handle created on this context (controllers first), then the context."""
+        if not self.handle:
+            return
+        kids = list(self._children)
+        for k in sorted(kids, key=lambda o: 0 if isinstance(o, BasisFunctionOnGridController) else 1):
+            k._release()
+        check(_lib.lib().xc_ctx_destroy(self.handle))
+        self.handle = None
 
 
 class BasisController:
@@ -206,37 +216,45 @@ def lebedev(npoints):
     return x, y, z, w
 
 
-class _DeviceBasis:
+class _Handle:
+    """A device handle owned by a Context; released explicitly or when collected."""
+    _destroy = None
+
+    def _release(self):
+        h, self.handle = getattr(self, "handle", None), None
+        if h:
+            getattr(_lib.lib(), self._destroy)(h)
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+
+class _DeviceBasis(_Handle):
+    _destroy = "xc_basis_destroy"
+
     def __init__(self, ctx, basis):
         h = C.c_void_p()
         check(_lib.lib().xc_basis_create(ctx.handle, basis.nshell, _i(basis.l), _i(basis.pure), _i(basis.nprim),
                                          _d(basis.center), _d(basis.alpha), _d(basis.coef), C.byref(h)))
         self.handle = h
-
-    def __del__(self):
-        try:
-            if self.handle:
-                _lib.lib().xc_basis_destroy(self.handle)
-        except Exception:
-            pass
+        ctx._register(self)
 
 
-class _DeviceGrid:
+class _DeviceGrid(_Handle):
+    _destroy = "xc_grid_destroy"
+
     def __init__(self, ctx, grid):
         h = C.c_void_p()
         check(_lib.lib().xc_grid_create(ctx.handle, grid.getNGridPoints(), _d(grid.points), _d(grid.weights),
                                         C.byref(h)))
         self.handle = h
-
-    def __del__(self):
-        try:
-            if self.handle:
-                _lib.lib().xc_grid_destroy(self.handle)
-        except Exception:
-            pass
+        ctx._register(self)
 
 
-class BasisFunctionOnGridController:
+class BasisFunctionOnGridController(_Handle):
     """Screened AO values (and gradients) of every grid block, resident in HBM.
 
     ``BasisFunctionOnGridController(basis, grid, maxBlockSize, radialThreshold,
@@ -256,6 +274,7 @@ class BasisFunctionOnGridController:
                                          float(radial_threshold), int(highest_derivative), int(bool(keep_fp32)),
                                          C.byref(h)))
         self.handle = h
+        ctx._register(self)
         self.highest_derivative = highest_derivative
         self.keep_fp32 = keep_fp32
         nb, npts, nao, mbs = C.c_int(), C.c_int64(), C.c_int(), C.c_int()
@@ -301,12 +320,7 @@ class BasisFunctionOnGridController:
         out["derivativeValues"] = tuple(g.reshape(n, B).T for g in grads) if grads else None
         return out
 
-    def __del__(self):
-        try:
-            if self.handle:
-                _lib.lib().xc_bfogc_destroy(self.handle)
-        except Exception:
-            pass
+    _destroy = "xc_bfogc_destroy"
 
 
 class DensityOnGridCalculator:
