@@ -38,6 +38,22 @@ struct DevBlock {
   int idx_off;        // offset into ao_idx
   int shell_off;      // offset into the significant-shell list
   int nshell;         // significant shells
+  int npass;          // density column passes (partial rho/grad slots written)
+  int pad_;
+};
+
+// Size classes of the macro-tile Fock kernel (blocks too wide for the pair
+// kernel), chosen from npad: class 0: npad <= 32, class 1: <= 64, class 2: 128-wide macro-tiles.
+constexpr int kNumClasses = 3;
+inline int size_class(int npad) { return npad <= 32 ? 0 : (npad <= 64 ? 1 : 2); }
+inline int class_width(int cls) { return cls == 0 ? 32 : (cls == 1 ? 64 : 128); }
+// Density passes hold NCT = 1..8 column tiles of 16 (pass width <= 128).
+constexpr int kMaxNct = 8;
+
+// Work item of the contraction kernels: block and two column offsets (density:
+// a = first column of the pass; fock: a, b = first rows of the macro-tile pair).
+struct WorkItem {
+  int block, a, b, pass;
 };
 
 // ---- launchers (xc_kernels.hip) -------------------------------------------
@@ -48,23 +64,39 @@ void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const in
                     const DevShell* shells, const double* alpha, const double* coef, const double* ang,
                     const double* pts, double radial_thr, double exp_thr, int ncomp, double* tiles64, float* tiles32);
 
+// Partial densities: part[(pass * 4 + c) * npts + q], c = rho, d/dx, d/dy, d/dz
+// (gradient slots hold sum T grad phi without the factor 2).
 template <typename T>
-void launch_density(hipStream_t s, const DevBlock* blocks, int nblocks, const int* ao_idx, const T* tiles,
-                    int ncomp, const T* P, int n, int want_grad, double* rho, double* grad, int64_t npts);
+void launch_density(hipStream_t s, int nct, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
+                    const T* tiles, const T* P, int n, int want_grad, double* part, int64_t npts);
 
+// rho[q] / grad[.][q] from the partials of the block of q (any block size).
+void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
+                            int want_grad, double* rho, double* grad);
+
+// Functional on rho/grad, or (part != nullptr) on the partials of 128-point blocks.
 void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* rho,
-                       const double* grad, const double* w, double* eps, double* v, double* g, double* vs,
-                       double* gw, double* e_blk);
+                       const double* grad, const DevBlock* blocks, const double* part, const double* w, double* eps,
+                       double* v, double* g, double* vs, double* gw, double* e_blk);
 
 template <typename T>
-void launch_fock(hipStream_t s, const DevBlock* blocks, int nblocks, const int* ao_idx, const T* tiles, int ncomp,
-                 const double* vs, const double* gw, int gga, double block_ave_thr, double* U, int n, int64_t npts);
+void launch_fock(hipStream_t s, int cls, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
+                 const T* tiles, const double* vs, const double* gw, int gga, double block_ave_thr, double* U, int n,
+                 int64_t npts);
+
+// Pair-tile Fock kernel; work item: block, a = first pair, b = pair count.
+template <typename T>
+void launch_fock_pairs(hipStream_t s, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
+                       const T* tiles, const double* vs, const double* gw, int gga, double block_ave_thr, double* U,
+                       int n, int64_t npts);
+int fock_pairs_rmax(int fp64);  // rows a pair-kernel work item may stage
+int fock_pairs_maxp(int fp64);  // pairs per wavefront
 
 void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
                              double* vs, double* gw);
 void launch_fill(hipStream_t s, double* p, int64_t n, double value);
 void launch_to_float(hipStream_t s, const double* src, float* dst, int64_t n);
-// F[i + n j] (+)= U[min(i,j) + n max(i,j)]
+// F[i + n j] (+)= U[max(i,j) + n min(i,j)] (U holds the lower triangle)
 void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accumulate);
 void launch_sum(hipStream_t s, const double* x, int64_t n, double* out);
 

@@ -146,67 +146,122 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
   }
 }
 
-// One workgroup (4 wavefronts) per grid block.  For every group of 32 AO
-// columns nu: T[p][nu] = sum_mu phi[mu][p] P[mu][nu] with wave w owning points
-// 32w..32w+31 (2 x 2 MFMA tiles), phi and the gathered P chunk staged in LDS;
-// then each thread dots its point's T row with phi / grad phi.
-template <typename T>
-__global__ __launch_bounds__(256) void density_kernel(const DevBlock* __restrict__ blocks, const int* __restrict__ ao_idx,
-                                                      const T* __restrict__ tiles, const T* __restrict__ P, int n,
-                                                      int want_grad, double* __restrict__ rho,
-                                                      double* __restrict__ grad, int64_t npts) {
+// Density contraction.  One workgroup (4 wavefronts) per (block, column pass):
+// T[p][nu] = sum_mu phi[mu][p] P[mu][nu] for the pass's NC = 16 NCT columns,
+// wave w owning points 32w..32w+31 (2 x NCT MFMA tiles in registers).  The AO
+// (K) dimension streams through two LDS buffers 16 rows at a time: the global
+// loads of chunk k+1 are issued before the MFMAs of chunk k and written to the
+// other buffer after them.  The pass's partial rho and sum T grad phi are then
+// formed by dotting T (transposed through LDS) with phi / grad phi read
+// coalesced from the tiles, and written to the pass's partial slot.
+template <typename T, int NCT>
+__global__ __launch_bounds__(256) void density_kernel(const WorkItem* __restrict__ work,
+                                                      const DevBlock* __restrict__ blocks,
+                                                      const int* __restrict__ ao_idx, const T* __restrict__ tiles,
+                                                      const T* __restrict__ P, int n, int want_grad,
+                                                      double* __restrict__ part, int64_t npts) {
   using M = Mfma<T>;
-  __shared__ T Pc[32][32];
-  __shared__ T buf[32][kTile];
+  constexpr int NC = 16 * NCT;
+  constexpr int KC = 16;
+  constexpr int PHI_PER_T = KC * kTile / 256;
+  constexpr int P_PER_T = KC * NC / 256;
+  // rows padded by 16 elements (bank spread of the MFMA operand reads)
+  constexpr int PS = kTile + 16, QS = NC + 16;
+  __shared__ T sPhi[2][KC][PS];
+  __shared__ T sP[2][KC][QS];
   __shared__ double red[4][kTile];
-  const DevBlock bk = blocks[blockIdx.x];
+  const WorkItem wk = work[blockIdx.x];
+  const DevBlock bk = blocks[wk.block];
+  const int col0 = wk.a;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int p = tid & (kTile - 1), h = tid >> 7;
   const int npad = bk.npad;
   const T* tb = tiles + bk.tile_off;
+  const int* idx = ao_idx + bk.idx_off;
   const int64_t cstride = (int64_t)npad * kTile;
-  double racc = 0.0, gxa = 0.0, gya = 0.0, gza = 0.0;
-  for (int g = 0; g < npad; g += 32) {
-    typename M::acc_t acc[2][2];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c) acc[r][c] = M::zero();
-    for (int k0 = 0; k0 < npad; k0 += 32) {
-      for (int e = tid; e < 32 * 32; e += 256) {
-        const int i = e & 31, j = e >> 5;
-        const int mu = ao_idx[bk.idx_off + k0 + i];
-        const int nu = ao_idx[bk.idx_off + g + j];
-        Pc[i][j] = (mu >= 0 && nu >= 0) ? P[mu + (int64_t)n * nu] : T(0);
-      }
-      const T* src = tb + (int64_t)k0 * kTile;
-      for (int e = tid; e < 32 * kTile; e += 256) buf[e / kTile][e % kTile] = src[e];
-      __syncthreads();
+
+  typename M::acc_t acc[2][NCT];
 #pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
-        const int kk = ks * 4 + (lane >> 4);
-        const T a0 = buf[kk][32 * w + (lane & 15)];
-        const T a1 = buf[kk][32 * w + 16 + (lane & 15)];
-        const T b0 = Pc[kk][lane & 15];
-        const T b1 = Pc[kk][16 + (lane & 15)];
-        acc[0][0] = M::step(a0, b0, acc[0][0]);
-        acc[0][1] = M::step(a0, b1, acc[0][1]);
-        acc[1][0] = M::step(a1, b0, acc[1][0]);
-        acc[1][1] = M::step(a1, b1, acc[1][1]);
-      }
-      __syncthreads();
+  for (int c = 0; c < NCT; ++c) acc[0][c] = acc[1][c] = M::zero();
+
+  // this thread's P-chunk elements: row i_loc of the chunk, columns tid/16 + 16 j
+  const int i_loc = tid & 15;
+  int nu_g[P_PER_T];
+#pragma unroll
+  for (int j = 0; j < P_PER_T; ++j) {
+    const int col = col0 + (tid >> 4) + 16 * j;
+    nu_g[j] = col < npad ? idx[col] : -1;
+  }
+  T rPhi[PHI_PER_T], rP[P_PER_T];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < PHI_PER_T; ++j) {
+      const int e = tid + 256 * j;
+      rPhi[j] = tb[(int64_t)(k0 + e / kTile) * kTile + (e % kTile)];
     }
+    const int mu = idx[k0 + i_loc];
+#pragma unroll
+    for (int j = 0; j < P_PER_T; ++j) rP[j] = (mu >= 0 && nu_g[j] >= 0) ? P[mu + (int64_t)n * nu_g[j]] : T(0);
+  };
+  auto store = [&](int b) {
+#pragma unroll
+    for (int j = 0; j < PHI_PER_T; ++j) {
+      const int e = tid + 256 * j;
+      sPhi[b][e / kTile][e % kTile] = rPhi[j];
+    }
+#pragma unroll
+    for (int j = 0; j < P_PER_T; ++j) sP[b][i_loc][(tid >> 4) + 16 * j] = rP[j];
+  };
+
+  const int nch = npad / KC;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int b = ch & 1;
+    if (ch + 1 < nch) load((ch + 1) * KC);
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int kk = ks * 4 + (lane >> 4);
+      const T a0 = sPhi[b][kk][32 * w + (lane & 15)];
+      const T a1 = sPhi[b][kk][32 * w + 16 + (lane & 15)];
+#pragma unroll
+      for (int c = 0; c < NCT; ++c) {
+        const T bv = sP[b][kk][16 * c + (lane & 15)];
+        acc[0][c] = M::step(a0, bv, acc[0][c]);
+        acc[1][c] = M::step(a1, bv, acc[1][c]);
+      }
+    }
+    if (ch + 1 < nch) store(b ^ 1);
+    __syncthreads();
+  }
+
+  // dots with phi and grad phi, 32 columns at a time through LDS (aliases sPhi)
+  T(*sT)[PS] = reinterpret_cast<T(*)[PS]>(&sPhi[0][0][0]);
+  double racc = 0.0, gxa = 0.0, gya = 0.0, gza = 0.0;
+#pragma unroll
+  for (int cg = 0; cg < NCT; cg += 2) {
+#pragma unroll
     for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c)
-        for (int i = 0; i < 4; ++i) buf[16 * c + (lane & 15)][32 * w + 16 * r + M::row(lane, i)] = acc[r][c][i];
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        if (cg + cc >= NCT) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sT[16 * cc + (lane & 15)][32 * w + 16 * r + M::row(lane, i)] = acc[r][cg + cc][i];
+      }
     __syncthreads();
     for (int j = 0; j < 16; ++j) {
-      const int nu = 16 * h + j;
-      const double tv = (double)buf[nu][p];
-      const int64_t row = (int64_t)(g + nu) * kTile + p;
-      racc += tv * (double)tb[row];
-      if (want_grad) {
-        gxa += tv * (double)tb[row + cstride];
-        gya += tv * (double)tb[row + 2 * cstride];
-        gza += tv * (double)tb[row + 3 * cstride];
+      const int nl = 16 * h + j;
+      const int col = col0 + 16 * cg + nl;
+      if (16 * cg + nl < NC && col < npad) {
+        const double tv = (double)sT[nl][p];
+        const int64_t row = (int64_t)col * kTile + p;
+        racc += tv * (double)tb[row];
+        if (want_grad) {
+          gxa += tv * (double)tb[row + cstride];
+          gya += tv * (double)tb[row + 2 * cstride];
+          gza += tv * (double)tb[row + 3 * cstride];
+        }
       }
     }
     __syncthreads();
@@ -220,12 +275,37 @@ __global__ __launch_bounds__(256) void density_kernel(const DevBlock* __restrict
   __syncthreads();
   if (h == 0 && p < bk.npts) {
     const int64_t q = bk.first + p;
-    rho[q] = racc + red[0][p];
+    double* dst = part + (int64_t)wk.pass * 4 * npts + q;
+    dst[0] = racc + red[0][p];
     if (want_grad) {
-      grad[q] = 2.0 * (gxa + red[1][p]);
-      grad[npts + q] = 2.0 * (gya + red[2][p]);
-      grad[2 * npts + q] = 2.0 * (gza + red[3][p]);
+      dst[npts] = gxa + red[1][p];
+      dst[2 * npts] = gya + red[2][p];
+      dst[3 * npts] = gza + red[3][p];
     }
+  }
+}
+
+__global__ void reduce_partials_kernel(const DevBlock* __restrict__ blocks, int bs, int64_t npts,
+                                       const double* __restrict__ part, int want_grad, double* __restrict__ rho,
+                                       double* __restrict__ grad) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npts) return;
+  const int np = blocks[q / bs].npass;
+  double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
+  for (int s = 0; s < np; ++s) {
+    const double* src = part + (int64_t)s * 4 * npts + q;
+    r += src[0];
+    if (want_grad) {
+      x += src[npts];
+      y += src[2 * npts];
+      z += src[3 * npts];
+    }
+  }
+  rho[q] = r;
+  if (want_grad) {
+    grad[q] = 2.0 * x;
+    grad[npts + q] = 2.0 * y;
+    grad[2 * npts + q] = 2.0 * z;
   }
 }
 
@@ -246,6 +326,8 @@ __device__ inline double block_sum_128(double x, double* red) {
 __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm, int64_t npts, int bs,
                                                          const double* __restrict__ rho,
                                                          const double* __restrict__ grad,
+                                                         const DevBlock* __restrict__ blocks,
+                                                         const double* __restrict__ part,
                                                          const double* __restrict__ w, double* __restrict__ eps,
                                                          double* __restrict__ v, double* __restrict__ g,
                                                          double* __restrict__ vs, double* __restrict__ gw,
@@ -256,17 +338,37 @@ __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm,
   const int B = (int)min((int64_t)bs, npts - first);
   const int64_t q = first + t;
   const bool in = t < B;
-  const double r = in ? rho[q] : 0.0;
+  const bool gga = functional_is_gga(func);
+  double r = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  if (in) {
+    if (part) {
+      // sum the density passes of this block (blocks of 128 points)
+      const int np = blocks[b].npass;
+      for (int s = 0; s < np; ++s) {
+        const double* src = part + (int64_t)s * 4 * npts + q;
+        r += src[0];
+        if (gga) {
+          gx += src[npts];
+          gy += src[2 * npts];
+          gz += src[3 * npts];
+        }
+      }
+      gx *= 2.0;
+      gy *= 2.0;
+      gz *= 2.0;
+    } else {
+      r = rho[q];
+      if (gga && grad) {
+        gx = grad[q];
+        gy = grad[npts + q];
+        gz = grad[2 * npts + q];
+      }
+    }
+  }
   const double sumabs = block_sum_128(fabs(r), red);
   const bool skip = sumabs < B * 1e-12;
-  const bool gga = functional_is_gga(func);
   double e = 0.0, vv = 0.0, gg[3] = {0.0, 0.0, 0.0};
-  if (in && !skip) {
-    const double gx = (gga && grad) ? grad[q] : 0.0;
-    const double gy = (gga && grad) ? grad[npts + q] : 0.0;
-    const double gz = (gga && grad) ? grad[2 * npts + q] : 0.0;
-    e = xc_point(func, r, gx, gy, gz, prm, &vv, gg);
-  }
+  if (in && !skip) e = xc_point(func, r, gx, gy, gz, prm, &vv, gg);
   const double wq = in ? w[q] : 0.0;
   if (in) {
     if (eps) eps[q] = e;
@@ -287,26 +389,35 @@ __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm,
   if (t == 0) e_blk[b] = eb;
 }
 
-// One workgroup per grid block.  Output super-tiles of 64 x 64 compact AOs
-// (I <= J); wave w owns the 32 x 32 quadrant (w / 2, w % 2) as 2 x 2 MFMA
-// tiles.  S_IJ = phi_I^T G_J + G_I^T phi_J over the block's points (K = 128
-// in chunks of 32), with G = v_w phi / 2 + sum_k g_w,k d_k phi built while
-// staging.  S_IJ goes into the upper triangle of U with FP64 atomics.
-template <typename T>
-__global__ __launch_bounds__(256) void fock_kernel(const DevBlock* __restrict__ blocks, const int* __restrict__ ao_idx,
-                                                   const T* __restrict__ tiles, const double* __restrict__ vs,
-                                                   const double* __restrict__ gw, int gga, double thr,
-                                                   double* __restrict__ U, int n, int64_t npts) {
+// Fock contraction.  One workgroup per (block, macro-tile pair I <= J) of edge
+// MS = 32 WT AOs; wave w owns the (w / 2, w % 2) quadrant as WT x WT MFMA
+// tiles in registers.  The block's points (K = 128) stream through two LDS
+// buffers 8 at a time, G = (w v / 2) phi + sum_k (w g_k) d_k phi being formed
+// while staging.  S_IJ = phi_I^T G_J + G_I^T phi_J is added with FP64 atomics
+// into the lower triangle of U: entry (row, col) with row <= col goes to
+// U[col + n row], so the 16 lanes of a row group hit consecutive addresses
+// (diagonal macro-tiles: only entries with row <= col).
+template <typename T, int WT>
+__global__ __launch_bounds__(256) void fock_kernel(const WorkItem* __restrict__ work, const DevBlock* __restrict__ blocks,
+                                                   const int* __restrict__ ao_idx, const T* __restrict__ tiles,
+                                                   const double* __restrict__ vs, const double* __restrict__ gw,
+                                                   int gga, double thr, double* __restrict__ U, int n, int64_t npts) {
   using M = Mfma<T>;
-  __shared__ T sPI[32][64], sGI[32][64], sPJ[32][64], sGJ[32][64];
+  constexpr int MS = 32 * WT;
+  constexpr int KC = 8;
+  constexpr int EPS = MS * KC / 256;
+  constexpr int MSP = MS + 16;  // padded rows (LDS bank spread)
+  __shared__ T sPI[2][KC][MSP], sGI[2][KC][MSP], sPJ[2][KC][MSP], sGJ[2][KC][MSP];
   __shared__ double spot[4][kTile];
   __shared__ double red[kTile];
-  const DevBlock bk = blocks[blockIdx.x];
-  if (bk.nb == 0) return;
+  const WorkItem wk = work[blockIdx.x];
+  const DevBlock bk = blocks[wk.block];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int npad = bk.npad;
   const T* tb = tiles + bk.tile_off;
   const int64_t cstride = (int64_t)npad * kTile;
+  const int I0 = wk.a, J0 = wk.b;
+  const bool diag = I0 == J0;
   double part = 0.0;
   if (tid < kTile) {
     const bool in = tid < bk.npts;
@@ -316,84 +427,285 @@ __global__ __launch_bounds__(256) void fock_kernel(const DevBlock* __restrict__ 
     spot[2][tid] = (in && gga) ? gw[npts + q] : 0.0;
     spot[3][tid] = (in && gga) ? gw[2 * npts + q] : 0.0;
     part = fabs(spot[0][tid]) + fabs(spot[1][tid]) + fabs(spot[2][tid]) + fabs(spot[3][tid]);
+    red[tid] = part;
   }
-  // block average of |w v| (+ |w g|) for the prescreening; all threads see the same value
-  if (tid < kTile) red[tid] = part;
   __syncthreads();
   for (int s = 64; s > 0; s >>= 1) {
     if (tid < s) red[tid] += red[tid + s];
     __syncthreads();
   }
-  const double average = red[0] / bk.npts;
-  if (average < thr) return;
+  // block average of |w v| (+ |w g|) for the prescreening; uniform over the workgroup
+  if (red[0] / bk.npts < thr) return;
 
-  const int ngroups = (npad + 63) / 64;
-  const int rowbase = 32 * (w >> 1), colbase = 32 * (w & 1);
-  for (int I = 0; I < ngroups; ++I) {
-    for (int J = I; J < ngroups; ++J) {
-      typename M::acc_t acc[2][2];
-      for (int r = 0; r < 2; ++r)
-        for (int c = 0; c < 2; ++c) acc[r][c] = M::zero();
-      for (int p0 = 0; p0 < kTile; p0 += 32) {
-        for (int e = tid; e < 64 * 32; e += 256) {
-          const int i = e >> 5, pp = e & 31;
-          const int pt = p0 + pp;
-          const double sv = 0.5 * spot[0][pt];
+  const int pp = tid & (KC - 1);
+  T rI[EPS][4], rJ[EPS][4];
+  auto load = [&](int p0) {
 #pragma unroll
-          for (int side = 0; side < 2; ++side) {
-            const int mu = 64 * (side ? J : I) + i;
-            T phi = T(0), G = T(0);
-            if (mu < npad) {
-              const T* src = tb + (int64_t)mu * kTile + pt;
-              phi = src[0];
-              double gv = sv * (double)phi;
-              if (gga)
-                gv += spot[1][pt] * (double)src[cstride] + spot[2][pt] * (double)src[2 * cstride] +
-                      spot[3][pt] * (double)src[3 * cstride];
-              G = (T)gv;
-            }
-            if (side) {
-              sPJ[pp][i] = phi;
-              sGJ[pp][i] = G;
-            } else {
-              sPI[pp][i] = phi;
-              sGI[pp][i] = G;
-            }
-          }
-        }
-        __syncthreads();
+    for (int j = 0; j < EPS; ++j) {
+      const int i = (tid >> 3) + 32 * j;
 #pragma unroll
-        for (int ks = 0; ks < 8; ++ks) {
-          const int kk = ks * 4 + (lane >> 4);
-          T aP[2], aG[2], bP[2], bG[2];
-          for (int r = 0; r < 2; ++r) {
-            aP[r] = sPI[kk][rowbase + 16 * r + (lane & 15)];
-            aG[r] = sGI[kk][rowbase + 16 * r + (lane & 15)];
+      for (int side = 0; side < 2; ++side) {
+        if (side == 1 && diag) break;
+        T(&r)[4] = side ? rJ[j] : rI[j];
+        const int mu = (side ? J0 : I0) + i;
+        if (mu < npad) {
+          const T* src = tb + (int64_t)mu * kTile + p0 + pp;
+          r[0] = src[0];
+          if (gga) {
+            r[1] = src[cstride];
+            r[2] = src[2 * cstride];
+            r[3] = src[3 * cstride];
           }
-          for (int c = 0; c < 2; ++c) {
-            bP[c] = sPJ[kk][colbase + 16 * c + (lane & 15)];
-            bG[c] = sGJ[kk][colbase + 16 * c + (lane & 15)];
-          }
-          for (int r = 0; r < 2; ++r)
-            for (int c = 0; c < 2; ++c) {
-              acc[r][c] = M::step(aP[r], bG[c], acc[r][c]);
-              acc[r][c] = M::step(aG[r], bP[c], acc[r][c]);
-            }
+        } else {
+          r[0] = r[1] = r[2] = r[3] = T(0);
         }
-        __syncthreads();
       }
-      for (int r = 0; r < 2; ++r)
-        for (int c = 0; c < 2; ++c)
-          for (int i = 0; i < 4; ++i) {
-            const int row = 64 * I + rowbase + 16 * r + M::row(lane, i);
-            const int col = 64 * J + colbase + 16 * c + (lane & 15);
-            if (row >= npad || col >= npad) continue;
-            const int gr = ao_idx[bk.idx_off + row], gc = ao_idx[bk.idx_off + col];
-            if (gr < 0 || gc < 0) continue;
-            if (I == J && gr > gc) continue;
-            unsafeAtomicAdd(U + gr + (int64_t)n * gc, (double)acc[r][c][i]);
-          }
     }
+  };
+  auto store = [&](int b, int p0) {
+    const int pt = p0 + pp;
+    const double sv = 0.5 * spot[0][pt];
+#pragma unroll
+    for (int j = 0; j < EPS; ++j) {
+      const int i = (tid >> 3) + 32 * j;
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        if (side == 1 && diag) break;
+        const T(&r)[4] = side ? rJ[j] : rI[j];
+        double gv = sv * (double)r[0];
+        if (gga) gv += spot[1][pt] * (double)r[1] + spot[2][pt] * (double)r[2] + spot[3][pt] * (double)r[3];
+        if (side) {
+          sPJ[b][pp][i] = r[0];
+          sGJ[b][pp][i] = (T)gv;
+        } else {
+          sPI[b][pp][i] = r[0];
+          sGI[b][pp][i] = (T)gv;
+        }
+      }
+    }
+  };
+
+  typename M::acc_t acc[WT][WT];
+#pragma unroll
+  for (int r = 0; r < WT; ++r)
+#pragma unroll
+    for (int c = 0; c < WT; ++c) acc[r][c] = M::zero();
+  const int rowbase = (w >> 1) * 16 * WT, colbase = (w & 1) * 16 * WT;
+  constexpr int NCH = kTile / KC;
+  load(0);
+  store(0, 0);
+  __syncthreads();
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int b = ch & 1;
+    if (ch + 1 < NCH) load((ch + 1) * KC);
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int kk = ks * 4 + (lane >> 4);
+      T aP[WT], aG[WT], bP[WT], bG[WT];
+#pragma unroll
+      for (int r = 0; r < WT; ++r) {
+        aP[r] = sPI[b][kk][rowbase + 16 * r + (lane & 15)];
+        aG[r] = sGI[b][kk][rowbase + 16 * r + (lane & 15)];
+      }
+#pragma unroll
+      for (int c = 0; c < WT; ++c) {
+        bP[c] = diag ? sPI[b][kk][colbase + 16 * c + (lane & 15)] : sPJ[b][kk][colbase + 16 * c + (lane & 15)];
+        bG[c] = diag ? sGI[b][kk][colbase + 16 * c + (lane & 15)] : sGJ[b][kk][colbase + 16 * c + (lane & 15)];
+      }
+#pragma unroll
+      for (int r = 0; r < WT; ++r)
+#pragma unroll
+        for (int c = 0; c < WT; ++c) {
+          acc[r][c] = M::step(aP[r], bG[c], acc[r][c]);
+          acc[r][c] = M::step(aG[r], bP[c], acc[r][c]);
+        }
+    }
+    if (ch + 1 < NCH) store(b ^ 1, (ch + 1) * KC);
+    __syncthreads();
+  }
+  const int* idx = ao_idx + bk.idx_off;
+#pragma unroll
+  for (int r = 0; r < WT; ++r)
+#pragma unroll
+    for (int c = 0; c < WT; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = I0 + rowbase + 16 * r + M::row(lane, i);
+        const int col = J0 + colbase + 16 * c + (lane & 15);
+        if (row >= npad || col >= npad) continue;
+        const int gr = idx[row], gc = idx[col];
+        if (gr < 0 || gc < 0) continue;
+        if (diag && gr > gc) continue;
+        unsafeAtomicAdd(U + gc + (int64_t)n * gr, (double)acc[r][c][i]);
+      }
+}
+
+// Fock contraction over the upper triangle of 32 x 32 AO tile pairs (I <= J,
+// enumerated row by row).  A work item holds up to 4 MAXP consecutive pairs of
+// one block; wave w takes pairs w, w + 4, ... with 2 x 2 MFMA tiles each in
+// registers.  Per chunk of 8 points the workgroup stages phi and G for the AO
+// rows [row0, npad) of the block (row0: first row tile of the item) through
+// two LDS buffers, G = (w v / 2) phi + sum_k (w g_k) d_k phi formed while
+// staging.  Only blocks with npad - row0 <= RMAX rows use this kernel.
+template <typename T>
+struct FockPairsCfg {
+  static constexpr int RMAX = sizeof(T) == 8 ? 256 : 512;
+  static constexpr int MAXP = sizeof(T) == 8 ? 4 : 8;
+};
+
+__device__ inline void decode_pair(int idx, int nt, int& I, int& J) {
+  // row I starts at I * nt - I (I - 1) / 2
+  I = 0;
+  int start = 0;
+  while (start + (nt - I) <= idx) {
+    start += nt - I;
+    ++I;
+  }
+  J = I + (idx - start);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fock_pairs_kernel(const WorkItem* __restrict__ work,
+                                                         const DevBlock* __restrict__ blocks,
+                                                         const int* __restrict__ ao_idx, const T* __restrict__ tiles,
+                                                         const double* __restrict__ vs, const double* __restrict__ gw,
+                                                         int gga, double thr, double* __restrict__ U, int n,
+                                                         int64_t npts) {
+  using M = Mfma<T>;
+  constexpr int RMAX = FockPairsCfg<T>::RMAX;
+  constexpr int MAXP = FockPairsCfg<T>::MAXP;
+  constexpr int KC = 8;
+  constexpr int JMAX = RMAX / 32;
+  // rows padded by 16 elements: the four lane groups of an MFMA operand read
+  // (consecutive rows) then fall on different LDS banks
+  constexpr int RS = RMAX + 16;
+  __shared__ T sPhi[2][KC][RS], sG[2][KC][RS];
+  __shared__ double spot[4][kTile];
+  __shared__ double red[kTile];
+  const WorkItem wk = work[blockIdx.x];
+  const DevBlock bk = blocks[wk.block];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int npad = bk.npad, nt = npad / 32;
+  const T* tb = tiles + bk.tile_off;
+  const int64_t cstride = (int64_t)npad * kTile;
+  if (tid < kTile) {
+    const bool in = tid < bk.npts;
+    const int64_t q = bk.first + tid;
+    spot[0][tid] = in ? vs[q] : 0.0;
+    spot[1][tid] = (in && gga) ? gw[q] : 0.0;
+    spot[2][tid] = (in && gga) ? gw[npts + q] : 0.0;
+    spot[3][tid] = (in && gga) ? gw[2 * npts + q] : 0.0;
+    red[tid] = fabs(spot[0][tid]) + fabs(spot[1][tid]) + fabs(spot[2][tid]) + fabs(spot[3][tid]);
+  }
+  __syncthreads();
+  for (int st = 64; st > 0; st >>= 1) {
+    if (tid < st) red[tid] += red[tid + st];
+    __syncthreads();
+  }
+  if (red[0] / bk.npts < thr) return;
+
+  int I0, J0;
+  decode_pair(wk.a, nt, I0, J0);
+  const int row0 = 32 * I0;
+  const int nrows = npad - row0;  // multiple of 32, <= RMAX
+  const int nj = nrows / 32;
+  int pI[MAXP], pJ[MAXP];
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q) {
+    const int idx = wk.a + w + 4 * q;
+    if (idx < wk.a + wk.b) {
+      decode_pair(idx, nt, pI[q], pJ[q]);
+    } else {
+      pI[q] = -1;
+      pJ[q] = -1;
+    }
+  }
+  const int pp = tid & (KC - 1);
+  T raw[JMAX][4];
+  auto load = [&](int p0) {
+#pragma unroll
+    for (int j = 0; j < JMAX; ++j) {
+      if (j < nj) {
+        const T* src = tb + (int64_t)(row0 + (tid >> 3) + 32 * j) * kTile + p0 + pp;
+        raw[j][0] = src[0];
+        if (gga) {
+          raw[j][1] = src[cstride];
+          raw[j][2] = src[2 * cstride];
+          raw[j][3] = src[3 * cstride];
+        }
+      }
+    }
+  };
+  auto store = [&](int b, int p0) {
+    const int pt = p0 + pp;
+    const double sv = 0.5 * spot[0][pt];
+#pragma unroll
+    for (int j = 0; j < JMAX; ++j) {
+      if (j < nj) {
+        double gv = sv * (double)raw[j][0];
+        if (gga) gv += spot[1][pt] * (double)raw[j][1] + spot[2][pt] * (double)raw[j][2] + spot[3][pt] * (double)raw[j][3];
+        const int lr = (tid >> 3) + 32 * j;
+        sPhi[b][pp][lr] = raw[j][0];
+        sG[b][pp][lr] = (T)gv;
+      }
+    }
+  };
+  typename M::acc_t acc[MAXP][2][2];
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q)
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[q][r][c] = M::zero();
+  constexpr int NCH = kTile / KC;
+  load(0);
+  store(0, 0);
+  __syncthreads();
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int b = ch & 1;
+    if (ch + 1 < NCH) load((ch + 1) * KC);
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int kk = ks * 4 + (lane >> 4);
+#pragma unroll
+      for (int q = 0; q < MAXP; ++q) {
+        if (pI[q] < 0) continue;
+        const int li = 32 * pI[q] - row0 + (lane & 15), lj = 32 * pJ[q] - row0 + (lane & 15);
+        const T aP0 = sPhi[b][kk][li], aP1 = sPhi[b][kk][li + 16];
+        const T aG0 = sG[b][kk][li], aG1 = sG[b][kk][li + 16];
+        const T bP0 = sPhi[b][kk][lj], bP1 = sPhi[b][kk][lj + 16];
+        const T bG0 = sG[b][kk][lj], bG1 = sG[b][kk][lj + 16];
+        acc[q][0][0] = M::step(aP0, bG0, acc[q][0][0]);
+        acc[q][0][1] = M::step(aP0, bG1, acc[q][0][1]);
+        acc[q][1][0] = M::step(aP1, bG0, acc[q][1][0]);
+        acc[q][1][1] = M::step(aP1, bG1, acc[q][1][1]);
+        acc[q][0][0] = M::step(aG0, bP0, acc[q][0][0]);
+        acc[q][0][1] = M::step(aG0, bP1, acc[q][0][1]);
+        acc[q][1][0] = M::step(aG1, bP0, acc[q][1][0]);
+        acc[q][1][1] = M::step(aG1, bP1, acc[q][1][1]);
+      }
+    }
+    if (ch + 1 < NCH) store(b ^ 1, (ch + 1) * KC);
+    __syncthreads();
+  }
+  const int* idx = ao_idx + bk.idx_off;
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q) {
+    if (pI[q] < 0) continue;
+    const bool diag = pI[q] == pJ[q];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 32 * pI[q] + 16 * r + M::row(lane, i);
+          const int col = 32 * pJ[q] + 16 * c + (lane & 15);
+          const int gr = idx[row], gc = idx[col];
+          if (gr < 0 || gc < 0) continue;
+          if (diag && gr > gc) continue;
+          unsafeAtomicAdd(U + gc + (int64_t)n * gr, (double)acc[q][r][c][i]);
+        }
   }
 }
 
@@ -425,7 +737,7 @@ __global__ void finalize_kernel(const double* __restrict__ U, double* __restrict
   if (e >= (int64_t)n * n) return;
   const int i = (int)(e % n), j = (int)(e / n);
   const int lo = i < j ? i : j, hi = i < j ? j : i;
-  const double u = U[lo + (int64_t)n * hi];
+  const double u = U[hi + (int64_t)n * lo];
   F[e] = accumulate ? F[e] + u : u;
 }
 
@@ -463,39 +775,89 @@ void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const in
 }
 
 template <typename T>
-void launch_density(hipStream_t s, const DevBlock* blocks, int nblocks, const int* ao_idx, const T* tiles, int ncomp,
-                    const T* P, int n, int want_grad, double* rho, double* grad, int64_t npts) {
-  (void)ncomp;
-  if (nblocks <= 0) return;
-  hipLaunchKernelGGL(density_kernel<T>, dim3(nblocks), dim3(256), 0, s, blocks, ao_idx, tiles, P, n, want_grad, rho,
-                     grad, npts);
+void launch_density(hipStream_t s, int nct, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
+                    const T* tiles, const T* P, int n, int want_grad, double* part, int64_t npts) {
+  if (nwork <= 0) return;
+#define SXC_DENSITY_CASE(K)                                                                                   \
+  case K:                                                                                                     \
+    hipLaunchKernelGGL((density_kernel<T, K>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, P, n, \
+                       want_grad, part, npts);                                                                \
+    break;
+  switch (nct) {
+    SXC_DENSITY_CASE(1)
+    SXC_DENSITY_CASE(2)
+    SXC_DENSITY_CASE(3)
+    SXC_DENSITY_CASE(4)
+    SXC_DENSITY_CASE(5)
+    SXC_DENSITY_CASE(6)
+    SXC_DENSITY_CASE(7)
+    default:
+      hipLaunchKernelGGL((density_kernel<T, 8>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, P, n,
+                         want_grad, part, npts);
+      break;
+  }
+#undef SXC_DENSITY_CASE
 }
-template void launch_density<double>(hipStream_t, const DevBlock*, int, const int*, const double*, int, const double*,
-                                     int, int, double*, double*, int64_t);
-template void launch_density<float>(hipStream_t, const DevBlock*, int, const int*, const float*, int, const float*,
-                                    int, int, double*, double*, int64_t);
+template void launch_density<double>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
+                                     const double*, const double*, int, int, double*, int64_t);
+template void launch_density<float>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*, const float*,
+                                    const float*, int, int, double*, int64_t);
+
+void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
+                            int want_grad, double* rho, double* grad) {
+  if (npts <= 0) return;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(grid1d(npts, 256)), dim3(256), 0, s, blocks, bs, npts, part,
+                     want_grad, rho, grad);
+}
 
 void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* rho,
-                       const double* grad, const double* w, double* eps, double* v, double* g, double* vs, double* gw,
-                       double* e_blk) {
+                       const double* grad, const DevBlock* blocks, const double* part, const double* w, double* eps,
+                       double* v, double* g, double* vs, double* gw, double* e_blk) {
   const int nb = (int)((npts + bs - 1) / bs);
   if (nb <= 0) return;
-  hipLaunchKernelGGL(functional_kernel, dim3(nb), dim3(128), 0, s, func, prm, npts, bs, rho, grad, w, eps, v, g, vs,
-                     gw, e_blk);
+  hipLaunchKernelGGL(functional_kernel, dim3(nb), dim3(128), 0, s, func, prm, npts, bs, rho, grad, blocks, part, w,
+                     eps, v, g, vs, gw, e_blk);
 }
 
 template <typename T>
-void launch_fock(hipStream_t s, const DevBlock* blocks, int nblocks, const int* ao_idx, const T* tiles, int ncomp,
-                 const double* vs, const double* gw, int gga, double thr, double* U, int n, int64_t npts) {
-  (void)ncomp;
-  if (nblocks <= 0) return;
-  hipLaunchKernelGGL(fock_kernel<T>, dim3(nblocks), dim3(256), 0, s, blocks, ao_idx, tiles, vs, gw, gga, thr, U, n,
-                     npts);
+void launch_fock(hipStream_t s, int cls, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
+                 const T* tiles, const double* vs, const double* gw, int gga, double thr, double* U, int n,
+                 int64_t npts) {
+  if (nwork <= 0) return;
+  switch (cls) {
+    case 0:
+      hipLaunchKernelGGL((fock_kernel<T, 1>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
+                         thr, U, n, npts);
+      break;
+    case 1:
+      hipLaunchKernelGGL((fock_kernel<T, 2>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
+                         thr, U, n, npts);
+      break;
+    default:
+      hipLaunchKernelGGL((fock_kernel<T, 4>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
+                         thr, U, n, npts);
+      break;
+  }
 }
-template void launch_fock<double>(hipStream_t, const DevBlock*, int, const int*, const double*, int, const double*,
-                                  const double*, int, double, double*, int, int64_t);
-template void launch_fock<float>(hipStream_t, const DevBlock*, int, const int*, const float*, int, const double*,
-                                 const double*, int, double, double*, int, int64_t);
+template void launch_fock<double>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*, const double*,
+                                  const double*, const double*, int, double, double*, int, int64_t);
+template void launch_fock<float>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*, const float*,
+                                 const double*, const double*, int, double, double*, int, int64_t);
+
+template <typename T>
+void launch_fock_pairs(hipStream_t s, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
+                       const T* tiles, const double* vs, const double* gw, int gga, double thr, double* U, int n,
+                       int64_t npts) {
+  if (nwork <= 0) return;
+  hipLaunchKernelGGL((fock_pairs_kernel<T>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
+                     thr, U, n, npts);
+}
+template void launch_fock_pairs<double>(hipStream_t, const WorkItem*, int, const DevBlock*, const int*, const double*,
+                                        const double*, const double*, int, double, double*, int, int64_t);
+template void launch_fock_pairs<float>(hipStream_t, const WorkItem*, int, const DevBlock*, const int*, const float*,
+                                       const double*, const double*, int, double, double*, int, int64_t);
+int fock_pairs_rmax(int fp64) { return fp64 ? FockPairsCfg<double>::RMAX : FockPairsCfg<float>::RMAX; }
+int fock_pairs_maxp(int fp64) { return fp64 ? FockPairsCfg<double>::MAXP : FockPairsCfg<float>::MAXP; }
 
 void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
                              double* vs, double* gw) {

@@ -159,10 +159,39 @@ struct xc_bfogc {
   DBuf<int> d_ao_idx, d_sig_shell, d_sig_pos;
   DBuf<double> d_t64;
   DBuf<float> d_t32;
+  // contraction work lists per size class
+  int npass_max = 0;
+  std::vector<WorkItem> dens_work[kMaxNct + 1];          // by NCT
+  std::vector<WorkItem> fock_work[2][kNumClasses];       // macro-tile kernel, [fp64][class]
+  std::vector<WorkItem> pair_work[2];                    // pair-tile kernel, [fp64]
+  DBuf<WorkItem> d_dens_work[kMaxNct + 1], d_fock_work[2][kNumClasses], d_pair_work[2];
   // per-build work buffers
-  DBuf<double> d_rho, d_grad, d_vs, d_gw, d_eblk, d_U, d_P, d_F, d_v, d_g, d_E;
+  DBuf<double> d_part, d_rho, d_grad, d_vs, d_gw, d_eblk, d_U, d_P, d_F, d_v, d_g, d_E;
   DBuf<float> d_P32;
 };
+
+namespace {
+
+template <typename T>
+void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad) {
+  for (int c = kMaxNct; c >= 1; --c)
+    launch_density<T>(bf->ctx->stream, c, bf->d_dens_work[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
+                      bf->d_ao_idx.p, tiles, P, bf->basis->nao, want_grad, bf->d_part.p, bf->grid->npts);
+}
+
+template <typename T>
+void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr) {
+  const int f = sizeof(T) == 8 ? 1 : 0;
+  for (int c = kNumClasses - 1; c >= 0; --c)
+    launch_fock<T>(bf->ctx->stream, c, bf->d_fock_work[f][c].p, (int)bf->fock_work[f][c].size(), bf->d_blocks.p,
+                   bf->d_ao_idx.p, tiles, bf->d_vs.p, bf->d_gw.p, gga, thr, bf->d_U.p, bf->basis->nao,
+                   bf->grid->npts);
+  launch_fock_pairs<T>(bf->ctx->stream, bf->d_pair_work[f].p, (int)bf->pair_work[f].size(), bf->d_blocks.p,
+                       bf->d_ao_idx.p, tiles, bf->d_vs.p, bf->d_gw.p, gga, thr, bf->d_U.p, bf->basis->nao,
+                       bf->grid->npts);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -499,6 +528,33 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
         tile_off += (int64_t)bk.npad * kTile * bf->ncomp;
         bf->sum_nb += cnt;
         bf->sum_B_nb2 += (double)bk.npts * cnt * cnt;
+        bk.npass = 0;
+        if (cnt > 0) {
+          // density: balanced passes of at most 128 columns, 16-column granularity
+          const int npass = (bk.npad + 16 * kMaxNct - 1) / (16 * kMaxNct);
+          const int width = ((bk.npad + npass - 1) / npass + 15) / 16 * 16;
+          int s = 0;
+          for (int c0 = 0; c0 < bk.npad; c0 += width, ++s) {
+            const int ncols = std::min(width, bk.npad - c0);
+            bf->dens_work[(ncols + 15) / 16].push_back(WorkItem{b, c0, ncols, s});
+          }
+          bk.npass = s;
+          bf->npass_max = std::max(bf->npass_max, s);
+          // Fock: pair-tile items where the staged rows fit, else 2D macro-tiles
+          const int nt = bk.npad / 32;
+          for (int fp64 = 0; fp64 < 2; ++fp64) {
+            const int rmax = fock_pairs_rmax(fp64), per = 4 * fock_pairs_maxp(fp64);
+            if (bk.npad <= rmax) {
+              const int npairs = nt * (nt + 1) / 2;
+              for (int a = 0; a < npairs; a += per) bf->pair_work[fp64].push_back(WorkItem{b, a, std::min(per, npairs - a), 0});
+            } else {
+              const int cls = size_class(bk.npad), mw = class_width(cls);
+              const int ng = (bk.npad + mw - 1) / mw;
+              for (int I = 0; I < ng; ++I)
+                for (int J = I; J < ng; ++J) bf->fock_work[fp64][cls].push_back(WorkItem{b, I * mw, J * mw, 0});
+            }
+          }
+        }
       }
       bf->tile_elems = tile_off;
       // 3. admission by footprint
@@ -513,6 +569,12 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       bf->d_ao_idx.upload(ctx, bf->ao_idx.data(), bf->ao_idx.size());
       bf->d_sig_shell.upload(ctx, sig_shell.data(), sig_shell.size());
       bf->d_sig_pos.upload(ctx, sig_pos.data(), sig_pos.size());
+      for (int c = 0; c <= kMaxNct; ++c) bf->d_dens_work[c].upload(ctx, bf->dens_work[c].data(), bf->dens_work[c].size());
+      for (int f = 0; f < 2; ++f) {
+        for (int c = 0; c < kNumClasses; ++c)
+          bf->d_fock_work[f][c].upload(ctx, bf->fock_work[f][c].data(), bf->fock_work[f][c].size());
+        bf->d_pair_work[f].upload(ctx, bf->pair_work[f].data(), bf->pair_work[f].size());
+      }
       bf->d_t64.alloc(ctx, (size_t)std::max<int64_t>(tile_off, 1));
       HIPX(hipMemsetAsync(bf->d_t64.p, 0, sizeof(double) * bf->d_t64.n, ctx->stream));
       if (keep_fp32) {
@@ -526,6 +588,7 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       // 5. work buffers of a build
       const int n = basis->nao;
       const int nxc = (int)((N + kTile - 1) / kTile);
+      bf->d_part.alloc(ctx, (size_t)std::max(bf->npass_max, 1) * 4 * N);
       bf->d_rho.alloc(ctx, N);
       bf->d_grad.alloc(ctx, 3 * N);
       bf->d_vs.alloc(ctx, N);
@@ -619,8 +682,9 @@ int xc_density(xc_bfogc* bf, const double* P, double* rho, double* grad, int* no
     const int n = bf->basis->nao;
     const int64_t N = bf->grid->npts;
     HIPX(hipMemcpyAsync(bf->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
-    launch_density<double>(ctx->stream, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t64.p, bf->ncomp,
-                           bf->d_P.p, n, grad ? 1 : 0, bf->d_rho.p, bf->d_grad.p, N);
+    run_density<double>(bf, bf->d_t64.p, bf->d_P.p, grad ? 1 : 0);
+    launch_reduce_partials(ctx->stream, bf->d_blocks.p, bf->bs, N, bf->d_part.p, grad ? 1 : 0, bf->d_rho.p,
+                           bf->d_grad.p);
     HIPX(hipGetLastError());
     HIPX(hipMemcpyAsync(rho, bf->d_rho.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
     if (grad) HIPX(hipMemcpyAsync(grad, bf->d_grad.p, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, ctx->stream));
@@ -649,8 +713,8 @@ int xc_functional(xc_ctx* ctx, int functional, int64_t npts, int block_size, con
     if (gga) d_g.alloc(ctx, 3 * npts);
     d_eblk.alloc(ctx, nxc);
     d_E.alloc(ctx, 1);
-    launch_functional(ctx->stream, functional, ctx->prm, npts, block_size, d_rho.p, d_grad.p, d_w.p, d_eps.p, d_v.p,
-                      d_g.p, nullptr, nullptr, d_eblk.p);
+    launch_functional(ctx->stream, functional, ctx->prm, npts, block_size, d_rho.p, d_grad.p, nullptr, nullptr, d_w.p,
+                      d_eps.p, d_v.p, d_g.p, nullptr, nullptr, d_eblk.p);
     launch_sum(ctx->stream, d_eblk.p, nxc, d_E.p);
     HIPX(hipGetLastError());
     if (eps) HIPX(hipMemcpyAsync(eps, d_eps.p, sizeof(double) * npts, hipMemcpyDeviceToHost, ctx->stream));
@@ -681,8 +745,7 @@ int xc_fock_add(xc_bfogc* bf, double thr, const double* v, const double* g, doub
     launch_weight_potential(ctx->stream, N, bf->grid->d_w.p, bf->d_v.p, g ? bf->d_g.p : nullptr, bf->d_vs.p,
                             bf->d_gw.p);
     launch_fill(ctx->stream, bf->d_U.p, (int64_t)n * n, 0.0);
-    launch_fock<double>(ctx->stream, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t64.p, bf->ncomp, bf->d_vs.p,
-                        bf->d_gw.p, g ? 1 : 0, thr, bf->d_U.p, n, N);
+    run_fock<double>(bf, bf->d_t64.p, g ? 1 : 0, thr);
     HIPX(hipMemcpyAsync(bf->d_F.p, F, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
     launch_finalize(ctx->stream, bf->d_U.p, bf->d_F.p, n, 1);
     HIPX(hipGetLastError());
@@ -706,23 +769,25 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   if (precision == XC_PREC_FP32) launch_to_float(s, dP, bf->d_P32.p, (int64_t)n * n);
   ctx->mark();
   if (precision == XC_PREC_FP64)
-    launch_density<double>(s, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t64.p, bf->ncomp, dP, n, gga ? 1 : 0,
-                           bf->d_rho.p, bf->d_grad.p, N);
+    run_density<double>(bf, bf->d_t64.p, dP, gga ? 1 : 0);
   else
-    launch_density<float>(s, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t32.p, bf->ncomp, bf->d_P32.p, n,
-                          gga ? 1 : 0, bf->d_rho.p, bf->d_grad.p, N);
+    run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0);
   ctx->mark();
-  launch_functional(s, functional, ctx->prm, N, kTile, bf->d_rho.p, gga ? bf->d_grad.p : nullptr, bf->grid->d_w.p,
-                    nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
+  if (bf->bs == kTile) {
+    launch_functional(s, functional, ctx->prm, N, kTile, nullptr, nullptr, bf->d_blocks.p, bf->d_part.p,
+                      bf->grid->d_w.p, nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
+  } else {
+    launch_reduce_partials(s, bf->d_blocks.p, bf->bs, N, bf->d_part.p, gga ? 1 : 0, bf->d_rho.p, bf->d_grad.p);
+    launch_functional(s, functional, ctx->prm, N, kTile, bf->d_rho.p, gga ? bf->d_grad.p : nullptr, nullptr,
+                      nullptr, bf->grid->d_w.p, nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
+  }
   ctx->mark();
   launch_fill(s, bf->d_U.p, (int64_t)n * n, 0.0);
   ctx->mark();
   if (precision == XC_PREC_FP64)
-    launch_fock<double>(s, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t64.p, bf->ncomp, bf->d_vs.p, bf->d_gw.p,
-                        gga ? 1 : 0, thr, bf->d_U.p, n, N);
+    run_fock<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr);
   else
-    launch_fock<float>(s, bf->d_blocks.p, bf->nblocks, bf->d_ao_idx.p, bf->d_t32.p, bf->ncomp, bf->d_vs.p, bf->d_gw.p,
-                       gga ? 1 : 0, thr, bf->d_U.p, n, N);
+    run_fock<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr);
   ctx->mark();
   launch_finalize(s, bf->d_U.p, dF, n, 0);
   launch_sum(s, bf->d_eblk.p, nxc, ctx->d_energy);
