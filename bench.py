@@ -61,6 +61,60 @@ def cpu_baseline(sysd, min_seconds):
     return dict(seconds=el, reps=reps, F=F, E=E, threads=O.max_threads())
 
 
+def fde_sweep(ctx, steps, warmup, check_oracle):
+    """C6H6 + 24 H2O / def2-SVP on the supersystem grid: one freeze-and-thaw sweep is, for every
+    subsystem as the active one, the PBE non-additive XC and the TF non-additive kinetic potential
+    (50 NAdd builds, FP64), with every density matrix resident on the device."""
+    from serenity_amd import xc, workloads as W
+    emb = W.build_embedding(24, accuracy=4)
+    g, subs = emb["grid"], emb["subsystems"]
+    bfs = [xc.BasisFunctionOnGridController(ctx, s["basis"], g, 128, 1e-9, 1) for s in subs]
+    dP, dF = [], []
+    for s in subs:
+        n = s["basis"].nao
+        d = ctx.alloc(8 * n * n)
+        ctx.upload(d, np.asfortranarray(s["P"]).ravel(order="F"))
+        dP.append(d)
+        dF.append(ctx.alloc(8 * n * n))
+    pots = []
+    for a in range(len(subs)):
+        env = [i for i in range(len(subs)) if i != a]
+        for f in ("PBE", "TF"):
+            pots.append((xc.NAddFuncPotential(bfs[a], [bfs[i] for i in env], f, "FP64"), a, env))
+
+    def sweep():
+        for pot, a, env in pots:
+            pot.build_device(dP[a], [dP[i] for i in env], dF[a], sync=False)
+
+    for _ in range(warmup):
+        sweep()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sweep()
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = dict(workload="C6H6+24H2O/def2-SVP, PBE + TF non-additive potentials, supersystem SSF accuracy-4 grid, "
+                        "25 subsystems, synthetic subsystem densities",
+               npoints=g.getNGridPoints(), nao_total=int(sum(s["basis"].nao for s in subs)),
+               nadd_builds_per_sweep=len(pots), ms_per_sweep=round(ms, 4),
+               ms_per_nadd_build=round(ms / len(pots), 5))
+    if check_oracle:
+        from oracle import oracle as O
+        pot, a, env = pots[0]
+        E = pot.build_device(dP[a], [dP[i] for i in env], dF[a], sync=True)
+        n = subs[a]["basis"].nao
+        F = np.zeros(n * n)
+        ctx.download(F, dF[a])
+        Fo, Eo = O.nadd_fock_build(subs[a]["basis"], [subs[i]["basis"] for i in env], g.points, g.weights,
+                                   subs[a]["P"], [subs[i]["P"] for i in env], "PBE")
+        out["benzene_pbe_nadd_dE_vs_oracle"] = abs(E - Eo)
+        out["benzene_pbe_nadd_max_dF_vs_oracle"] = float(np.abs(F.reshape(n, n, order="F") - Fo).max())
+    for d in dP + dF:
+        ctx.free(d)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +123,9 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fde", action="store_true", help="skip the embedding (non-additive potential) sweep")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="issue one device copy of this size first (HBM byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the XC Fock build runs on one MI355X here "
@@ -92,6 +149,14 @@ def main():
     nblocks = bf.getNBlocks()
     log("AO tiles resident: %d blocks, mean n_b %.1f, %.2f GB, setup %.2fs" %
         (nblocks, sum_nb / nblocks, tile_bytes / 1e9, t_ao))
+
+    if args.copy_calibration_mb > 0:
+        nb = args.copy_calibration_mb << 20
+        a, c = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy(c, a, nb)
+        ctx.free(a)
+        ctx.free(c)
+        log("calibration copy: %d MiB" % args.copy_calibration_mb)
 
     dP, dF = ctx.alloc(8 * n * n), ctx.alloc(8 * n * n)
     ctx.upload(dP, np.asfortranarray(P).ravel(order="F"))
@@ -147,6 +212,12 @@ def main():
         out_extra["max_dF_fp32_vs_oracle"] = float(np.abs(r32["F"] - cb["F"]).max())
         log("oracle: %.3f builds/s on %d threads; FP64 |dE| %.2e max|dF| %.2e" %
             (cpu["builds_per_s"], cb["threads"], out_extra["dE_xc_fp64_vs_oracle"], out_extra["max_dF_fp64_vs_oracle"]))
+
+    if not args.no_fde:
+        fde = fde_sweep(ctx, max(1, args.steps // 4), 1, not args.no_cpu_baseline)
+        log("FDE sweep: %.3f ms (%d NAdd builds, %.4f ms each)" %
+            (fde["ms_per_sweep"], fde["nadd_builds_per_sweep"], fde["ms_per_nadd_build"]))
+        out_extra["fde"] = fde
 
     line = dict(
         metric="XC Fock-build grid pts/s (FP32-MFMA contractions, FP64 functional; FP64 path and dE stated)",

@@ -173,6 +173,23 @@ int xc_fock_build_device(xc_bfogc* bfogc, int functional, int precision, double 
                          const double* dP, double* dF, double* energy);
 int xc_last_energy(xc_ctx* ctx, double* energy);
 
+/* NAddFuncPotential::getMatrix + getEnergy, restricted (src/potentials/
+ * NAddFuncPotential.cpp:192-301): the non-additive potential of the active
+ * subsystem in an embedding.  rho_sup = rho_act + sum of the environment
+ * densities on the shared supersystem grid (SupersystemDensityOnGridController
+ * .cpp:157-190); F = integral of v[rho_sup] - v[rho_act] (and the gradient
+ * terms) in the active basis, overwritten; energy = E[rho_sup] - E[rho_act]
+ * - sum_env E[rho_env] (NAddEnergyHelper::getEnergy, NAddFuncPotential.cpp
+ * :507-514).  Every controller must be built on the same grid handle.
+ * P_active: n_act x n_act; P_env[i]: n_i x n_i; F: n_act x n_act (host). */
+int xc_nadd_fock_build(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
+                       double block_ave_threshold, const double* P_active, const double* const* P_env, double* F,
+                       double* energy);
+/* Same on device-resident P and F (energy as for xc_fock_build_device). */
+int xc_nadd_fock_build_device(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
+                              double block_ave_threshold, const double* dP_active, const double* const* dP_env,
+                              double* dF, double* energy);
+
 /* Stage timing with HIP events on the context stream.  While enabled, every
  * build records an event between its stages; xc_profile_collect waits for
  * them, returns the summed milliseconds of each stage over the builds since
@@ -188,6 +205,9 @@ int xc_profile_collect(xc_ctx* ctx, double* stage_ms, int* nbuilds);
 int xc_device_alloc(xc_ctx* ctx, size_t bytes, void** dptr);
 int xc_device_free(xc_ctx* ctx, void* dptr);
 int xc_memcpy_h2d(xc_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel of the library (reads and
+ * writes `bytes` once; used to calibrate HBM byte counters). */
+int xc_device_copy(xc_ctx* ctx, void* dst, const void* src, size_t bytes);
 int xc_memcpy_d2h(xc_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 #ifdef __cplusplus

@@ -1,0 +1,371 @@
+/*
+ * serenity_xc.hpp -- C++ host classes of the XC Fock build, shaped like
+ * Serenity's grid-path classes, over the C ABI in serenity_xc.h.
+ *
+ * Names, constructor arguments and method names follow the reference so that
+ * Serenity code (or a test written like Serenity's gtest cases) reads alike:
+ *
+ *   serenity_amd::BasisFunctionOnGridController   BasisFunctionOnGridController.h:105-131
+ *   serenity_amd::DensityOnGridCalculator         DensityOnGridCalculator.h:45-134
+ *   serenity_amd::ScalarOperatorToMatrixAdder     ScalarOperatorToMatrixAdder.h:48-196
+ *   serenity_amd::FunctionalLibrary               FunctionalLibrary.h:51-64
+ *   serenity_amd::FuncPotential                   FuncPotential.h:48-145
+ *
+ * Matrices are column-major (Eigen's default): element (i, j) of an n x n
+ * matrix at data[i + n * j].  Grid points are 3 x N column-major.  Every
+ * failure of the library throws serenity_amd::SerenityError (the reference's
+ * SerenityError counterpart) with the library's message.  Header only; link
+ * against libserenity_xc.so.
+ */
+#ifndef SERENITY_XC_HPP
+#define SERENITY_XC_HPP
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "serenity_xc.h"
+
+namespace serenity_amd {
+
+struct SerenityError : std::runtime_error {
+  int code;
+  SerenityError(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+inline void check(int rc) {
+  if (rc != XC_OK) throw SerenityError(rc, xc_last_error());
+}
+
+/* Column-major dense matrix (the shape Serenity's Eigen matrices have on this path). */
+struct Matrix {
+  int rows = 0, cols = 0;
+  std::vector<double> data;
+  Matrix() = default;
+  Matrix(int r, int c) : rows(r), cols(c), data((size_t)r * c, 0.0) {}
+  double& operator()(int i, int j) { return data[(size_t)i + (size_t)rows * j]; }
+  double operator()(int i, int j) const { return data[(size_t)i + (size_t)rows * j]; }
+  void setZero() { std::fill(data.begin(), data.end(), 0.0); }
+};
+
+enum class Functional { LDA = XC_FUNC_LDA, PBE = XC_FUNC_PBE, TF = XC_FUNC_TF };
+enum class Precision { FP64 = XC_PREC_FP64, FP32 = XC_PREC_FP32 };
+
+/* One HIP device and its stream. */
+class Context {
+ public:
+  explicit Context(int device = 0) { check(xc_ctx_create(device, &_ctx)); }
+  ~Context() { xc_ctx_destroy(_ctx); }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  xc_ctx* get() const { return _ctx; }
+  void synchronize() { check(xc_ctx_synchronize(_ctx)); }
+
+ private:
+  xc_ctx* _ctx = nullptr;
+};
+
+/* Contracted shells in AO order (Serenity's BasisController as seen by the grid path). */
+class BasisController {
+ public:
+  struct Shells {
+    std::vector<int> l, pure, nprim;
+    std::vector<double> center, alpha, coef;  // center 3 x nshell; coef renormalised
+  };
+  BasisController(std::shared_ptr<Context> ctx, const Shells& s) : _ctx(std::move(ctx)), _shells(s) {
+    check(xc_basis_create(_ctx->get(), (int)s.l.size(), s.l.data(), s.pure.data(), s.nprim.data(), s.center.data(),
+                          s.alpha.data(), s.coef.data(), &_basis));
+    check(xc_basis_nao(_basis, &_nao));
+  }
+  /* Loads a Turbomole-format basis file for a geometry (Z, coordinates in bohr). */
+  static std::shared_ptr<BasisController> fromFile(std::shared_ptr<Context> ctx, const std::string& path,
+                                                   const std::vector<int>& Z, const std::vector<double>& xyz,
+                                                   bool spherical) {
+    int ns = 0;
+    int *l, *np, *atom;
+    double *c, *a, *k;
+    check(xc_load_basis(path.c_str(), (int)Z.size(), Z.data(), xyz.data(), spherical ? 1 : 0, &ns, &l, &np, &atom, &c,
+                        &a, &k));
+    Shells s;
+    int nprim = 0;
+    for (int i = 0; i < ns; ++i) nprim += np[i];
+    s.l.assign(l, l + ns);
+    s.nprim.assign(np, np + ns);
+    s.pure.assign(ns, spherical ? 1 : 0);
+    s.center.assign(c, c + 3 * ns);
+    s.alpha.assign(a, a + nprim);
+    s.coef.assign(k, k + nprim);
+    for (void* p : {(void*)l, (void*)np, (void*)atom, (void*)c, (void*)a, (void*)k}) xc_free_host(p);
+    return std::make_shared<BasisController>(std::move(ctx), s);
+  }
+  ~BasisController() { xc_basis_destroy(_basis); }
+  BasisController(const BasisController&) = delete;
+  BasisController& operator=(const BasisController&) = delete;
+  unsigned int getNBasisFunctions() const { return (unsigned int)_nao; }
+  const Shells& getShells() const { return _shells; }
+  xc_basis* get() const { return _basis; }
+
+ private:
+  std::shared_ptr<Context> _ctx;
+  Shells _shells;
+  xc_basis* _basis = nullptr;
+  int _nao = 0;
+};
+
+/* Grid points (3 x N) and weights (GridController::getGridPoints / getWeights). */
+class GridController {
+ public:
+  GridController(std::shared_ptr<Context> ctx, std::vector<double> points, std::vector<double> weights)
+      : _ctx(std::move(ctx)), _points(std::move(points)), _weights(std::move(weights)) {
+    if (_points.size() != 3 * _weights.size())
+      throw SerenityError(XC_ERR_ARG, "GridController: points and weights differ in length");
+    check(xc_grid_create(_ctx->get(), (int64_t)_weights.size(), _points.data(), _weights.data(), &_grid));
+  }
+  /* Atom-centred molecular grid (GridFactory::produce + Hilbert sort). */
+  static std::shared_ptr<GridController> molecular(std::shared_ptr<Context> ctx, const std::vector<int>& Z,
+                                                   const std::vector<double>& xyz, int gridType = XC_GRID_SSF,
+                                                   int accuracy = 4, double weightThreshold = 1e-14,
+                                                   int smoothing = 3) {
+    int64_t n = 0, nb = 0;
+    double *p, *w;
+    check(xc_molecular_grid((int)Z.size(), Z.data(), xyz.data(), gridType, XC_RAD_AHLRICHS, accuracy, smoothing,
+                            weightThreshold, 1, &n, &p, &w, &nb));
+    std::vector<double> pts(p, p + 3 * n), wts(w, w + n);
+    xc_free_host(p);
+    xc_free_host(w);
+    return std::make_shared<GridController>(std::move(ctx), std::move(pts), std::move(wts));
+  }
+  ~GridController() { xc_grid_destroy(_grid); }
+  GridController(const GridController&) = delete;
+  GridController& operator=(const GridController&) = delete;
+  unsigned int getNGridPoints() const { return (unsigned int)_weights.size(); }
+  const std::vector<double>& getGridPoints() const { return _points; }
+  const std::vector<double>& getWeights() const { return _weights; }
+  xc_grid* get() const { return _grid; }
+
+ private:
+  std::shared_ptr<Context> _ctx;
+  std::vector<double> _points, _weights;
+  xc_grid* _grid = nullptr;
+};
+
+/* BasisFunctionOnGridController(basis, grid, maxBlockSize, radialThreshold, highestDerivative). */
+class BasisFunctionOnGridController {
+ public:
+  struct BasisFunctionBlockOnGridData {
+    Matrix functionValues;                     // B x n
+    std::vector<Matrix> derivativeValues;      // x, y, z (B x n each) when highestDerivative >= 1
+    std::vector<int> negligible;               // n flags
+  };
+  BasisFunctionOnGridController(std::shared_ptr<BasisController> basis, std::shared_ptr<GridController> grid,
+                                unsigned int maxBlockSize, double radialThreshold, unsigned int highestDerivative,
+                                bool keepFP32 = false)
+      : _basis(std::move(basis)), _grid(std::move(grid)), _highestDerivative(highestDerivative) {
+    if (highestDerivative > 2)
+      throw SerenityError(XC_ERR_ARG, "Third derivatives of basis functions on the grid are not implemented!");
+    check(xc_bfogc_create(_basis->get(), _grid->get(), (int)maxBlockSize, radialThreshold, (int)highestDerivative,
+                          keepFP32 ? 1 : 0, &_bf));
+    int64_t np = 0;
+    check(xc_bfogc_info(_bf, &_nBlocks, &np, &_nao, &_maxBlockSize));
+  }
+  ~BasisFunctionOnGridController() { xc_bfogc_destroy(_bf); }
+  BasisFunctionOnGridController(const BasisFunctionOnGridController&) = delete;
+  BasisFunctionOnGridController& operator=(const BasisFunctionOnGridController&) = delete;
+  std::shared_ptr<BasisFunctionBlockOnGridData> getBlockOnGridData(unsigned int blockIndex) const {
+    if ((int)blockIndex >= _nBlocks) throw SerenityError(XC_ERR_ARG, "block index out of range");
+    const unsigned int first = getFirstIndexOfBlock(blockIndex);
+    const int B = (int)std::min<unsigned int>(_maxBlockSize, getNGridPoints() - first);
+    auto d = std::make_shared<BasisFunctionBlockOnGridData>();
+    d->functionValues = Matrix(B, _nao);
+    d->negligible.assign(_nao, 0);
+    double* g[3] = {nullptr, nullptr, nullptr};
+    if (_highestDerivative >= 1) {
+      d->derivativeValues.assign(3, Matrix(B, _nao));
+      for (int c = 0; c < 3; ++c) g[c] = d->derivativeValues[c].data.data();
+    }
+    int bs = 0;
+    check(xc_bfogc_block_data(_bf, (int)blockIndex, d->functionValues.data.data(), g[0], g[1], g[2],
+                              d->negligible.data(), &bs));
+    return d;
+  }
+  unsigned int getNBlocks() const { return (unsigned int)_nBlocks; }
+  unsigned int getFirstIndexOfBlock(unsigned int blockIndex) const { return blockIndex * (unsigned int)_maxBlockSize; }
+  unsigned int getNGridPoints() const { return _grid->getNGridPoints(); }
+  unsigned int getNBasisFunctions() const { return (unsigned int)_nao; }
+  unsigned int getMaxBlockSize() const { return (unsigned int)_maxBlockSize; }
+  unsigned int getHighestDerivative() const { return _highestDerivative; }
+  std::shared_ptr<BasisController> getBasisController() const { return _basis; }
+  std::shared_ptr<GridController> getGridController() const { return _grid; }
+  xc_bfogc* get() const { return _bf; }
+
+ private:
+  std::shared_ptr<BasisController> _basis;
+  std::shared_ptr<GridController> _grid;
+  xc_bfogc* _bf = nullptr;
+  int _nBlocks = 0, _nao = 0, _maxBlockSize = 0;
+  unsigned int _highestDerivative;
+};
+
+/* DensityOnGridCalculator(bfogc, blockAverageThreshold): rho and grad rho of a restricted P. */
+class DensityOnGridCalculator {
+ public:
+  DensityOnGridCalculator(std::shared_ptr<BasisFunctionOnGridController> bf, double blockAverageThreshold)
+      : _bf(std::move(bf)), _blockAverageThreshold(blockAverageThreshold) {}
+  std::vector<double> calcDensityOnGrid(const Matrix& P) {
+    std::vector<double> rho(_bf->getNGridPoints());
+    run(P, rho, nullptr);
+    return rho;
+  }
+  /* rho (N) and gradient (3 x N, x[N] y[N] z[N]) are resized and overwritten. */
+  void calcDensityAndGradientOnGrid(const Matrix& P, std::vector<double>& rho, std::vector<double>& grad) {
+    rho.assign(_bf->getNGridPoints(), 0.0);
+    grad.assign(3 * (size_t)_bf->getNGridPoints(), 0.0);
+    run(P, rho, &grad);
+  }
+  const std::vector<int>& getNonNegligibleBlocks() const { return _nonNegligible; }
+  std::shared_ptr<BasisFunctionOnGridController> getBasisFunctionOnGridController() const { return _bf; }
+
+ private:
+  void run(const Matrix& P, std::vector<double>& rho, std::vector<double>* grad) {
+    const int n = (int)_bf->getNBasisFunctions();
+    if (P.rows != n || P.cols != n)
+      throw SerenityError(XC_ERR_ARG, "DensityOnGridCalculator: density matrix and basis differ in size");
+    _nonNegligible.assign(_bf->getNBlocks(), 0);
+    check(xc_density(_bf->get(), P.data.data(), rho.data(), grad ? grad->data() : nullptr, _nonNegligible.data()));
+  }
+  std::shared_ptr<BasisFunctionOnGridController> _bf;
+  double _blockAverageThreshold;
+  std::vector<int> _nonNegligible;
+};
+
+/* FunctionalData of calcData(GRADIENTS, ...): epuv, dFdRho, dFdGradRho (3 x N), energy. */
+struct FunctionalData {
+  std::vector<double> epuv, dFdRho, dFdGradRho;
+  double energy = 0.0;
+};
+
+class FunctionalLibrary {
+ public:
+  FunctionalLibrary(std::shared_ptr<Context> ctx, unsigned int maxBlockSize) : _ctx(std::move(ctx)), _bs(maxBlockSize) {}
+  FunctionalData calcData(Functional f, const std::vector<double>& rho, const std::vector<double>* grad,
+                          const std::vector<double>& weights) {
+    FunctionalData d;
+    const int64_t N = (int64_t)rho.size();
+    d.epuv.resize(N);
+    d.dFdRho.resize(N);
+    const bool gga = f == Functional::PBE;
+    if (gga) d.dFdGradRho.resize(3 * N);
+    check(xc_functional(_ctx->get(), (int)f, N, (int)_bs, rho.data(), grad ? grad->data() : nullptr, weights.data(),
+                        d.epuv.data(), d.dFdRho.data(), gga ? d.dFdGradRho.data() : nullptr, &d.energy));
+    return d;
+  }
+
+ private:
+  std::shared_ptr<Context> _ctx;
+  unsigned int _bs;
+};
+
+/* ScalarOperatorToMatrixAdder(bfogc, blockAveThreshold): F += integral of v (and g). */
+class ScalarOperatorToMatrixAdder {
+ public:
+  ScalarOperatorToMatrixAdder(std::shared_ptr<BasisFunctionOnGridController> bf, double blockAveThreshold)
+      : _bf(std::move(bf)), _thr(blockAveThreshold) {}
+  void addScalarOperatorToMatrix(Matrix& F, const std::vector<double>& scalar) { add(F, scalar, nullptr); }
+  void addScalarOperatorToMatrix(Matrix& F, const std::vector<double>& scalar, const std::vector<double>& gradient) {
+    add(F, scalar, &gradient);
+  }
+
+ private:
+  void add(Matrix& F, const std::vector<double>& v, const std::vector<double>* g) {
+    const unsigned int N = _bf->getNGridPoints();
+    if (v.size() != N || (g && g->size() != 3 * (size_t)N))
+      throw SerenityError(XC_ERR_ARG,
+                          "The scalar operator and the BasisFunctionOnGridController passed to "
+                          "addScalarOperatorToMatrix are not defined on the same grid!");
+    if (F.rows != (int)_bf->getNBasisFunctions() || F.cols != F.rows)
+      throw SerenityError(XC_ERR_ARG, "ScalarOperatorToMatrixAdder: matrix and basis differ in size");
+    check(xc_fock_add(_bf->get(), _thr, v.data(), g ? g->data() : nullptr, F.data.data()));
+  }
+  std::shared_ptr<BasisFunctionOnGridController> _bf;
+  double _thr;
+};
+
+/* FuncPotential: getMatrix(P) returns V_xc (FuncPotential.cpp:73-111), getEnergy the E_xc of that build. */
+class FuncPotential {
+ public:
+  FuncPotential(std::shared_ptr<BasisFunctionOnGridController> bf, Functional f, Precision p = Precision::FP64,
+                double blockAveThreshold = 1e-11)
+      : _bf(std::move(bf)), _f(f), _p(p), _thr(blockAveThreshold) {}
+  Matrix getMatrix(const Matrix& P) {
+    const int n = (int)_bf->getNBasisFunctions();
+    if (P.rows != n || P.cols != n)
+      throw SerenityError(XC_ERR_ARG, "FuncPotential: density matrix and basis differ in size");
+    Matrix F(n, n);
+    check(xc_fock_build(_bf->get(), (int)_f, (int)_p, _thr, P.data.data(), F.data.data(), &_energy));
+    _built = true;
+    return F;
+  }
+  double getEnergy() const {
+    if (!_built) throw SerenityError(XC_ERR_STATE, "FuncPotential: getMatrix has not been called");
+    return _energy;
+  }
+
+ private:
+  std::shared_ptr<BasisFunctionOnGridController> _bf;
+  Functional _f;
+  Precision _p;
+  double _thr;
+  double _energy = 0.0;
+  bool _built = false;
+};
+
+/* NAddFuncPotential (NAddFuncPotential.cpp:192-301), restricted: the non-additive potential of the active
+ * subsystem, F = integral of v[rho_sup] - v[rho_act] in the active basis, getEnergy() =
+ * E[rho_sup] - E[rho_act] - sum_env E[rho_env].  Every controller must share one GridController. */
+class NAddFuncPotential {
+ public:
+  NAddFuncPotential(std::shared_ptr<BasisFunctionOnGridController> active,
+                    std::vector<std::shared_ptr<BasisFunctionOnGridController>> environment, Functional f,
+                    Precision p = Precision::FP64, double blockAveThreshold = 1e-11)
+      : _act(std::move(active)), _env(std::move(environment)), _f(f), _p(p), _thr(blockAveThreshold) {}
+  Matrix getMatrix(const Matrix& Pact, const std::vector<Matrix>& Penv) {
+    const int n = (int)_act->getNBasisFunctions();
+    if (Pact.rows != n || Pact.cols != n)
+      throw SerenityError(XC_ERR_ARG, "NAddFuncPotential: active density matrix and basis differ in size");
+    if (Penv.size() != _env.size())
+      throw SerenityError(XC_ERR_ARG, "NAddFuncPotential: one density matrix per environment subsystem is needed");
+    std::vector<xc_bfogc*> env;
+    std::vector<const double*> P;
+    for (size_t i = 0; i < _env.size(); ++i) {
+      const int m = (int)_env[i]->getNBasisFunctions();
+      if (Penv[i].rows != m || Penv[i].cols != m)
+        throw SerenityError(XC_ERR_ARG, "NAddFuncPotential: environment density matrix and basis differ in size");
+      env.push_back(_env[i]->get());
+      P.push_back(Penv[i].data.data());
+    }
+    Matrix F(n, n);
+    check(xc_nadd_fock_build(_act->get(), (int)env.size(), env.data(), (int)_f, (int)_p, _thr, Pact.data.data(),
+                             P.data(), F.data.data(), &_energy));
+    _built = true;
+    return F;
+  }
+  double getEnergy() const {
+    if (!_built) throw SerenityError(XC_ERR_STATE, "NAddFuncPotential: getMatrix has not been called");
+    return _energy;
+  }
+
+ private:
+  std::shared_ptr<BasisFunctionOnGridController> _act;
+  std::vector<std::shared_ptr<BasisFunctionOnGridController>> _env;
+  Functional _f;
+  Precision _p;
+  double _thr;
+  double _energy = 0.0;
+  bool _built = false;
+};
+
+}  // namespace serenity_amd
+
+#endif /* SERENITY_XC_HPP */

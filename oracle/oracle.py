@@ -137,5 +137,31 @@ def fock_build(basis, points, weights, P, name, block_size=128, thr=1e-9, block_
     return F, E.value, rho, grad.reshape(3, N)
 
 
+def nadd_fock_build(active_basis, env_bases, points, weights, P_active, P_envs, name, block_size=128, thr=1e-9,
+                    block_ave_thr=1e-11):
+    """NAddFuncPotential::getMatrix/getEnergy, restricted (NAddFuncPotential.cpp:192-301).
+
+    rho_sup = sum of the subsystem densities on the shared grid
+    (SupersystemDensityOnGridController.cpp:157-190); v = v[rho_sup] - v[rho_act]
+    (likewise d eps/d grad rho) is integrated with the active basis;
+    E = E[rho_sup] - E[rho_act] - sum_env E[rho_env] (NAddEnergyHelper::getEnergy,
+    NAddFuncPotential.cpp:507-514).  Returns (F, E)."""
+    gga = name in GGA
+    dens = [density(active_basis, points, P_active, block_size=block_size, thr=thr, grad=gga)]
+    for b, P in zip(env_bases, P_envs):
+        dens.append(density(b, points, P, block_size=block_size, thr=thr, grad=gga))
+    rho_sup = np.sum([d["rho"] for d in dens], axis=0)
+    grad_sup = np.sum([d["grad"] for d in dens], axis=0) if gga else None
+    _, v_sup, g_sup, e_sup = functional(name, rho_sup, grad_sup, weights, block_size=block_size)
+    _, v_act, g_act, e_act = functional(name, dens[0]["rho"], dens[0]["grad"] if gga else None, weights,
+                                        block_size=block_size)
+    E = e_sup - e_act
+    for d in dens[1:]:
+        E -= functional(name, d["rho"], d["grad"] if gga else None, weights, block_size=block_size)[3]
+    F = fock_add(active_basis, points, weights, v_sup - v_act, (g_sup - g_act) if gga else None,
+                 block_size=block_size, thr=thr, block_ave_thr=block_ave_thr)
+    return F, E
+
+
 def max_threads():
     return lib().oracle_max_threads()

@@ -79,6 +79,10 @@ def lib():
     _sig("xc_fock_build", C.c_int, vp, C.c_int, C.c_int, C.c_double, dp, dp, dp)
     _sig("xc_fock_build_device", C.c_int, vp, C.c_int, C.c_int, C.c_double, vp, vp, dp)
     _sig("xc_last_energy", C.c_int, vp, dp)
+    _sig("xc_nadd_fock_build", C.c_int, vp, C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.c_double, dp,
+         C.POINTER(dp), dp, dp)
+    _sig("xc_nadd_fock_build_device", C.c_int, vp, C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.c_double, vp,
+         C.POINTER(vp), vp, dp)
     _sig("xc_profile", C.c_int, vp, C.c_int)
     _sig("xc_profile_collect", C.c_int, vp, dp, ip)
     _sig("xc_hilbert_sort", C.c_int, i64, dp, dp)
@@ -86,6 +90,7 @@ def lib():
     _sig("xc_device_free", C.c_int, vp, vp)
     _sig("xc_memcpy_h2d", C.c_int, vp, vp, vp, C.c_size_t)
     _sig("xc_memcpy_d2h", C.c_int, vp, vp, vp, C.c_size_t)
+    _sig("xc_device_copy", C.c_int, vp, vp, vp, C.c_size_t)
     return _lib
 
 

@@ -54,7 +54,26 @@ def build(verbose=True):
         print("[build]", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_cpp_driver(verbose)
     return LIB
+
+
+CPP_DRIVER_SRC = os.path.join(HERE, "..", "tests", "cpp", "cpp_parity.cpp")
+CPP_DRIVER = os.path.join(HERE, "..", "tests", "cpp", "cpp_parity")
+
+
+def build_cpp_driver(verbose=True):
+    """The C++ parity driver over include/serenity_xc.hpp, linked to the in-tree library."""
+    if not os.path.exists(CPP_DRIVER_SRC):
+        return None
+    inc = os.path.join(HERE, "..", "include")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", inc, CPP_DRIVER_SRC, "-o", CPP_DRIVER + ".tmp", "-L", HERE,
+           "-lserenity_xc", "-Wl,-rpath,$ORIGIN/../../serenity_amd"]
+    if verbose:
+        print("[build]", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(CPP_DRIVER + ".tmp", CPP_DRIVER)
+    return CPP_DRIVER
 
 
 if __name__ == "__main__":

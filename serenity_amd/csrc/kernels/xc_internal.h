@@ -72,12 +72,13 @@ void launch_density(hipStream_t s, int nct, const WorkItem* work, int nwork, con
 
 // rho[q] / grad[.][q] from the partials of the block of q (any block size).
 void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
-                            int want_grad, double* rho, double* grad);
+                            int want_grad, double* rho, double* grad, int accumulate = 0);
 
 // Functional on rho/grad, or (part != nullptr) on the partials of 128-point blocks.
 void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* rho,
                        const double* grad, const DevBlock* blocks, const double* part, const double* w, double* eps,
-                       double* v, double* g, double* vs, double* gw, double* e_blk);
+                       double* v, double* g, double* vs, double* gw, double* e_blk,
+                       const double* rho_sub = nullptr, const double* grad_sub = nullptr);
 
 template <typename T>
 void launch_fock(hipStream_t s, int cls, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
@@ -95,9 +96,10 @@ int fock_pairs_maxp(int fp64);  // pairs per wavefront
 void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
                              double* vs, double* gw);
 void launch_fill(hipStream_t s, double* p, int64_t n, double value);
+void launch_copy(hipStream_t s, const void* src, void* dst, int64_t n32);  // n32 units of 32 bytes
 void launch_to_float(hipStream_t s, const double* src, float* dst, int64_t n);
 // F[i + n j] (+)= U[max(i,j) + n min(i,j)] (U holds the lower triangle)
 void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accumulate);
-void launch_sum(hipStream_t s, const double* x, int64_t n, double* out);
+void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int subtract = 0);
 
 }  // namespace sxc

@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void density_kernel(const WorkItem* __restrict
 
 __global__ void reduce_partials_kernel(const DevBlock* __restrict__ blocks, int bs, int64_t npts,
                                        const double* __restrict__ part, int want_grad, double* __restrict__ rho,
-                                       double* __restrict__ grad) {
+                                       double* __restrict__ grad, int accumulate) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= npts) return;
   const int np = blocks[q / bs].npass;
@@ -301,11 +301,20 @@ __global__ void reduce_partials_kernel(const DevBlock* __restrict__ blocks, int 
       z += src[3 * npts];
     }
   }
-  rho[q] = r;
-  if (want_grad) {
-    grad[q] = 2.0 * x;
-    grad[npts + q] = 2.0 * y;
-    grad[2 * npts + q] = 2.0 * z;
+  if (accumulate) {
+    rho[q] += r;
+    if (want_grad) {
+      grad[q] += 2.0 * x;
+      grad[npts + q] += 2.0 * y;
+      grad[2 * npts + q] += 2.0 * z;
+    }
+  } else {
+    rho[q] = r;
+    if (want_grad) {
+      grad[q] = 2.0 * x;
+      grad[npts + q] = 2.0 * y;
+      grad[2 * npts + q] = 2.0 * z;
+    }
   }
 }
 
@@ -331,7 +340,9 @@ __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm,
                                                          const double* __restrict__ w, double* __restrict__ eps,
                                                          double* __restrict__ v, double* __restrict__ g,
                                                          double* __restrict__ vs, double* __restrict__ gw,
-                                                         double* __restrict__ e_blk) {
+                                                         double* __restrict__ e_blk,
+                                                         const double* __restrict__ rho_sub,
+                                                         const double* __restrict__ grad_sub) {
   __shared__ double red[128];
   const int b = blockIdx.x, t = threadIdx.x;
   const int64_t first = (int64_t)b * bs;
@@ -369,6 +380,23 @@ __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm,
   const bool skip = sumabs < B * 1e-12;
   double e = 0.0, vv = 0.0, gg[3] = {0.0, 0.0, 0.0};
   if (in && !skip) e = xc_point(func, r, gx, gy, gz, prm, &vv, gg);
+  if (rho_sub) {
+    // non-additive form: the same functional of a second density is subtracted
+    // (its own block skip applies to it)
+    const double r2 = in ? rho_sub[q] : 0.0;
+    const bool skip2 = block_sum_128(fabs(r2), red) < B * 1e-12;
+    if (in && !skip2) {
+      double v2 = 0.0, g2[3] = {0.0, 0.0, 0.0};
+      const bool has_g = gga && grad_sub;
+      const double e2 = xc_point(func, r2, has_g ? grad_sub[q] : 0.0, has_g ? grad_sub[npts + q] : 0.0,
+                                 has_g ? grad_sub[2 * npts + q] : 0.0, prm, &v2, g2);
+      e -= e2;
+      vv -= v2;
+      gg[0] -= g2[0];
+      gg[1] -= g2[1];
+      gg[2] -= g2[2];
+    }
+  }
   const double wq = in ? w[q] : 0.0;
   if (in) {
     if (eps) eps[q] = e;
@@ -722,6 +750,11 @@ __global__ void weight_potential_kernel(int64_t npts, const double* __restrict__
   }
 }
 
+__global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 __global__ void fill_kernel(double* __restrict__ p, int64_t n, double value) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = value;
@@ -741,7 +774,8 @@ __global__ void finalize_kernel(const double* __restrict__ U, double* __restrict
   F[e] = accumulate ? F[e] + u : u;
 }
 
-__global__ __launch_bounds__(256) void sum_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void sum_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ out,
+                                                  int subtract) {
   __shared__ double red[256];
   const int t = threadIdx.x;
   double s = 0.0;
@@ -752,7 +786,7 @@ __global__ __launch_bounds__(256) void sum_kernel(const double* __restrict__ x, 
     if (t < k) red[t] += red[t + k];
     __syncthreads();
   }
-  if (t == 0) out[0] = red[0];
+  if (t == 0) out[0] = subtract ? out[0] - red[0] : red[0];
 }
 
 inline unsigned grid1d(int64_t n, int bsz) { return (unsigned)((n + bsz - 1) / bsz); }
@@ -804,19 +838,20 @@ template void launch_density<float>(hipStream_t, int, const WorkItem*, int, cons
                                     const float*, int, int, double*, int64_t);
 
 void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
-                            int want_grad, double* rho, double* grad) {
+                            int want_grad, double* rho, double* grad, int accumulate) {
   if (npts <= 0) return;
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(grid1d(npts, 256)), dim3(256), 0, s, blocks, bs, npts, part,
-                     want_grad, rho, grad);
+                     want_grad, rho, grad, accumulate);
 }
 
 void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* rho,
                        const double* grad, const DevBlock* blocks, const double* part, const double* w, double* eps,
-                       double* v, double* g, double* vs, double* gw, double* e_blk) {
+                       double* v, double* g, double* vs, double* gw, double* e_blk, const double* rho_sub,
+                       const double* grad_sub) {
   const int nb = (int)((npts + bs - 1) / bs);
   if (nb <= 0) return;
   hipLaunchKernelGGL(functional_kernel, dim3(nb), dim3(128), 0, s, func, prm, npts, bs, rho, grad, blocks, part, w,
-                     eps, v, g, vs, gw, e_blk);
+                     eps, v, g, vs, gw, e_blk, rho_sub, grad_sub);
 }
 
 template <typename T>
@@ -865,6 +900,11 @@ void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const
   hipLaunchKernelGGL(weight_potential_kernel, dim3(grid1d(npts, 256)), dim3(256), 0, s, npts, w, v, g, vs, gw);
 }
 
+void launch_copy(hipStream_t s, const void* src, void* dst, int64_t n32) {
+  if (n32 <= 0) return;
+  hipLaunchKernelGGL(copy_kernel, dim3(grid1d(n32, 256)), dim3(256), 0, s, (const double4*)src, (double4*)dst, n32);
+}
+
 void launch_fill(hipStream_t s, double* p, int64_t n, double value) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, p, n, value);
@@ -881,8 +921,8 @@ void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accum
   hipLaunchKernelGGL(finalize_kernel, dim3(grid1d(nn, 256)), dim3(256), 0, s, U, F, n, accumulate);
 }
 
-void launch_sum(hipStream_t s, const double* x, int64_t n, double* out) {
-  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, x, n, out);
+void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int subtract) {
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, x, n, out, subtract);
 }
 
 }  // namespace sxc

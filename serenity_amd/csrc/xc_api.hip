@@ -168,6 +168,8 @@ struct xc_bfogc {
   // per-build work buffers
   DBuf<double> d_part, d_rho, d_grad, d_vs, d_gw, d_eblk, d_U, d_P, d_F, d_v, d_g, d_E;
   DBuf<float> d_P32;
+  // supersystem density of a non-additive build with this controller active (allocated on first use)
+  DBuf<double> d_rho_sup, d_grad_sup;
 };
 
 namespace {
@@ -575,15 +577,17 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
           bf->d_fock_work[f][c].upload(ctx, bf->fock_work[f][c].data(), bf->fock_work[f][c].size());
         bf->d_pair_work[f].upload(ctx, bf->pair_work[f].data(), bf->pair_work[f].size());
       }
-      bf->d_t64.alloc(ctx, (size_t)std::max<int64_t>(tile_off, 1));
-      HIPX(hipMemsetAsync(bf->d_t64.p, 0, sizeof(double) * bf->d_t64.n, ctx->stream));
+      const size_t nt_el = (size_t)std::max<int64_t>(tile_off, 1);
+      bf->d_t64.alloc(ctx, nt_el);
+      HIPX(hipMemsetAsync(bf->d_t64.p, 0, sizeof(double) * nt_el, ctx->stream));
       if (keep_fp32) {
-        bf->d_t32.alloc(ctx, (size_t)std::max<int64_t>(tile_off, 1));
-        HIPX(hipMemsetAsync(bf->d_t32.p, 0, sizeof(float) * bf->d_t32.n, ctx->stream));
+        bf->d_t32.alloc(ctx, nt_el);
+        HIPX(hipMemsetAsync(bf->d_t32.p, 0, sizeof(float) * nt_el, ctx->stream));
       }
       launch_ao_eval(ctx->stream, bf->d_blocks.p, nb, bf->d_sig_shell.p, bf->d_sig_pos.p, basis->d_shells.p,
                      basis->d_alpha.p, basis->d_coef.p, basis->d_ang.p, grid->d_pts.p, radial_threshold,
-                     radial_threshold > 0.0 ? -std::log(radial_threshold) : HUGE_VAL, bf->ncomp, bf->d_t64.p, keep_fp32 ? bf->d_t32.p : nullptr);
+                     radial_threshold > 0.0 ? -std::log(radial_threshold) : HUGE_VAL, bf->ncomp, bf->d_t64.p,
+                     keep_fp32 ? bf->d_t32.p : nullptr);
       HIPX(hipGetLastError());
       // 5. work buffers of a build
       const int n = basis->nao;
@@ -795,6 +799,124 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   HIPX(hipGetLastError());
 }
 
+/* Non-additive build: rho_sup = rho_act + sum_env rho_env on the shared grid,
+ * v = v[rho_sup] - v[rho_act] integrated with the active basis,
+ * E = E[rho_sup] - E[rho_act] - sum_env E[rho_env] (left in ctx->d_energy). */
+static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int functional, int precision, double thr,
+                         const double* dP_act, const double* const* dP_env, double* dF) {
+  xc_ctx* ctx = act->ctx;
+  require(functional >= 0 && functional <= 6, XC_ERR_ARG, "unknown functional");
+  require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+  require(nenv >= 0 && (nenv == 0 || (env && dP_env)), XC_ERR_ARG, "non-additive build: null environment list");
+  const bool gga = functional_is_gga(functional);
+  const bool fp32 = precision == XC_PREC_FP32;
+  std::vector<xc_bfogc*> sub(1, act);
+  std::vector<const double*> Ps(1, dP_act);
+  for (int i = 0; i < nenv; ++i) {
+    require(env[i] && dP_env[i], XC_ERR_ARG, "non-additive build: null environment controller or density");
+    require(env[i] != act, XC_ERR_ARG, "non-additive build: the active controller is also listed as environment");
+    for (int j = 0; j < i; ++j)
+      require(env[j] != env[i], XC_ERR_ARG, "non-additive build: an environment controller is listed twice");
+    sub.push_back(env[i]);
+    Ps.push_back(dP_env[i]);
+  }
+  for (xc_bfogc* b : sub) {
+    require(b->ctx == ctx, XC_ERR_ARG, "non-additive build: controllers live on different contexts");
+    require(b->grid == act->grid, XC_ERR_ARG,
+            "non-additive build: all subsystem controllers must be defined on the same (supersystem) grid");
+    require(b->bs == act->bs, XC_ERR_ARG, "non-additive build: controllers differ in block size");
+    require(!gga || b->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
+    require(!fp32 || b->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+  }
+  const int n = act->basis->nao;
+  const int64_t N = act->grid->npts;
+  const int nxc = (int)((N + kTile - 1) / kTile);
+  if (!act->d_rho_sup.p) {
+    act->d_rho_sup.alloc(ctx, N);
+    act->d_grad_sup.alloc(ctx, 3 * N);
+  }
+  hipStream_t s = ctx->stream;
+  const double* w = act->grid->d_w.p;
+  ctx->mark();
+  if (fp32)
+    for (size_t i = 0; i < sub.size(); ++i)
+      launch_to_float(s, Ps[i], sub[i]->d_P32.p, (int64_t)sub[i]->basis->nao * sub[i]->basis->nao);
+  ctx->mark();
+  for (size_t i = 0; i < sub.size(); ++i) {
+    xc_bfogc* b = sub[i];
+    if (fp32)
+      run_density<float>(b, b->d_t32.p, b->d_P32.p, gga ? 1 : 0);
+    else
+      run_density<double>(b, b->d_t64.p, Ps[i], gga ? 1 : 0);
+    launch_reduce_partials(s, b->d_blocks.p, b->bs, N, b->d_part.p, gga ? 1 : 0, b->d_rho.p, b->d_grad.p, 0);
+    launch_reduce_partials(s, b->d_blocks.p, b->bs, N, b->d_part.p, gga ? 1 : 0, act->d_rho_sup.p,
+                           act->d_grad_sup.p, i == 0 ? 0 : 1);
+  }
+  ctx->mark();
+  launch_functional(s, functional, ctx->prm, N, kTile, act->d_rho_sup.p, gga ? act->d_grad_sup.p : nullptr, nullptr,
+                    nullptr, w, nullptr, nullptr, nullptr, act->d_vs.p, act->d_gw.p, act->d_eblk.p, act->d_rho.p,
+                    gga ? act->d_grad.p : nullptr);
+  launch_sum(s, act->d_eblk.p, nxc, ctx->d_energy, 0);
+  for (size_t i = 1; i < sub.size(); ++i) {
+    xc_bfogc* b = sub[i];
+    launch_functional(s, functional, ctx->prm, N, kTile, b->d_rho.p, gga ? b->d_grad.p : nullptr, nullptr, nullptr, w,
+                      nullptr, nullptr, nullptr, nullptr, nullptr, b->d_eblk.p);
+    launch_sum(s, b->d_eblk.p, nxc, ctx->d_energy, 1);
+  }
+  ctx->mark();
+  launch_fill(s, act->d_U.p, (int64_t)n * n, 0.0);
+  ctx->mark();
+  if (fp32)
+    run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr);
+  else
+    run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr);
+  ctx->mark();
+  launch_finalize(s, act->d_U.p, dF, n, 0);
+  ctx->mark();
+  HIPX(hipGetLastError());
+}
+
+int xc_nadd_fock_build_device(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
+                              double thr, const double* dP_active, const double* const* dP_env, double* dF,
+                              double* energy) {
+  return guarded([&] {
+    require(active && dP_active && dF, XC_ERR_ARG, "xc_nadd_fock_build_device: null argument");
+    xc_ctx* ctx = active->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    enqueue_nadd(active, nenv, env, functional, precision, thr, dP_active, dP_env, dF);
+    if (energy) {
+      HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIPX(hipStreamSynchronize(ctx->stream));
+      *energy = ctx->last_energy;
+    }
+  });
+}
+
+int xc_nadd_fock_build(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision, double thr,
+                       const double* P_active, const double* const* P_env, double* F, double* energy) {
+  return guarded([&] {
+    require(active && P_active && F, XC_ERR_ARG, "xc_nadd_fock_build: null argument");
+    require(nenv >= 0 && (nenv == 0 || (env && P_env)), XC_ERR_ARG, "xc_nadd_fock_build: null environment list");
+    xc_ctx* ctx = active->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const int n = active->basis->nao;
+    HIPX(hipMemcpyAsync(active->d_P.p, P_active, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<const double*> dP(nenv);
+    for (int i = 0; i < nenv; ++i) {
+      require(env[i] && P_env[i], XC_ERR_ARG, "xc_nadd_fock_build: null environment controller or density");
+      require(env[i]->ctx == ctx, XC_ERR_ARG, "xc_nadd_fock_build: controllers live on different contexts");
+      const int m = env[i]->basis->nao;
+      HIPX(hipMemcpyAsync(env[i]->d_P.p, P_env[i], sizeof(double) * m * m, hipMemcpyHostToDevice, ctx->stream));
+      dP[i] = env[i]->d_P.p;
+    }
+    enqueue_nadd(active, nenv, env, functional, precision, thr, active->d_P.p, dP.data(), active->d_F.p);
+    HIPX(hipMemcpyAsync(F, active->d_F.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+    if (energy) *energy = ctx->last_energy;
+  });
+}
+
 int xc_fock_build_device(xc_bfogc* bf, int functional, int precision, double thr, const double* dP, double* dF,
                          double* energy) {
   return guarded([&] {
@@ -891,6 +1013,18 @@ int xc_memcpy_h2d(xc_ctx* ctx, void* dst, const void* src, size_t bytes) {
     require(ctx && dst && src, XC_ERR_ARG, "null argument");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int xc_device_copy(xc_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    require(ctx && dst && src, XC_ERR_ARG, "null argument");
+    require(bytes % 32 == 0, XC_ERR_ARG, "xc_device_copy: size must be a multiple of 32 bytes");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipSetDevice(ctx->device));
+    launch_copy(ctx->stream, src, dst, (int64_t)(bytes / 32));
+    HIPX(hipGetLastError());
     HIPX(hipStreamSynchronize(ctx->stream));
   });
 }

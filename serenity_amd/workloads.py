@@ -193,3 +193,20 @@ def build_system(config, accuracy=4, seed=SEED, spherical=True):
     S = basis.overlap()
     P = synthetic_density(S, n_electrons(Z) // 2, seed=seed)
     return dict(cfg=cfg, Z=Z, X=X, basis=basis, grid=grid, S=S, P=P)
+
+
+def build_embedding(nwater=24, basis="DEF2-SVP", accuracy=4, seed=SEED, spherical=True):
+    """Benzene + waters as 1 + nwater subsystems on the supersystem grid.
+
+    Each subsystem has its own basis (its atoms only) and a synthetic closed-shell
+    density P_i = 2 C C^T (C = S_i^-1/2 Q_i, seed + i); the grid is the molecular
+    grid of all atoms, shared by every subsystem.  Subsystem 0 is benzene."""
+    Z, X, sub = benzene_water(nwater, seed=seed)
+    grid = molecular_grid(Z, X, accuracy=accuracy)
+    subsystems = []
+    for i in range(int(sub.max()) + 1):
+        m = sub == i
+        b = BasisController.from_file(basis, Z[m], X[m], spherical=spherical)
+        P = synthetic_density(b.overlap(), n_electrons(Z[m]) // 2, seed=seed + i)
+        subsystems.append(dict(Z=Z[m], X=X[m], basis=b, P=P))
+    return dict(Z=Z, X=X, grid=grid, subsystems=subsystems)

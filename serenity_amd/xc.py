@@ -91,6 +91,10 @@ class Context:
         check(_lib.lib().xc_memcpy_d2h(self.handle, array.ctypes.data_as(C.c_void_p), dptr,
                                        C.c_size_t(array.nbytes)))
 
+    def copy(self, dst, src, nbytes):
+        """Device-to-device copy by the library's streaming copy kernel."""
+        check(_lib.lib().xc_device_copy(self.handle, dst, src, C.c_size_t(nbytes)))
+
     def profile(self, enable=True):
         """Record HIP events between the build stages of every following build."""
         check(_lib.lib().xc_profile(self.handle, int(bool(enable))))
@@ -184,6 +188,15 @@ class GridController:
     def getNGridPoints(self):
         return len(self.weights)
 
+    def _device(self, ctx):
+        """The grid's resident copy on ``ctx``, shared by every controller built on this grid."""
+        cache = self.__dict__.setdefault("_device_grids", {})
+        dg = cache.get(id(ctx))
+        if dg is None or dg.handle is None or dg.ctx_ref() is not ctx:
+            dg = _DeviceGrid(ctx, self)
+            cache[id(ctx)] = dg
+        return dg
+
 
 def molecular_grid(Z, coords_bohr, grid_type="SSF", radial_type="AHLRICHS", accuracy=4, smoothing=3,
                    weight_threshold=1e-14, sort=True, return_before_cut=False):
@@ -251,6 +264,7 @@ class _DeviceGrid(_Handle):
         check(_lib.lib().xc_grid_create(ctx.handle, grid.getNGridPoints(), _d(grid.points), _d(grid.weights),
                                         C.byref(h)))
         self.handle = h
+        self.ctx_ref = weakref.ref(ctx)
         ctx._register(self)
 
 
@@ -268,7 +282,7 @@ class BasisFunctionOnGridController(_Handle):
         self.basis = basis
         self.grid = grid
         self._db = _DeviceBasis(ctx, basis)
-        self._dg = _DeviceGrid(ctx, grid)
+        self._dg = grid._device(ctx)
         h = C.c_void_p()
         check(_lib.lib().xc_bfogc_create(self._db.handle, self._dg.handle, int(max_block_size),
                                          float(radial_threshold), int(highest_derivative), int(bool(keep_fp32)),
@@ -449,6 +463,76 @@ class FuncPotential:
         check(_lib.lib().xc_fock_build_device(self.bfogc.handle, FUNCTIONALS[self.functional],
                                               PRECISIONS[self.precision], float(self.blockAveThreshold), dP, dF,
                                               C.byref(E) if sync else None))
+        if sync:
+            self._energy = E.value
+            return E.value
+        return None
+
+
+class NAddFuncPotential:
+    """Non-additive functional potential of the active subsystem in an embedding
+    (NAddFuncPotential::getMatrix/getEnergy, restricted, NAddFuncPotential.cpp:192-301).
+
+    ``active`` and every controller in ``environment`` are BasisFunctionOnGridControllers of
+    the subsystem bases on ONE shared (supersystem) grid.  getMatrix(P_act, [P_env...]) returns
+    the integral of v[rho_sup] - v[rho_act] in the active basis; getEnergy() the energy
+    E[rho_sup] - E[rho_act] - sum_env E[rho_env] of that build."""
+
+    def __init__(self, active, environment, functional="PBE", precision="FP64", block_ave_threshold=1e-11):
+        if functional not in FUNCTIONALS:
+            raise XCError(1, "unknown functional " + functional)
+        self.active = active
+        self.environment = list(environment)
+        self.functional = functional
+        self.precision = precision
+        self.blockAveThreshold = block_ave_threshold
+        self._energy = None
+
+    def _env_handles(self):
+        return (C.c_void_p * max(1, len(self.environment)))(*[e.handle for e in self.environment])
+
+    def getMatrix(self, P_active, P_environment):
+        n = self.active.getNBasisFunctions()
+        Pa = np.asfortranarray(P_active, dtype=np.float64)
+        if Pa.shape != (n, n):
+            raise XCError(1, "NAddFuncPotential: active density matrix is %s, basis has %d functions" % (Pa.shape, n))
+        if len(P_environment) != len(self.environment):
+            raise XCError(1, "NAddFuncPotential: %d environment densities for %d environment controllers"
+                          % (len(P_environment), len(self.environment)))
+        Pe = []
+        for ctl, P in zip(self.environment, P_environment):
+            m = ctl.getNBasisFunctions()
+            Pf = np.asfortranarray(P, dtype=np.float64)
+            if Pf.shape != (m, m):
+                raise XCError(1, "NAddFuncPotential: environment density matrix is %s, basis has %d functions"
+                              % (Pf.shape, m))
+            Pe.append(Pf)
+        ptrs = (_lib.dp * max(1, len(Pe)))(*[p.ctypes.data_as(_lib.dp) for p in Pe])
+        F = np.zeros((n, n), order="F")
+        E = C.c_double()
+        check(_lib.lib().xc_nadd_fock_build(self.active.handle, len(Pe), self._env_handles(),
+                                            FUNCTIONALS[self.functional], PRECISIONS[self.precision],
+                                            float(self.blockAveThreshold), Pa.ctypes.data_as(_lib.dp), ptrs,
+                                            F.ctypes.data_as(_lib.dp), C.byref(E)))
+        self._energy = E.value
+        return F
+
+    def getEnergy(self):
+        if self._energy is None:
+            raise XCError(4, "NAddFuncPotential: getMatrix has not been called")
+        return self._energy
+
+    def build_device(self, dP_active, dP_environment, dF, sync=True):
+        """Resident form: device pointers (ctypes c_void_p) for every density matrix and F."""
+        if len(dP_environment) != len(self.environment):
+            raise XCError(1, "NAddFuncPotential: %d environment densities for %d environment controllers"
+                          % (len(dP_environment), len(self.environment)))
+        ptrs = (C.c_void_p * max(1, len(dP_environment)))(*dP_environment)
+        E = C.c_double()
+        check(_lib.lib().xc_nadd_fock_build_device(self.active.handle, len(dP_environment), self._env_handles(),
+                                                   FUNCTIONALS[self.functional], PRECISIONS[self.precision],
+                                                   float(self.blockAveThreshold), dP_active, ptrs, dF,
+                                                   C.byref(E) if sync else None))
         if sync:
             self._energy = E.value
             return E.value

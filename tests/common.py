@@ -51,3 +51,18 @@ def load_json(name):
 
 def fixtures():
     return np.load(os.path.join(GOLDEN, "fixtures.npz"))
+
+
+def h2_fde_system():
+    """Active and environment H2 of TestSystem_H2_6_31Gs_{ACTIVE,ENVIRONMENT}_FDE on the
+    active system's grid (SSF, accuracy 4, weight threshold 1e-14)."""
+    from serenity_amd import xc
+    fx = fixtures()
+    out = {}
+    for tag in ("act", "env"):
+        Z, X = fx["h2fde_%s_Z" % tag], fx["h2fde_%s_xyz_angstrom" % tag] * xc.ANGSTROM_TO_BOHR
+        out[tag] = (xc.BasisController.from_file(GOLDEN + "/H-6-31GS", Z, X, spherical=False),
+                    fx["h2fde_%s_P" % tag])
+        if tag == "act":
+            out["grid"] = xc.molecular_grid(Z, X, grid_type="SSF", accuracy=4, weight_threshold=1e-14)
+    return out

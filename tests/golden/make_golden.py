@@ -117,6 +117,17 @@ def kat_funcpot():
     return dict(tolerance=1e-6, functional="LDA", F=np.array(F).reshape(12, 12).tolist())
 
 
+def kat_nadd():
+    """Restricted LDA NAdd Fock entries of the H2 dimer (NAddFuncPotential_test.cpp:58-82)."""
+    text = read("src/potentials/NAddFuncPotential_test.cpp")
+    body = text.split("TEST_F(NAddFuncPotentialTest, H2Dimer_FockMatrix_LDA)")[1].split("TEST_F(")[0]
+    F = [dict(row=int(m.group(1)), col=int(m.group(2)), expected=float(m.group(3)))
+         for m in re.finditer(r"EXPECT_NEAR\(F\((\d+),\s*(\d+)\),\s*([-0-9.eE]+),\s*([-0-9.eE]+)\)", body)]
+    tol = float(re.search(r"EXPECT_NEAR\(F\(0, 0\),\s*[-0-9.eE]+,\s*([-0-9.eE]+)\)", body).group(1))
+    return dict(tolerance=tol, functional="LDA", basis="6-31GS", grid="SSF accuracy 4 of the active system",
+                entries=F)
+
+
 def h5_dataset(path, name):
     """Dataset `name` of an HDF5 file as a float64 array (h5dump binary export)."""
     import tempfile
@@ -155,7 +166,8 @@ def fixtures():
     base = os.path.join(REF, "data/testresources")
     out = {}
     for tag, d in (("h2", "TestSystem_H2_DEF2_TZVP"), ("he", "TestSystem_He_Def2-TZVP_DFT"),
-                   ("water", "TestSystem_Water_Def2-TZVP_DFT")):
+                   ("water", "TestSystem_Water_Def2-TZVP_DFT"), ("h2fde_act", "TestSystem_H2_6_31Gs_ACTIVE_FDE"),
+                   ("h2fde_env", "TestSystem_H2_6_31Gs_ENVIRONMENT_FDE")):
         P = h5_dataset(os.path.join(base, d, d + ".dmat.res.h5"), "densityMatrix")
         sym, pos = xyz(os.path.join(base, d, d + ".xyz"))
         e = energies(os.path.join(base, d, d + ".energies.res"))
@@ -202,22 +214,29 @@ def hilbert_case():
                 wref=row_major("wref", 1, 8).ravel().tolist(), pref=row_major("pref", 3, 8).T.tolist())
 
 
-def sc_basis():
-    text = read("data/basis/CC-PV5Z")
-    m = re.search(r"\nsc\s+CC-PV5Z\s*\n\*\n(.*?)\n\*", text, re.S)
-    with open(os.path.join(HERE, "SC-CC-PV5Z"), "w") as f:
-        f.write("# Sc entry of Serenity data/basis/CC-PV5Z (Turbomole format)\n$basis\n*\nsc   CC-PV5Z\n*\n")
+def basis_entry(src, element, out):
+    """One element's entry of a Serenity basis file, kept in Turbomole format."""
+    text = read("data/basis/" + src)
+    m = re.search(r"\n%s\s+%s\s*\n\*\n(.*?)\n\*" % (element, re.escape(src)), text, re.S)
+    with open(os.path.join(HERE, out), "w") as f:
+        f.write("# %s entry of Serenity data/basis/%s (Turbomole format)\n$basis\n*\n%s   %s\n*\n"
+                % (element.capitalize(), src, element, src))
         f.write(m.group(1).strip("\n") + "\n*\n$end\n")
+
+
+def sc_basis():
+    basis_entry("CC-PV5Z", "sc", "SC-CC-PV5Z")
+    basis_entry("6-31GS", "h", "H-6-31GS")
 
 
 if __name__ == "__main__":
     for name, fn in (("kat_ao.json", kat_ao), ("kat_density.json", kat_density), ("kat_fock.json", kat_fock),
                      ("kat_funcpot_h2.json", kat_funcpot), ("grid_factory.json", grid_factory_cases),
-                     ("hilbert.json", hilbert_case)):
+                     ("hilbert.json", hilbert_case), ("kat_nadd_h2.json", kat_nadd)):
         data = fn()
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(data, f, indent=1)
         print(name, len(data.get("entries", data.get("F", data.get("cases", [])))))
     fixtures()
     sc_basis()
-    print("fixtures.npz, SC-CC-PV5Z written")
+    print("fixtures.npz, SC-CC-PV5Z, H-6-31GS written")

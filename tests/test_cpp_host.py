@@ -1,0 +1,135 @@
+"""The Serenity-shaped C++ host classes (include/serenity_xc.hpp) driven by
+tests/cpp/cpp_parity: the reference's known-answer cases and a FuncPotential
+build, each checked against the golden vectors and the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import common
+from oracle import oracle as O
+from serenity_amd import xc
+from serenity_amd import workloads as W
+
+DRIVER = os.path.join(common.REPO, "tests", "cpp", "cpp_parity")
+
+
+def test_cpp_driver_links_library():
+    out = subprocess.run([DRIVER, "--version"], check=True, capture_output=True, text=True).stdout
+    assert "xc_version 1" in out
+
+
+def _write(path, sections, extra=""):
+    with open(path, "w") as f:
+        for k, v in sections.items():
+            v = np.asarray(v, dtype=np.float64).ravel()
+            f.write("%s %d\n" % (k, v.size))
+            f.write("\n".join("%.17g" % x for x in v) + "\n")
+        f.write(extra)
+
+
+def _read(path):
+    out, toks = {}, open(path).read().split()
+    i = 0
+    while i < len(toks):
+        name, n = toks[i], int(toks[i + 1])
+        out[name] = np.array([float(x) for x in toks[i + 2:i + 2 + n]])
+        i += 2 + n
+    return out
+
+
+def _basis_sections(b):
+    return dict(shell_l=b.l, shell_pure=b.pure, shell_nprim=b.nprim, shell_center=b.center, alpha=b.alpha,
+                coef=b.coef)
+
+
+def _run(case, sections, tmp_path, extra=""):
+    inp, outp = str(tmp_path / (case + ".in")), str(tmp_path / (case + ".out"))
+    _write(inp, sections, extra)
+    print("[gpu-test] cpp_parity", case, flush=True)
+    r = subprocess.run([DRIVER, case, inp, outp], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return _read(outp)
+
+
+@pytest.mark.gpu
+def test_cpp_bfogc_known_answers(tmp_path):
+    kat = common.load_json("kat_ao.json")
+    pts, w = common.tiny_grid()
+    b = common.small_mixed_basis()
+    res = _run("bfogc", dict(_basis_sections(b), points=pts, weights=w, block_size=[kat["block_size"]],
+                             radial_threshold=[kat["radial_threshold"]]), tmp_path)
+    vals = res["values"].reshape(b.nao, 4).T
+    grads = [res["grad" + c].reshape(b.nao, 4).T for c in "xyz"]
+    for e in kat["entries"]:
+        k = e["kind"]
+        if k.startswith("hessian"):
+            continue
+        v = vals[e["point"], e["ao"]] if k == "values" else grads["xyz".index(k[-1])][e["point"], e["ao"]]
+        assert abs(v - e["expected"]) < kat["tolerance"], e
+
+
+@pytest.mark.gpu
+def test_cpp_density_known_answers(tmp_path):
+    kat = common.load_json("kat_density.json")
+    pts, w = common.tiny_grid()
+    b = common.small_mixed_basis()
+    P = np.array(kat["P"])
+    res = _run("density", dict(_basis_sections(b), points=pts, weights=w, block_size=[3], radial_threshold=[0.0],
+                               P=P.ravel(order="F")), tmp_path)
+    grad = res["grad"].reshape(3, 4)
+    for e in kat["entries"]:
+        k = e["kind"]
+        if k.startswith("hess"):
+            continue
+        v = res["rho"][e["point"]] if k == "densOnGrid" else grad["xyz".index(k[-1])][e["point"]]
+        assert abs(v - e["expected"]) < kat["tolerance"], e
+
+
+@pytest.mark.gpu
+def test_cpp_fock_known_answers(tmp_path):
+    kat = common.load_json("kat_fock.json")
+    pts, w = common.very_small_grid()
+    b = common.small_mixed_basis()
+    res = _run("fock", dict(_basis_sections(b), points=pts, weights=w, block_size=[3],
+                            radial_threshold=[kat["radial_threshold"]], block_ave_threshold=[kat["block_ave_threshold"]],
+                            potential=kat["potential"], gradient=np.array(kat["gradient"])), tmp_path)
+    F = res["F"].reshape(b.nao, b.nao, order="F")
+    for e in kat["entries"]:
+        assert abs(F[e["row"], e["col"]] - e["expected"]) < kat["tolerance"], e
+
+
+@pytest.mark.gpu
+def test_cpp_funcpot_matches_oracle(tmp_path):
+    Z, X = W.alkane(3)
+    b = xc.BasisController.from_file("DEF2-SVP", Z, X)
+    g = xc.molecular_grid(Z, X)
+    P = W.synthetic_density(b.overlap(), W.n_electrons(Z) // 2)
+    res = _run("funcpot", dict(Z=Z, xyz=X, spherical=[1], grid_type=[2], accuracy=[4], weight_threshold=[1e-14],
+                               functional=[1], P=P.ravel(order="F")), tmp_path,
+               extra="basis_path %s\n" % os.path.join(xc.BASIS_DIR, "DEF2-SVP"))
+    F = res["F"].reshape(b.nao, b.nao, order="F")
+    Fo, Eo, _, _ = O.fock_build(b, g.points, g.weights, P, "PBE")
+    assert abs(res["energy"][0] - Eo) < 1e-9
+    assert np.abs(F - Fo).max() < 1e-9
+    assert res["threw_on_bad_P"][0] == 1.0
+
+
+@pytest.mark.gpu
+def test_cpp_nadd_known_answers(tmp_path):
+    """NAddFuncPotential through the C++ class: NAddFuncPotential_test.cpp:58-82 (LDA, 1e-5), oracle to 1e-12."""
+    s = common.h2_fde_system()
+    (ba, Pa), (be, Pe), g = s["act"], s["env"], s["grid"]
+    sec = dict(points=g.points, weights=g.weights, functional=[0], P_act=Pa.ravel(order="F"),
+               P_env=Pe.ravel(order="F"))
+    for pre, b in (("act_", ba), ("env_", be)):
+        sec.update({pre + k: v for k, v in _basis_sections(b).items()})
+    res = _run("nadd", sec, tmp_path)
+    F = res["F"].reshape(ba.nao, ba.nao, order="F")
+    kat = common.load_json("kat_nadd_h2.json")
+    for e in kat["entries"]:
+        assert abs(F[e["row"], e["col"]] - e["expected"]) < kat["tolerance"], e
+    Fo, Eo = O.nadd_fock_build(ba, [be], g.points, g.weights, Pa, [Pe], "LDA")
+    assert np.abs(F - Fo).max() < 1e-12 and abs(res["energy"][0] - Eo) < 1e-12
+    assert res["threw_on_missing_env"][0] == 1.0

@@ -1,0 +1,128 @@
+"""Non-additive (FDE) potential through the HIP path against the oracle and the
+reference's known answers.  FP64 bar as for the XC build: |dE| <= 1e-9 Eh and
+max|dF| <= 1e-9 Eh against the oracle on the same grid, bases and densities."""
+import numpy as np
+import pytest
+
+import common
+from oracle import oracle as O
+from serenity_amd import xc
+from serenity_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL_E = 1e-9
+TOL_F = 1e-9
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+_EMB = {}
+
+
+def embedding(nwater, accuracy):
+    key = (nwater, accuracy)
+    if key not in _EMB:
+        _EMB[key] = W.build_embedding(nwater, accuracy=accuracy)
+    return _EMB[key]
+
+
+def controllers(ctx, emb, keep_fp32=False):
+    g = emb["grid"]
+    return [xc.BasisFunctionOnGridController(ctx, s["basis"], g, 128, 1e-9, 1, keep_fp32=keep_fp32)
+            for s in emb["subsystems"]]
+
+
+def test_nadd_h2_lda_known_answers(ctx):
+    """NAddFuncPotential_test.cpp:58-82 (restricted LDA, 1e-5) through the C ABI, and the oracle to 1e-12."""
+    s = common.h2_fde_system()
+    (ba, Pa), (be, Pe), g = s["act"], s["env"], s["grid"]
+    say("NAdd H2 dimer LDA, npts", g.getNGridPoints())
+    bfa = xc.BasisFunctionOnGridController(ctx, ba, g, 128, 1e-9, 1)
+    bfe = xc.BasisFunctionOnGridController(ctx, be, g, 128, 1e-9, 1)
+    pot = xc.NAddFuncPotential(bfa, [bfe], "LDA")
+    F = pot.getMatrix(Pa, [Pe])
+    kat = common.load_json("kat_nadd_h2.json")
+    for e in kat["entries"]:
+        assert abs(F[e["row"], e["col"]] - e["expected"]) < kat["tolerance"], e
+    Fo, Eo = O.nadd_fock_build(ba, [be], g.points, g.weights, Pa, [Pe], "LDA")
+    say("  |dE| %.2e max|dF| %.2e" % (abs(pot.getEnergy() - Eo), np.abs(F - Fo).max()))
+    assert np.abs(F - Fo).max() < 1e-12
+    assert abs(pot.getEnergy() - Eo) < 1e-12
+
+
+@pytest.mark.parametrize("functional", ["PBE", "TF", "LDA"])
+def test_nadd_embedding_matches_oracle(ctx, functional):
+    """Benzene + 24 waters on the supersystem grid (accuracy 4), benzene active, 24 water environments."""
+    emb = embedding(24, 4)
+    g, subs = emb["grid"], emb["subsystems"]
+    say("NAdd C6H6+24H2O", functional, "npts", g.getNGridPoints(), "active nao", subs[0]["basis"].nao)
+    bfs = controllers(ctx, emb, keep_fp32=True)
+    Fo, Eo = O.nadd_fock_build(subs[0]["basis"], [s["basis"] for s in subs[1:]], g.points, g.weights,
+                               subs[0]["P"], [s["P"] for s in subs[1:]], functional)
+    pot = xc.NAddFuncPotential(bfs[0], bfs[1:], functional, "FP64")
+    F = pot.getMatrix(subs[0]["P"], [s["P"] for s in subs[1:]])
+    dE, dF = abs(pot.getEnergy() - Eo), np.abs(F - Fo).max()
+    say("  FP64 E_nadd %.12f |dE| %.2e max|dF| %.2e" % (pot.getEnergy(), dE, dF))
+    assert dE <= TOL_E and dF <= TOL_F
+    assert np.array_equal(F, F.T)
+    pm = xc.NAddFuncPotential(bfs[0], bfs[1:], functional, "FP32")
+    Fm = pm.getMatrix(subs[0]["P"], [s["P"] for s in subs[1:]])
+    dEm, dFm = abs(pm.getEnergy() - Eo), np.abs(Fm - Fo).max()
+    say("  FP32-MFMA |dE| %.2e max|dF| %.2e" % (dEm, dFm))
+    assert dEm < 1e-5 * max(1.0, abs(Eo)) and dFm < 1e-5
+
+
+def test_nadd_water_active_and_device_form(ctx):
+    """A water as the active subsystem (benzene in the environment); the device-resident form equals the
+    host form; the full FaT sweep of active subsystems keeps the controllers' state independent."""
+    emb = embedding(4, 3)
+    g, subs = emb["grid"], emb["subsystems"]
+    bfs = controllers(ctx, emb)
+    for a in range(len(subs)):
+        env = [i for i in range(len(subs)) if i != a]
+        Fo, Eo = O.nadd_fock_build(subs[a]["basis"], [subs[i]["basis"] for i in env], g.points, g.weights,
+                                   subs[a]["P"], [subs[i]["P"] for i in env], "PBE")
+        pot = xc.NAddFuncPotential(bfs[a], [bfs[i] for i in env], "PBE")
+        F = pot.getMatrix(subs[a]["P"], [subs[i]["P"] for i in env])
+        say("  active %d: |dE| %.2e max|dF| %.2e" % (a, abs(pot.getEnergy() - Eo), np.abs(F - Fo).max()))
+        assert abs(pot.getEnergy() - Eo) <= TOL_E and np.abs(F - Fo).max() <= TOL_F
+    # device-resident P and F for the last active subsystem
+    n = subs[a]["basis"].nao
+    dPs = []
+    for i in [a] + env:
+        m = subs[i]["basis"].nao
+        d = ctx.alloc(8 * m * m)
+        ctx.upload(d, np.asfortranarray(subs[i]["P"]).ravel(order="F"))
+        dPs.append(d)
+    dF = ctx.alloc(8 * n * n)
+    E = pot.build_device(dPs[0], dPs[1:], dF)
+    Fd = np.zeros(n * n)
+    ctx.download(Fd, dF)
+    assert np.abs(Fd.reshape(n, n, order="F") - F).max() < 1e-13 and abs(E - pot.getEnergy()) < 1e-13
+    for d in dPs + [dF]:
+        ctx.free(d)
+
+
+def test_nadd_error_paths(ctx):
+    emb = embedding(4, 3)
+    g, subs = emb["grid"], emb["subsystems"]
+    bfs = controllers(ctx, emb)
+    P = [s["P"] for s in subs]
+    with pytest.raises(xc.XCError, match="same"):
+        other = xc.molecular_grid(subs[1]["Z"], subs[1]["X"], accuracy=2)
+        bf_other = xc.BasisFunctionOnGridController(ctx, subs[1]["basis"], other, 128, 1e-9, 1)
+        xc.NAddFuncPotential(bfs[0], [bf_other], "PBE").getMatrix(P[0], [P[1]])
+    with pytest.raises(xc.XCError, match="twice"):
+        xc.NAddFuncPotential(bfs[0], [bfs[1], bfs[1]], "PBE").getMatrix(P[0], [P[1], P[1]])
+    with pytest.raises(xc.XCError, match="also listed"):
+        xc.NAddFuncPotential(bfs[0], [bfs[0]], "PBE").getMatrix(P[0], [P[0]])
+    with pytest.raises(xc.XCError, match="FP32"):
+        xc.NAddFuncPotential(bfs[0], [bfs[1]], "PBE", "FP32").getMatrix(P[0], [P[1]])
+    with pytest.raises(xc.XCError, match="environment densities"):
+        xc.NAddFuncPotential(bfs[0], [bfs[1]], "PBE").getMatrix(P[0], [])
+    # no environment: v[rho_act] - v[rho_act] = 0 (the two inlined evaluations may round differently)
+    pot = xc.NAddFuncPotential(bfs[0], [], "PBE")
+    assert np.abs(pot.getMatrix(P[0], [])).max() < 1e-15 and abs(pot.getEnergy()) < 1e-15

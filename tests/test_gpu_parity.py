@@ -137,6 +137,9 @@ def _system(kind):
     elif kind == "config2_c20h42_svp_pbe":
         Z, X = W.alkane(20)
         func, basis = "PBE", "DEF2-SVP"
+    elif kind == "config3_h2o64_tzvp_pbe":
+        Z, X = W.water_cluster(4, 4, 4)
+        func, basis = "PBE", "DEF2-TZVP"
     else:
         raise ValueError(kind)
     b = xc.BasisController.from_file(basis, Z, X)
@@ -146,7 +149,7 @@ def _system(kind):
 
 
 SYSTEMS = ["h2o_fixture_pbe", "h2o2_svp_lda", "butane_svp_pbe", "h2o4_tzvp_pbe", "config1_h2o8_svp_lda",
-           "config2_c20h42_svp_pbe"]
+           "config2_c20h42_svp_pbe", "config3_h2o64_tzvp_pbe"]
 
 
 @pytest.mark.parametrize("kind", SYSTEMS)

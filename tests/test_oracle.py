@@ -228,3 +228,41 @@ def test_electron_count_invariant():
     r = O.density(basis, grid.points, P, grad=False)
     assert abs(r["rho"] @ grid.weights - np.sum(P * S)) < 5e-4  # grid quadrature error of a random density
     assert abs(np.sum(P * S) - 20.0) < 1e-10
+
+
+def test_nadd_lda_fock_known_answers():
+    """NAddFuncPotential_test.cpp:58-82: restricted LDA non-additive potential of the H2 dimer at 1e-5."""
+    s = common.h2_fde_system()
+    kat = common.load_json("kat_nadd_h2.json")
+    (ba, Pa), (be, Pe), g = s["act"], s["env"], s["grid"]
+    F, E = O.nadd_fock_build(ba, [be], g.points, g.weights, Pa, [Pe], "LDA")
+    for e in kat["entries"]:
+        assert abs(F[e["row"], e["col"]] - e["expected"]) < kat["tolerance"], e
+    # LDA is local: with a zero environment density the non-additive potential vanishes
+    F0, E0 = O.nadd_fock_build(ba, [be], g.points, g.weights, Pa, [np.zeros_like(Pe)], "LDA")
+    assert np.abs(F0).max() < 1e-15 and abs(E0) < 1e-15
+
+
+def test_nadd_sum_rules():
+    """Non-additive potential and energy from the parts: F = F_xc[rho_sup] - F_xc[rho_act] integrated with
+    the active basis, and E = E[sup] - E[act] - E[env], checked through independent full builds."""
+    s = common.h2_fde_system()
+    (ba, Pa), (be, Pe), g = s["act"], s["env"], s["grid"]
+    for name in ("LDA", "PBE", "TF"):
+        F, E = O.nadd_fock_build(ba, [be], g.points, g.weights, Pa, [Pe], name)
+        assert np.array_equal(F, F.T)
+        # E[sup] from a build in the union basis with the block-diagonal density matrix
+        Zs = np.concatenate([common.fixtures()["h2fde_act_Z"], common.fixtures()["h2fde_env_Z"]])
+        Xs = np.concatenate([common.fixtures()["h2fde_act_xyz_angstrom"],
+                             common.fixtures()["h2fde_env_xyz_angstrom"]]) * xc.ANGSTROM_TO_BOHR
+        bs = xc.BasisController.from_file(common.GOLDEN + "/H-6-31GS", Zs, Xs, spherical=False)
+        na = ba.nao
+        Ps = np.zeros((bs.nao, bs.nao))
+        Ps[:na, :na], Ps[na:, na:] = Pa, Pe
+        Fs, Es, _, _ = O.fock_build(bs, g.points, g.weights, Ps, name)
+        _, Ea, _, _ = O.fock_build(ba, g.points, g.weights, Pa, name)
+        _, Ee, _, _ = O.fock_build(be, g.points, g.weights, Pe, name)
+        assert abs(E - (Es - Ea - Ee)) < 1e-12 * max(1.0, abs(Es)), name
+        # the active-active block of the supersystem potential minus the active potential
+        Fa, _, _, _ = O.fock_build(ba, g.points, g.weights, Pa, name)
+        assert np.abs(F - (Fs[:na, :na] - Fa)).max() < 1e-12, name
