@@ -8,7 +8,12 @@ on the device.  Both contraction precisions are timed on the same P: the
 FP32-MFMA path (the headline value) and the all-FP64 path; dE_xc and max|dF|
 of each against the other and against the CPU oracle are reported.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+Beside the headline line the run reports (in the same JSON object) the
+embedding sweep of C6H6 + 24 H2O (non-additive PBE + TF potentials) and a
+serving stream of 2000 Poisson-arriving builds at a fixed QPS (p50/p99 latency,
+builds/s; setup of the resident systems is not timed).
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-fde] [--no-serving]
 """
 import argparse
 import json
@@ -23,6 +28,19 @@ sys.path.insert(0, HERE)
 
 PEAK_TFLOPS = {"FP64": 78.6, "FP32": 157.3}  # MI355X dense matrix peaks
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "xc_fock_build_c20h42_svp_pbe_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per build of each contraction stage, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes summarised in the profile (lines "traffic <stage> <precision> <bytes>"), or {}."""
+    out = {}
+    if os.path.exists(TRAFFIC_PROFILE):
+        for line in open(TRAFFIC_PROFILE):
+            p = line.split()
+            if len(p) == 4 and p[0] == "traffic":
+                out[(p[1], p[2])] = float(p[3])
+    return out
 
 
 def log(*a):
@@ -62,9 +80,11 @@ def cpu_baseline(sysd, min_seconds):
 
 
 def fde_sweep(ctx, steps, warmup, check_oracle):
-    """C6H6 + 24 H2O / def2-SVP on the supersystem grid: one freeze-and-thaw sweep is, for every
-    subsystem as the active one, the PBE non-additive XC and the TF non-additive kinetic potential
-    (50 NAdd builds, FP64), with every density matrix resident on the device."""
+    """C6H6 + 24 H2O / def2-SVP on the supersystem grid (FP64).  One freeze-and-thaw sweep visits every
+    subsystem as the active one: its density is updated (as after its SCF step) and the PBE
+    non-additive XC and TF non-additive kinetic potentials are built on the cached subsystem
+    densities (25 density updates + 50 NAdd builds), every density matrix resident on the device.
+    A direct NAdd build (all 25 densities recomputed inside the call) is timed beside it."""
     from serenity_amd import xc, workloads as W
     emb = W.build_embedding(24, accuracy=4)
     g, subs = emb["grid"], emb["subsystems"]
@@ -76,15 +96,19 @@ def fde_sweep(ctx, steps, warmup, check_oracle):
         ctx.upload(d, np.asfortranarray(s["P"]).ravel(order="F"))
         dP.append(d)
         dF.append(ctx.alloc(8 * n * n))
-    pots = []
+    steps_a = []
     for a in range(len(subs)):
         env = [i for i in range(len(subs)) if i != a]
-        for f in ("PBE", "TF"):
-            pots.append((xc.NAddFuncPotential(bfs[a], [bfs[i] for i in env], f, "FP64"), a, env))
+        steps_a.append((a, env, [xc.NAddFuncPotential(bfs[a], [bfs[i] for i in env], f, "FP64")
+                                 for f in ("PBE", "TF")]))
+    for i in range(len(subs)):
+        bfs[i].update_density(dP[i])
 
     def sweep():
-        for pot, a, env in pots:
-            pot.build_device(dP[a], [dP[i] for i in env], dF[a], sync=False)
+        for a, env, pots in steps_a:
+            bfs[a].update_density(dP[a])
+            for pot in pots:
+                pot.build_cached(dF[a], sync=False)
 
     for _ in range(warmup):
         sweep()
@@ -94,15 +118,22 @@ def fde_sweep(ctx, steps, warmup, check_oracle):
         sweep()
     ctx.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    a, env, pots = steps_a[0]
+    direct = pots[0]
+    direct.build_device(dP[a], [dP[i] for i in env], dF[a], sync=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        direct.build_device(dP[a], [dP[i] for i in env], dF[a], sync=False)
+    ctx.synchronize()
+    ms_direct = (time.perf_counter() - t0) * 1e3 / steps
     out = dict(workload="C6H6+24H2O/def2-SVP, PBE + TF non-additive potentials, supersystem SSF accuracy-4 grid, "
                         "25 subsystems, synthetic subsystem densities",
                npoints=g.getNGridPoints(), nao_total=int(sum(s["basis"].nao for s in subs)),
-               nadd_builds_per_sweep=len(pots), ms_per_sweep=round(ms, 4),
-               ms_per_nadd_build=round(ms / len(pots), 5))
+               density_updates_per_sweep=len(subs), nadd_builds_per_sweep=2 * len(subs), ms_per_sweep=round(ms, 4),
+               ms_direct_nadd_build=round(ms_direct, 4))
     if check_oracle:
         from oracle import oracle as O
-        pot, a, env = pots[0]
-        E = pot.build_device(dP[a], [dP[i] for i in env], dF[a], sync=True)
+        E = pots[0].build_cached(dF[a], sync=True)
         n = subs[a]["basis"].nao
         F = np.zeros(n * n)
         ctx.download(F, dF[a])
@@ -124,6 +155,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fde", action="store_true", help="skip the embedding (non-additive potential) sweep")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serving-requests", type=int, default=2000)
+    ap.add_argument("--serving-qps", type=float, default=400.0)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="issue one device copy of this size first (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -176,12 +210,16 @@ def main():
     # algorithmic work: 2 sum_b B_b n_b^2 flops per contraction (phi.P and phi^T.G)
     flops_per_kernel = 2.0 * sum_Bnb2
 
+    traffic = measured_traffic() if args.config == 2 else {}
+
     def roofline(prec):
         st = results[prec]["stages"]
         kern = max(("density", "fock"), key=lambda k: st[k])
         achieved = flops_per_kernel / (st[kern] * 1e-3) / 1e12
+        tb = traffic.get((kern, prec))
         return dict(bound="mfma", kernel=kern, achieved=round(achieved, 3), peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
-                    frac=round(achieved / PEAK_TFLOPS[prec], 5), traffic=None,
+                    frac=round(achieved / PEAK_TFLOPS[prec], 5), traffic=tb,
+                    hbm_gbs=round(tb / (st[kern] * 1e-3) / 1e9, 1) if tb else None,
                     flops_per_launch=flops_per_kernel, avg_launch_ms=round(st[kern], 5))
 
     r64, r32 = results["FP64"], results["FP32"]
@@ -215,9 +253,23 @@ def main():
 
     if not args.no_fde:
         fde = fde_sweep(ctx, max(1, args.steps // 4), 1, not args.no_cpu_baseline)
-        log("FDE sweep: %.3f ms (%d NAdd builds, %.4f ms each)" %
-            (fde["ms_per_sweep"], fde["nadd_builds_per_sweep"], fde["ms_per_nadd_build"]))
+        log("FDE sweep: %.3f ms (%d density updates + %d cached NAdd builds); direct NAdd build %.3f ms" %
+            (fde["ms_per_sweep"], fde["density_updates_per_sweep"], fde["nadd_builds_per_sweep"],
+             fde["ms_direct_nadd_build"]))
         out_extra["fde"] = fde
+
+    if not args.no_serving:
+        from serenity_amd import serving as SV
+        stream = SV.make_stream(args.serving_requests, args.serving_qps)
+        sv = SV.run_stream(ctx, stream, nslots=4, log=log)
+        energies, host = sv.pop("_energies"), sv.pop("_host")
+        sv["max_dE_xc_fp32_vs_fp64"], sv["fp64_sample"] = SV.fp64_check(ctx, host, energies, 8)
+        sv["workload"] = ("Poisson stream: (H2O)_n n~U{1..48} / CkH2k+2 k~U{1..30}, def2-SVP/TZVP, LDA/PBE, "
+                          "FP32-MFMA builds on resident FP32 tiles")
+        log("serving: %d requests at %.0f QPS: p50 %.3f ms, p99 %.3f ms, %.1f builds/s, max dE vs FP64 %.2e" %
+            (sv["requests"], sv["qps_offered"], sv["p50_ms"], sv["p99_ms"], sv["builds_per_s"],
+             sv["max_dE_xc_fp32_vs_fp64"]))
+        out_extra["serving"] = sv
 
     line = dict(
         metric="XC Fock-build grid pts/s (FP32-MFMA contractions, FP64 functional; FP64 path and dE stated)",

@@ -30,7 +30,8 @@ enum {
   XC_ERR_DEVICE = 2,   /* HIP runtime error */
   XC_ERR_MEMORY = 3,   /* device footprint above the admitted budget */
   XC_ERR_STATE = 4,    /* call order / missing data */
-  XC_ERR_INPUT = 5     /* bad input data (parse error, duplicated atoms, ...) */
+  XC_ERR_INPUT = 5,    /* bad input data (parse error, duplicated atoms, ...) */
+  XC_BUSY = 6          /* server: no free slot, or the controller has a build in flight (retry) */
 };
 
 /* Functionals available on the device.  LDA = Slater + VWN5, PBE = PBE
@@ -51,6 +52,7 @@ typedef struct xc_ctx xc_ctx;       /* one device + one stream */
 typedef struct xc_basis xc_basis;   /* contracted shells, resident on the device */
 typedef struct xc_grid xc_grid;     /* grid points and weights, resident on the device */
 typedef struct xc_bfogc xc_bfogc;   /* basis functions on grid blocks: screening lists + resident AO tiles */
+typedef struct xc_server xc_server; /* asynchronous builds on resident controllers (serving) */
 
 /* ---- errors, version ---------------------------------------------------- */
 const char* xc_last_error(void);
@@ -121,7 +123,9 @@ int xc_grid_destroy(xc_grid* grid);
  * block screening and evaluates the AO values (and gradients if
  * highest_derivative >= 1) of every block into resident HBM tiles.
  * highest_derivative must be 0 or 1 (the Fock build needs at most 1).
- * keep_fp32 additionally keeps an FP32 copy of the tiles for XC_PREC_FP32. */
+ * keep_fp32: 0 = FP64 tiles only, 1 = FP64 and FP32 tiles (XC_PREC_FP32 builds),
+ * 2 = FP32 tiles only (XC_PREC_FP32 builds; a third of the footprint, for
+ * serving many resident systems). */
 int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double radial_threshold,
                     int highest_derivative, int keep_fp32, xc_bfogc** out);
 int xc_bfogc_destroy(xc_bfogc* bfogc);
@@ -185,10 +189,42 @@ int xc_last_energy(xc_ctx* ctx, double* energy);
 int xc_nadd_fock_build(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
                        double block_ave_threshold, const double* P_active, const double* const* P_env, double* F,
                        double* energy);
+/* DensityMatrixDensityOnGridController::calculateData (DensityMatrixDensityOnGrid
+ * Controller.cpp:134-170), explicit form of its lazy cache: computes rho (and
+ * grad rho if with_gradient) of the device density matrix dP and keeps them
+ * resident in the controller until the next update.  Clears the controller's
+ * cached energies. */
+int xc_bfogc_update_density(xc_bfogc* bfogc, int precision, const double* dP, int with_gradient);
+/* NAddFuncPotential::getMatrix on the cached densities of every subsystem (the
+ * reference's DensityOnGridControllers and NAddEnergyHelper keep theirs until
+ * the density matrix changes, NAddFuncPotential.cpp:507-514).  E[rho_env] is
+ * computed once per environment controller, functional and density update.
+ * Results equal xc_nadd_fock_build_device on the same densities. */
+int xc_nadd_fock_build_cached(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
+                              double block_ave_threshold, double* dF, double* energy);
 /* Same on device-resident P and F (energy as for xc_fock_build_device). */
 int xc_nadd_fock_build_device(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
                               double block_ave_threshold, const double* dP_active, const double* const* dP_env,
                               double* dF, double* energy);
+
+/* ---- serving ------------------------------------------------------------ */
+/* Asynchronous XC Fock builds on resident controllers (no counterpart in the
+ * reference, whose builds are synchronous calls of FuncPotential::getMatrix):
+ * nslots builds overlap on the device, each on its own stream with pinned
+ * staging of P and F.  With use_graphs, the first build of a controller for a
+ * (functional, precision, threshold) is captured into a HIP graph and later
+ * ones replay it.  A controller may have one build in flight at a time. */
+int xc_server_create(xc_ctx* ctx, int nslots, int use_graphs, xc_server** out);
+int xc_server_destroy(xc_server* server);
+/* Starts F = V_xc[P], energy = E_xc[P] (host P, F and energy, written when the
+ * ticket completes).  Returns XC_BUSY without side effects when no slot is free
+ * or the controller has a build in flight. */
+int xc_server_submit(xc_server* server, xc_bfogc* bfogc, int functional, int precision,
+                     double block_ave_threshold, const double* P, double* F, double* energy, int64_t* ticket);
+/* done = 1 once the build has finished and F / energy are written. */
+int xc_server_poll(xc_server* server, int64_t ticket, int* done);
+int xc_server_wait(xc_server* server, int64_t ticket);
+int xc_server_stats(const xc_server* server, int* busy_slots, int64_t* submitted, int64_t* graph_launches);
 
 /* Stage timing with HIP events on the context stream.  While enabled, every
  * build records an event between its stages; xc_profile_collect waits for

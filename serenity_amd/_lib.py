@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libserenity_xc.so")
 
 XC_OK = 0
+XC_BUSY = 6
 FUNCTIONALS = {"LDA": 0, "PBE": 1, "TF": 2, "SLATER": 3, "VWN5": 4, "PBEX": 5, "PBEC": 6}
 GGA = {"PBE", "PBEX", "PBEC"}
 PRECISIONS = {"FP64": 0, "FP32": 1}
@@ -81,6 +82,14 @@ def lib():
     _sig("xc_last_energy", C.c_int, vp, dp)
     _sig("xc_nadd_fock_build", C.c_int, vp, C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.c_double, dp,
          C.POINTER(dp), dp, dp)
+    _sig("xc_server_create", C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp))
+    _sig("xc_server_destroy", C.c_int, vp)
+    _sig("xc_server_submit", C.c_int, vp, vp, C.c_int, C.c_int, C.c_double, dp, dp, dp, C.POINTER(C.c_int64))
+    _sig("xc_server_poll", C.c_int, vp, C.c_int64, C.POINTER(C.c_int))
+    _sig("xc_server_wait", C.c_int, vp, C.c_int64)
+    _sig("xc_server_stats", C.c_int, vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int64))
+    _sig("xc_bfogc_update_density", C.c_int, vp, C.c_int, vp, C.c_int)
+    _sig("xc_nadd_fock_build_cached", C.c_int, vp, C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.c_double, vp, dp)
     _sig("xc_nadd_fock_build_device", C.c_int, vp, C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.c_double, vp,
          C.POINTER(vp), vp, dp)
     _sig("xc_profile", C.c_int, vp, C.c_int)

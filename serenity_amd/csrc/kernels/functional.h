@@ -127,6 +127,7 @@ SXC_HD D2 tf_k(D2 rho) {
 }
 
 // Functional ids as in serenity_xc.h.
+constexpr int kNumFunctionals = 7;  // LDA, PBE, TF, SLATER, VWN5, PBEX, PBEC
 SXC_HD bool functional_is_gga(int f) { return f == 1 || f == 5 || f == 6; }
 
 SXC_HD D2 eval_functional(int f, D2 rho, D2 sigma, const XcParams& p) {

@@ -42,6 +42,21 @@ struct DevBlock {
   int pad_;
 };
 
+// Work item of the sub-tile Fock kernel: the block's AO rows [a0, a0 + na)
+// (A side) and [b0, b0 + nb) (B side), multiples of 16.  nb == 0 marks a
+// diagonal item (B = A, sub-tiles i <= j).  nsub = 16 x 16 sub-tiles.
+struct FockItem {
+  int block;
+  int a0, na;
+  int b0, nb;
+  int nsub;
+};
+constexpr int kFockRows = 256;        // rows a sub-tile item may stage
+constexpr int kFockPanel = 128;       // panel height of blocks split into panel pairs
+constexpr int kFockNumS = 12;         // slot-count classes of the sub-tile kernel
+constexpr int kFockS[kFockNumS] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 17};
+inline int fock_smax(int fp64) { return fp64 ? 12 : 17; }  // register budget of the accumulators
+
 // Size classes of the macro-tile Fock kernel (blocks too wide for the pair
 // kernel), chosen from npad: class 0: npad <= 32, class 1: <= 64, class 2: 128-wide macro-tiles.
 constexpr int kNumClasses = 3;
@@ -70,6 +85,13 @@ template <typename T>
 void launch_density(hipStream_t s, int nct, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
                     const T* tiles, const T* P, int n, int want_grad, double* part, int64_t npts);
 
+// Sub-tile density kernel: 8 wavefronts, S = pass columns / 16 (1..kDensMaxS).
+constexpr int kDensMaxS = 12;
+template <typename T>
+void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
+                        const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
+                        int64_t npts);
+
 // rho[q] / grad[.][q] from the partials of the block of q (any block size).
 void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
                             int want_grad, double* rho, double* grad, int accumulate = 0);
@@ -90,6 +112,11 @@ template <typename T>
 void launch_fock_pairs(hipStream_t s, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
                        const T* tiles, const double* vs, const double* gw, int gga, double block_ave_thr, double* U,
                        int n, int64_t npts);
+// Sub-tile Fock kernel: 8 wavefronts, slot class sc (kFockS[sc] sub-tiles per wavefront).
+template <typename T>
+void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,
+                     const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
+                     double block_ave_thr, double* U, int n, int64_t npts);
 int fock_pairs_rmax(int fp64);  // rows a pair-kernel work item may stage
 int fock_pairs_maxp(int fp64);  // pairs per wavefront
 
@@ -97,6 +124,19 @@ void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const
                              double* vs, double* gw);
 void launch_fill(hipStream_t s, double* p, int64_t n, double value);
 void launch_copy(hipStream_t s, const void* src, void* dst, int64_t n32);  // n32 units of 32 bytes
+
+// Up to kListMax device arrays passed by value (kernel argument).
+constexpr int kListMax = 32;
+struct DensityList {
+  const double* rho[kListMax];
+  const double* grad[kListMax];
+  int n;
+};
+// rho (+)= sum of list.rho[i] in list order; grad likewise (3 x npts each).
+void launch_sum_densities(hipStream_t s, const DensityList& list, int64_t npts, int want_grad, int accumulate,
+                          double* rho, double* grad);
+// out[0] -= list.rho[i][0] for i in order.
+void launch_subtract_scalars(hipStream_t s, const DensityList& list, double* out);
 void launch_to_float(hipStream_t s, const double* src, float* dst, int64_t n);
 // F[i + n j] (+)= U[max(i,j) + n min(i,j)] (U holds the lower triangle)
 void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accumulate);

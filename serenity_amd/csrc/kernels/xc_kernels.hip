@@ -11,6 +11,8 @@
 //               triangle with FP64 atomics (ScalarOperatorToMatrixAdder.cpp:179-303)
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "mfma.h"
 #include "xc_internal.h"
 
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
     if (fabs(R) < thr) {
       for (int f = 0; f < sh.nfunc; ++f)
         for (int c = 0; c < ncomp; ++c) {
-          t64[base + (int64_t)f * kTile + c * cstride] = 0.0;
+          if (t64) t64[base + (int64_t)f * kTile + c * cstride] = 0.0;
           if (t32) t32[base + (int64_t)f * kTile + c * cstride] = 0.0f;
         }
       continue;
@@ -127,15 +129,17 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
         }
       const int64_t o = base + (int64_t)f * kTile;
       const double val = R * Y;
-      t64[o] = val;
+      if (t64) t64[o] = val;
       if (t32) t32[o] = (float)val;
       if (ncomp > 1) {
         const double gx = R * Yx + dR * dx * Y;
         const double gy = R * Yy + dR * dy * Y;
         const double gz = R * Yz + dR * dz * Y;
-        t64[o + cstride] = gx;
-        t64[o + 2 * cstride] = gy;
-        t64[o + 3 * cstride] = gz;
+        if (t64) {
+          t64[o + cstride] = gx;
+          t64[o + 2 * cstride] = gy;
+          t64[o + 3 * cstride] = gz;
+        }
         if (t32) {
           t32[o + cstride] = (float)gx;
           t32[o + 2 * cstride] = (float)gy;
@@ -281,6 +285,162 @@ __global__ __launch_bounds__(256) void density_kernel(const WorkItem* __restrict
       dst[npts] = gxa + red[1][p];
       dst[2 * npts] = gya + red[2][p];
       dst[3 * npts] = gza + red[3][p];
+    }
+  }
+}
+
+// Density contraction on 16 x 16 sub-tiles.  One workgroup of 8 wavefronts
+// per (block, column pass): T[p][nu] = sum_mu phi[mu][p] P[mu][nu] for the
+// pass's 16 S columns, wavefront w owning points 16 w .. 16 w + 15 and all S
+// column tiles (S independent accumulators).  The AO (K) dimension streams in
+// chunks of KC (16 FP32 / 8 FP64) through two LDS buffers that hold phi
+// point-major and the pass's P columns, both with the AO index fastest: with
+// k-step t of lane group g taking AO KV g + t (KV = KC / 4) every MFMA operand
+// of a chunk is one 16-byte LDS read.  The global loads of chunk k + 1 are
+// issued before the MFMAs of chunk k.  The pass's partial rho and
+// sum_nu T grad phi are then formed by dotting T (transposed through LDS)
+// with phi / grad phi read coalesced from the tiles.
+template <typename T, int S>
+__global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __restrict__ work,
+                                                          const DevBlock* __restrict__ blocks,
+                                                          const int* __restrict__ ao_idx,
+                                                          const T* __restrict__ tiles, const T* __restrict__ P,
+                                                          int n, int want_grad, double* __restrict__ part,
+                                                          int64_t npts) {
+  using M = Mfma<T>;
+  constexpr int KC = sizeof(T) == 8 ? 8 : 16;
+  constexpr int KV = KC / 4;
+  constexpr int RS = KC + 16 / (int)sizeof(T);  // 80-byte rows
+  constexpr int NC = 16 * S;
+  constexpr int NB = (4 * NC + 511) / 512;      // P staging tasks per thread
+  constexpr int GS = 4;
+  constexpr int CG = 32;                        // columns per transposition round of the dot phase
+  constexpr int PS = kTile + 1;                 // odd row stride of the transposed T
+  constexpr int STAGE_BYTES = 2 * (kTile + NC) * RS * (int)sizeof(T);
+  constexpr int DOT_BYTES = CG * PS * (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(KV)));
+  __shared__ __attribute__((aligned(16))) unsigned char smem[STAGE_BYTES > DOT_BYTES ? STAGE_BYTES : DOT_BYTES];
+  __shared__ double redv[4][4][kTile];
+  T(*sA)[kTile][RS] = reinterpret_cast<T(*)[kTile][RS]>(smem);
+  T(*sB)[NC][RS] = reinterpret_cast<T(*)[NC][RS]>(smem + 2 * kTile * RS * sizeof(T));
+
+  const WorkItem wk = work[blockIdx.x];
+  const DevBlock bk = blocks[wk.block];
+  const int col0 = wk.a, ncols = wk.b;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r = lane & 15, g = lane >> 4;
+  const int npad = bk.npad;
+  const T* tb = tiles + bk.tile_off;
+  const int* idx = ao_idx + bk.idx_off;
+  const int64_t cstride = (int64_t)npad * kTile;
+
+  // staging: phi task = (point pa, AO quad qa); P tasks = (column cb, AO quad qb)
+  const int pa = tid & (kTile - 1), qa = tid >> 7;
+  int nu[NB];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    const int task = tid + 512 * m, cb = task >> 2;
+    nu[m] = (cb < ncols) ? idx[col0 + cb] : -1;
+  }
+  vec_t rA, rB[NB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < KV; ++e) rA[e] = tb[(int64_t)(k0 + KV * qa + e) * kTile + pa];
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+      const int qb = (tid + 512 * m) & 3;
+#pragma unroll
+      for (int e = 0; e < KV; ++e) {
+        const int mu = idx[k0 + KV * qb + e];
+        rB[m][e] = (mu >= 0 && nu[m] >= 0) ? P[mu + (int64_t)n * nu[m]] : T(0);
+      }
+    }
+  };
+  auto store = [&](int b) {
+    *reinterpret_cast<vec_t*>(&sA[b][pa][KV * qa]) = rA;
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+      const int task = tid + 512 * m;
+      if ((task >> 2) < NC) *reinterpret_cast<vec_t*>(&sB[b][task >> 2][KV * (task & 3)]) = rB[m];
+    }
+  };
+
+  typename M::acc_t acc[S];
+#pragma unroll
+  for (int c = 0; c < S; ++c) acc[c] = M::zero();
+  const int nch = npad / KC;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int b = ch & 1;
+    if (ch + 1 < nch) load((ch + 1) * KC);
+    const vec_t av = *reinterpret_cast<const vec_t*>(&sA[b][16 * w + r][KV * g]);
+#pragma unroll
+    for (int s0 = 0; s0 < S; s0 += GS) {
+      vec_t bv[GS];
+#pragma unroll
+      for (int u = 0; u < GS; ++u)
+        if (s0 + u < S) bv[u] = *reinterpret_cast<const vec_t*>(&sB[b][16 * (s0 + u) + r][KV * g]);
+#pragma unroll
+      for (int t = 0; t < KV; ++t)
+#pragma unroll
+        for (int u = 0; u < GS; ++u)
+          if (s0 + u < S) acc[s0 + u] = M::step(av[t], bv[u][t], acc[s0 + u]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (ch + 1 < nch) store(b ^ 1);
+    __syncthreads();
+  }
+
+  // dots: thread (point p, column group h of 8) over CG = 32 columns per round
+  T(*sT)[PS] = reinterpret_cast<T(*)[PS]>(smem);
+  const int p = tid & (kTile - 1), h = tid >> 7;
+  double racc = 0.0, gxa = 0.0, gya = 0.0, gza = 0.0;
+#pragma unroll
+  for (int c0 = 0; c0 < NC; c0 += CG) {
+#pragma unroll
+    for (int u = 0; u < CG / 16; ++u) {
+      if (c0 / 16 + u < S) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) sT[16 * u + r][16 * w + M::row(lane, e)] = acc[c0 / 16 + u][e];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CG / 4; ++j) {
+      const int nl = 8 * h + j;
+      const int col = c0 + nl;
+      if (col < ncols) {
+        const double tv = (double)sT[nl][p];
+        const int64_t row = (int64_t)(col0 + col) * kTile + p;
+        racc += tv * (double)tb[row];
+        if (want_grad) {
+          gxa += tv * (double)tb[row + cstride];
+          gya += tv * (double)tb[row + 2 * cstride];
+          gza += tv * (double)tb[row + 3 * cstride];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // sum the four column groups: h = 1..3 through LDS, in a fixed order
+  if (h > 0) {
+    redv[0][h][p] = racc;
+    redv[1][h][p] = gxa;
+    redv[2][h][p] = gya;
+    redv[3][h][p] = gza;
+  }
+  __syncthreads();
+  if (h == 0 && p < bk.npts) {
+    const int64_t q = bk.first + p;
+    double* dst = part + (int64_t)wk.pass * 4 * npts + q;
+    dst[0] = racc + redv[0][1][p] + redv[0][2][p] + redv[0][3][p];
+    if (want_grad) {
+      dst[npts] = gxa + redv[1][1][p] + redv[1][2][p] + redv[1][3][p];
+      dst[2 * npts] = gya + redv[2][1][p] + redv[2][2][p] + redv[2][3][p];
+      dst[3 * npts] = gza + redv[3][1][p] + redv[3][2][p] + redv[3][3][p];
     }
   }
 }
@@ -737,6 +897,188 @@ __global__ __launch_bounds__(256) void fock_pairs_kernel(const WorkItem* __restr
   }
 }
 
+// Sub-tile Fock contraction.  One workgroup of 8 wavefronts per work item; the
+// item's 16 x 16 AO sub-tiles (i, j) are dealt to the wavefronts round-robin,
+// S per wavefront, each accumulating S_ij = phi_i^T G_j + G_i^T phi_j over the
+// block's 128 points in registers (dummy slots of a short item compute on row
+// 0 and are dropped).  The points stream in chunks of KC (16 FP32 / 8 FP64):
+// the 16-byte global loads of chunk c + 1 (along the point axis of the
+// AO-major tiles) are issued before the MFMAs of chunk c, and a chunk is
+// turned into phi and G = (w v / 2) phi + sum_k (w g_k) d_k phi rows of LDS
+// on its way in.  Inside a chunk, k-step t of lane group g takes point
+// KV g + t (KV = KC / 4), so each MFMA operand of a chunk is one 16-byte LDS
+// read of KV consecutive points of one AO row.  MFMAs are issued in groups of
+// four independent accumulators.
+template <typename T, int S>
+__global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restrict__ work,
+                                                       const DevBlock* __restrict__ blocks,
+                                                       const int* __restrict__ ao_idx, const T* __restrict__ tiles,
+                                                       const double* __restrict__ vs, const double* __restrict__ gw,
+                                                       int gga, double thr, double* __restrict__ U, int n,
+                                                       int64_t npts) {
+  using M = Mfma<T>;
+  constexpr int KC = sizeof(T) == 8 ? 8 : 16;
+  constexpr int KV = KC / 4;
+  constexpr int RS = KC + 16 / (int)sizeof(T);  // 80-byte LDS rows: the 16 rows of an operand read hit distinct banks
+  constexpr int NCH = kTile / KC;
+  constexpr int NTASK = kFockRows * 4 / 512;
+  constexpr int GS = sizeof(T) == 8 ? 2 : 4;  // independent accumulators per MFMA group
+  typedef T vec_t __attribute__((ext_vector_type(KV)));
+  __shared__ __attribute__((aligned(16))) T sPhi[kFockRows][RS];
+  __shared__ __attribute__((aligned(16))) T sG[kFockRows][RS];
+  __shared__ double spot[4][kTile];
+  __shared__ double red[2];
+
+  const FockItem it = work[blockIdx.x];
+  const DevBlock bk = blocks[it.block];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (tid < kTile) {
+    const bool in = tid < bk.npts;
+    const int64_t q = bk.first + tid;
+    spot[0][tid] = in ? vs[q] : 0.0;
+    spot[1][tid] = (in && gga) ? gw[q] : 0.0;
+    spot[2][tid] = (in && gga) ? gw[npts + q] : 0.0;
+    spot[3][tid] = (in && gga) ? gw[2 * npts + q] : 0.0;
+    double a = fabs(spot[0][tid]) + fabs(spot[1][tid]) + fabs(spot[2][tid]) + fabs(spot[3][tid]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    if (lane == 0) red[w] = a;
+  }
+  __syncthreads();
+  if ((red[0] + red[1]) / bk.npts < thr) return;
+
+  const int npad = bk.npad;
+  const bool diag = it.nb == 0;
+  const int R = it.na + it.nb;
+  const T* tb = tiles + bk.tile_off;
+  const int64_t cstride = (int64_t)npad * kTile;
+
+  // sub-tile t of the item -> (i, j) in 16-row units (i <= j on a diagonal item)
+  auto decode = [&](int t, int& i, int& j) {
+    i = 0;
+    if (diag) {
+      const int nt = it.na / 16;
+      int start = 0;
+      while (start + (nt - i) <= t) {
+        start += nt - i;
+        ++i;
+      }
+      j = i + (t - start);
+    } else {
+      const int ntb = it.nb / 16;
+      i = t / ntb;
+      j = t - i * ntb;
+    }
+  };
+  // this wavefront's sub-tiles as LDS row offsets (wavefront-uniform)
+  int li[S], lj[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int t = w + 8 * s;
+    int i = 0, j = 0;
+    if (t < it.nsub) decode(t, i, j);
+    li[s] = __builtin_amdgcn_readfirstlane(t < it.nsub ? 16 * i : 0);
+    lj[s] = __builtin_amdgcn_readfirstlane(t < it.nsub ? (diag ? 16 * j : it.na + 16 * j) : 0);
+  }
+
+  vec_t raw[NTASK][4];
+  auto grow_of = [&](int lr) { return lr < it.na ? it.a0 + lr : it.b0 + (lr - it.na); };
+  auto load = [&](int c) {
+    const int p0 = c * KC;
+#pragma unroll
+    for (int m = 0; m < NTASK; ++m) {
+      const int task = tid + 512 * m;
+      if (task < 4 * R) {
+        const T* src = tb + (int64_t)grow_of(task >> 2) * kTile + p0 + KV * (task & 3);
+        raw[m][0] = *reinterpret_cast<const vec_t*>(src);
+        if (gga) {
+          raw[m][1] = *reinterpret_cast<const vec_t*>(src + cstride);
+          raw[m][2] = *reinterpret_cast<const vec_t*>(src + 2 * cstride);
+          raw[m][3] = *reinterpret_cast<const vec_t*>(src + 3 * cstride);
+        }
+      }
+    }
+  };
+  auto store = [&](int c) {
+    const int p0 = c * KC;
+#pragma unroll
+    for (int m = 0; m < NTASK; ++m) {
+      const int task = tid + 512 * m;
+      if (task < 4 * R) {
+        const int lr = task >> 2, q = task & 3;
+        vec_t gv;
+#pragma unroll
+        for (int e = 0; e < KV; ++e) {
+          const int pt = p0 + KV * q + e;
+          double v = 0.5 * spot[0][pt] * (double)raw[m][0][e];
+          if (gga)
+            v += spot[1][pt] * (double)raw[m][1][e] + spot[2][pt] * (double)raw[m][2][e] +
+                 spot[3][pt] * (double)raw[m][3][e];
+          gv[e] = (T)v;
+        }
+        *reinterpret_cast<vec_t*>(&sPhi[lr][KV * q]) = raw[m][0];
+        *reinterpret_cast<vec_t*>(&sG[lr][KV * q]) = gv;
+      }
+    }
+  };
+
+  typename M::acc_t acc[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) acc[s] = M::zero();
+  const int r = lane & 15, g = lane >> 4;
+  load(0);
+  for (int c = 0; c < NCH; ++c) {
+    store(c);
+    __syncthreads();
+    if (c + 1 < NCH) load(c + 1);
+#pragma unroll
+    for (int s0 = 0; s0 < S; s0 += GS) {
+      vec_t aP[GS], aG[GS], bP[GS], bG[GS];
+#pragma unroll
+      for (int u = 0; u < GS; ++u)
+        if (s0 + u < S) {
+          aP[u] = *reinterpret_cast<const vec_t*>(&sPhi[li[s0 + u] + r][KV * g]);
+          aG[u] = *reinterpret_cast<const vec_t*>(&sG[li[s0 + u] + r][KV * g]);
+          bP[u] = *reinterpret_cast<const vec_t*>(&sPhi[lj[s0 + u] + r][KV * g]);
+          bG[u] = *reinterpret_cast<const vec_t*>(&sG[lj[s0 + u] + r][KV * g]);
+        }
+#pragma unroll
+      for (int t = 0; t < KV; ++t)
+#pragma unroll
+        for (int u = 0; u < GS; ++u)
+          if (s0 + u < S) acc[s0 + u] = M::step(aP[u][t], bG[u][t], acc[s0 + u]);
+#pragma unroll
+      for (int t = 0; t < KV; ++t)
+#pragma unroll
+        for (int u = 0; u < GS; ++u)
+          if (s0 + u < S) acc[s0 + u] = M::step(aG[u][t], bP[u][t], acc[s0 + u]);
+      // keep one group's operands live at a time (bounds the register file)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+  }
+
+  const int* idx = ao_idx + bk.idx_off;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int t = w + 8 * s;
+    if (t >= it.nsub) continue;
+    int i, j;
+    decode(t, i, j);
+    const int gi = it.a0 + 16 * i, gj = (diag ? it.a0 : it.b0) + 16 * j;
+    const bool dsub = diag && i == j;
+    const int gc = idx[gj + r];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int gr = idx[gi + M::row(lane, e)];
+      if (gr < 0 || gc < 0) continue;
+      if (dsub && gr > gc) continue;
+      unsafeAtomicAdd(U + gc + (int64_t)n * gr, (double)acc[s][e]);
+    }
+  }
+}
+
 __global__ void weight_potential_kernel(int64_t npts, const double* __restrict__ w, const double* __restrict__ v,
                                         const double* __restrict__ g, double* __restrict__ vs,
                                         double* __restrict__ gw) {
@@ -748,6 +1090,31 @@ __global__ void weight_potential_kernel(int64_t npts, const double* __restrict__
     gw[npts + q] = w[q] * g[npts + q];
     gw[2 * npts + q] = w[q] * g[2 * npts + q];
   }
+}
+
+// rho[q] (+)= sum_i list.rho[i][q] in list order (likewise the gradient).
+__global__ void sum_densities_kernel(DensityList list, int64_t npts, int want_grad, int accumulate,
+                                     double* __restrict__ rho, double* __restrict__ grad) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npts) return;
+  double r = accumulate ? rho[q] : 0.0;
+  for (int i = 0; i < list.n; ++i) r += list.rho[i][q];
+  rho[q] = r;
+  if (want_grad) {
+    for (int c = 0; c < 3; ++c) {
+      double g = accumulate ? grad[c * npts + q] : 0.0;
+      for (int i = 0; i < list.n; ++i) g += list.grad[i][c * npts + q];
+      grad[c * npts + q] = g;
+    }
+  }
+}
+
+// out[0] -= list.rho[0][0], then list.rho[1][0], ... (one thread, fixed order).
+__global__ void subtract_scalars_kernel(DensityList list, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double e = out[0];
+  for (int i = 0; i < list.n; ++i) e -= list.rho[i][0];
+  out[0] = e;
 }
 
 __global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n) {
@@ -837,6 +1204,38 @@ template void launch_density<double>(hipStream_t, int, const WorkItem*, int, con
 template void launch_density<float>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*, const float*,
                                     const float*, int, int, double*, int64_t);
 
+template <typename T>
+void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
+                        const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
+                        int64_t npts) {
+  if (nwork <= 0) return;
+  switch (S) {
+#define SXC_DENS_S(SV)                                                                                        \
+  case SV:                                                                                                    \
+    hipLaunchKernelGGL((density_sub_kernel<T, SV>), dim3(nwork), dim3(512), 0, s, work, blocks, ao_idx, tiles, \
+                       P, n, want_grad, part, npts);                                                          \
+    return;
+    SXC_DENS_S(1)
+    SXC_DENS_S(2)
+    SXC_DENS_S(3)
+    SXC_DENS_S(4)
+    SXC_DENS_S(5)
+    SXC_DENS_S(6)
+    SXC_DENS_S(7)
+    SXC_DENS_S(8)
+    SXC_DENS_S(9)
+    SXC_DENS_S(10)
+    SXC_DENS_S(11)
+    SXC_DENS_S(12)
+#undef SXC_DENS_S
+  }
+  throw std::runtime_error("launch_density_sub: column-tile count out of range");
+}
+template void launch_density_sub<double>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
+                                         const double*, const double*, int, int, double*, int64_t);
+template void launch_density_sub<float>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
+                                        const float*, const float*, int, int, double*, int64_t);
+
 void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
                             int want_grad, double* rho, double* grad, int accumulate) {
   if (npts <= 0) return;
@@ -894,10 +1293,69 @@ template void launch_fock_pairs<float>(hipStream_t, const WorkItem*, int, const 
 int fock_pairs_rmax(int fp64) { return fp64 ? FockPairsCfg<double>::RMAX : FockPairsCfg<float>::RMAX; }
 int fock_pairs_maxp(int fp64) { return fp64 ? FockPairsCfg<double>::MAXP : FockPairsCfg<float>::MAXP; }
 
+template <typename T, int S>
+static void launch_fock_sub_s(hipStream_t s, const FockItem* work, int nwork, const DevBlock* blocks,
+                              const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
+                              double thr, double* U, int n, int64_t npts) {
+  hipLaunchKernelGGL((fock_sub_kernel<T, S>), dim3(nwork), dim3(512), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
+                     thr, U, n, npts);
+}
+
+template <typename T>
+void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,
+                     const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga, double thr,
+                     double* U, int n, int64_t npts) {
+  if (nwork <= 0) return;
+  switch (kFockS[sc]) {
+#define SXC_FOCK_S(SV) \
+  case SV:             \
+    return launch_fock_sub_s<T, SV>(s, work, nwork, blocks, ao_idx, tiles, vs, gw, gga, thr, U, n, npts);
+    SXC_FOCK_S(1)
+    SXC_FOCK_S(2)
+    SXC_FOCK_S(3)
+    SXC_FOCK_S(4)
+    SXC_FOCK_S(5)
+    SXC_FOCK_S(6)
+    SXC_FOCK_S(7)
+    SXC_FOCK_S(8)
+    SXC_FOCK_S(10)
+    SXC_FOCK_S(12)
+#undef SXC_FOCK_S
+    case 14:
+    case 17:
+      if constexpr (sizeof(T) == 4) {
+        switch (kFockS[sc]) {
+          case 14:
+            return launch_fock_sub_s<T, 14>(s, work, nwork, blocks, ao_idx, tiles, vs, gw, gga, thr, U, n, npts);
+          case 17:
+            return launch_fock_sub_s<T, 17>(s, work, nwork, blocks, ao_idx, tiles, vs, gw, gga, thr, U, n, npts);
+        }
+      }
+      break;
+  }
+  throw std::runtime_error("launch_fock_sub: slot class beyond the accumulator budget of this precision");
+}
+template void launch_fock_sub<double>(hipStream_t, int, const FockItem*, int, const DevBlock*, const int*,
+                                      const double*, const double*, const double*, int, double, double*, int,
+                                      int64_t);
+template void launch_fock_sub<float>(hipStream_t, int, const FockItem*, int, const DevBlock*, const int*,
+                                     const float*, const double*, const double*, int, double, double*, int, int64_t);
+
 void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
                              double* vs, double* gw) {
   if (npts <= 0) return;
   hipLaunchKernelGGL(weight_potential_kernel, dim3(grid1d(npts, 256)), dim3(256), 0, s, npts, w, v, g, vs, gw);
+}
+
+void launch_sum_densities(hipStream_t s, const DensityList& list, int64_t npts, int want_grad, int accumulate,
+                          double* rho, double* grad) {
+  if (npts <= 0) return;
+  hipLaunchKernelGGL(sum_densities_kernel, dim3(grid1d(npts, 256)), dim3(256), 0, s, list, npts, want_grad,
+                     accumulate, rho, grad);
+}
+
+void launch_subtract_scalars(hipStream_t s, const DensityList& list, double* out) {
+  hipLaunchKernelGGL(subtract_scalars_kernel, dim3(1), dim3(64), 0, s, list, out);
 }
 
 void launch_copy(hipStream_t s, const void* src, void* dst, int64_t n32) {

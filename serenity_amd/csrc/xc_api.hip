@@ -161,36 +161,43 @@ struct xc_bfogc {
   DBuf<float> d_t32;
   // contraction work lists per size class
   int npass_max = 0;
-  std::vector<WorkItem> dens_work[kMaxNct + 1];          // by NCT
-  std::vector<WorkItem> fock_work[2][kNumClasses];       // macro-tile kernel, [fp64][class]
-  std::vector<WorkItem> pair_work[2];                    // pair-tile kernel, [fp64]
-  DBuf<WorkItem> d_dens_work[kMaxNct + 1], d_fock_work[2][kNumClasses], d_pair_work[2];
+  std::vector<WorkItem> dens_work[kDensMaxS + 1];        // by column tiles per pass
+  std::vector<FockItem> fock_items[2][kFockNumS];       // sub-tile Fock kernel, [fp64][slot class]
+  DBuf<WorkItem> d_dens_work[kDensMaxS + 1];
+  DBuf<FockItem> d_fock_items[2][kFockNumS];
   // per-build work buffers
   DBuf<double> d_part, d_rho, d_grad, d_vs, d_gw, d_eblk, d_U, d_P, d_F, d_v, d_g, d_E;
   DBuf<float> d_P32;
   // supersystem density of a non-additive build with this controller active (allocated on first use)
   DBuf<double> d_rho_sup, d_grad_sup;
+  // cached grid density (xc_bfogc_update_density) and the energies of it per functional
+  DBuf<double> d_rho_c, d_grad_c, d_ecache;
+  bool rho_c_valid = false, rho_c_grad = false;
+  bool ecache_valid[kNumFunctionals] = {};
+  // serving: a build of this controller is in flight on a server slot; captured builds
+  bool in_flight = false;
+  std::map<std::pair<int, double>, hipGraphExec_t> graphs;  // (functional + 16 precision, threshold)
+  ~xc_bfogc() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  }
 };
 
 namespace {
 
 template <typename T>
-void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad) {
-  for (int c = kMaxNct; c >= 1; --c)
-    launch_density<T>(bf->ctx->stream, c, bf->d_dens_work[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
+void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad, hipStream_t s) {
+  for (int c = kDensMaxS; c >= 1; --c)
+    launch_density_sub<T>(s, c, bf->d_dens_work[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
                       bf->d_ao_idx.p, tiles, P, bf->basis->nao, want_grad, bf->d_part.p, bf->grid->npts);
 }
 
 template <typename T>
-void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr) {
+void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr, hipStream_t s) {
   const int f = sizeof(T) == 8 ? 1 : 0;
-  for (int c = kNumClasses - 1; c >= 0; --c)
-    launch_fock<T>(bf->ctx->stream, c, bf->d_fock_work[f][c].p, (int)bf->fock_work[f][c].size(), bf->d_blocks.p,
-                   bf->d_ao_idx.p, tiles, bf->d_vs.p, bf->d_gw.p, gga, thr, bf->d_U.p, bf->basis->nao,
-                   bf->grid->npts);
-  launch_fock_pairs<T>(bf->ctx->stream, bf->d_pair_work[f].p, (int)bf->pair_work[f].size(), bf->d_blocks.p,
-                       bf->d_ao_idx.p, tiles, bf->d_vs.p, bf->d_gw.p, gga, thr, bf->d_U.p, bf->basis->nao,
-                       bf->grid->npts);
+  for (int c = kFockNumS - 1; c >= 0; --c)
+    launch_fock_sub<T>(s, c, bf->d_fock_items[f][c].p, (int)bf->fock_items[f][c].size(),
+                       bf->d_blocks.p, bf->d_ao_idx.p, tiles, bf->d_vs.p, bf->d_gw.p, gga, thr, bf->d_U.p,
+                       bf->basis->nao, bf->grid->npts);
 }
 
 }  // namespace
@@ -475,6 +482,7 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
     require(highest_derivative == 0 || highest_derivative == 1, XC_ERR_ARG,
             "xc_bfogc_create: highest derivative must be 0 or 1");
     require(radial_threshold >= 0.0, XC_ERR_ARG, "xc_bfogc_create: radial threshold must not be negative");
+    require(keep_fp32 >= 0 && keep_fp32 <= 2, XC_ERR_ARG, "xc_bfogc_create: keep_fp32 must be 0, 1 or 2");
     xc_ctx* ctx = basis->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     HIPX(hipSetDevice(ctx->device));
@@ -532,8 +540,8 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
         bf->sum_B_nb2 += (double)bk.npts * cnt * cnt;
         bk.npass = 0;
         if (cnt > 0) {
-          // density: balanced passes of at most 128 columns, 16-column granularity
-          const int npass = (bk.npad + 16 * kMaxNct - 1) / (16 * kMaxNct);
+          // density: balanced passes of at most 16 kDensMaxS columns, 16-column granularity
+          const int npass = (bk.npad + 16 * kDensMaxS - 1) / (16 * kDensMaxS);
           const int width = ((bk.npad + npass - 1) / npass + 15) / 16 * 16;
           int s = 0;
           for (int c0 = 0; c0 < bk.npad; c0 += width, ++s) {
@@ -542,18 +550,29 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
           }
           bk.npass = s;
           bf->npass_max = std::max(bf->npass_max, s);
-          // Fock: pair-tile items where the staged rows fit, else 2D macro-tiles
-          const int nt = bk.npad / 32;
+          // Fock: the whole upper triangle of 16 x 16 sub-tiles as one item where
+          // its accumulators fit, else pairs of 128-row panels
           for (int fp64 = 0; fp64 < 2; ++fp64) {
-            const int rmax = fock_pairs_rmax(fp64), per = 4 * fock_pairs_maxp(fp64);
-            if (bk.npad <= rmax) {
-              const int npairs = nt * (nt + 1) / 2;
-              for (int a = 0; a < npairs; a += per) bf->pair_work[fp64].push_back(WorkItem{b, a, std::min(per, npairs - a), 0});
+            auto add = [&](int a0, int na, int b0, int nbr) {
+              const int ta = na / 16, tb = nbr / 16;
+              const int nsub = nbr == 0 ? ta * (ta + 1) / 2 : ta * tb;
+              int sc = 0;
+              while (8 * kFockS[sc] < nsub) ++sc;
+              bf->fock_items[fp64][sc].push_back(FockItem{b, a0, na, b0, nbr, nsub});
+            };
+            const int nt16 = bk.npad / 16;
+            if (bk.npad <= kFockRows && nt16 * (nt16 + 1) / 2 <= 8 * fock_smax(fp64)) {
+              add(0, bk.npad, 0, 0);
             } else {
-              const int cls = size_class(bk.npad), mw = class_width(cls);
-              const int ng = (bk.npad + mw - 1) / mw;
-              for (int I = 0; I < ng; ++I)
-                for (int J = I; J < ng; ++J) bf->fock_work[fp64][cls].push_back(WorkItem{b, I * mw, J * mw, 0});
+              const int np = (bk.npad + kFockPanel - 1) / kFockPanel;
+              for (int P = 0; P < np; ++P) {
+                const int a0 = P * kFockPanel, na = std::min(kFockPanel, bk.npad - a0);
+                add(a0, na, 0, 0);
+                for (int Q = P + 1; Q < np; ++Q) {
+                  const int b0 = Q * kFockPanel;
+                  add(a0, na, b0, std::min(kFockPanel, bk.npad - b0));
+                }
+              }
             }
           }
         }
@@ -562,7 +581,8 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       // 3. admission by footprint
       size_t free_b = 0, total_b = 0;
       HIPX(hipMemGetInfo(&free_b, &total_b));
-      const double need = (double)tile_off * (sizeof(double) + (keep_fp32 ? sizeof(float) : 0));
+      const double need =
+          (double)tile_off * ((keep_fp32 != 2 ? sizeof(double) : 0) + (keep_fp32 ? sizeof(float) : 0));
       require(need < 0.9 * (double)free_b, XC_ERR_MEMORY,
               "xc_bfogc_create: AO tiles need " + std::to_string(need / 1e9) + " GB, " +
                   std::to_string(free_b / 1e9) + " GB free");
@@ -571,15 +591,16 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       bf->d_ao_idx.upload(ctx, bf->ao_idx.data(), bf->ao_idx.size());
       bf->d_sig_shell.upload(ctx, sig_shell.data(), sig_shell.size());
       bf->d_sig_pos.upload(ctx, sig_pos.data(), sig_pos.size());
-      for (int c = 0; c <= kMaxNct; ++c) bf->d_dens_work[c].upload(ctx, bf->dens_work[c].data(), bf->dens_work[c].size());
-      for (int f = 0; f < 2; ++f) {
-        for (int c = 0; c < kNumClasses; ++c)
-          bf->d_fock_work[f][c].upload(ctx, bf->fock_work[f][c].data(), bf->fock_work[f][c].size());
-        bf->d_pair_work[f].upload(ctx, bf->pair_work[f].data(), bf->pair_work[f].size());
-      }
+      for (int c = 0; c <= kDensMaxS; ++c)
+        bf->d_dens_work[c].upload(ctx, bf->dens_work[c].data(), bf->dens_work[c].size());
+      for (int f = 0; f < 2; ++f)
+        for (int c = 0; c < kFockNumS; ++c)
+          bf->d_fock_items[f][c].upload(ctx, bf->fock_items[f][c].data(), bf->fock_items[f][c].size());
       const size_t nt_el = (size_t)std::max<int64_t>(tile_off, 1);
-      bf->d_t64.alloc(ctx, nt_el);
-      HIPX(hipMemsetAsync(bf->d_t64.p, 0, sizeof(double) * nt_el, ctx->stream));
+      if (keep_fp32 != 2) {
+        bf->d_t64.alloc(ctx, nt_el);
+        HIPX(hipMemsetAsync(bf->d_t64.p, 0, sizeof(double) * nt_el, ctx->stream));
+      }
       if (keep_fp32) {
         bf->d_t32.alloc(ctx, nt_el);
         HIPX(hipMemsetAsync(bf->d_t32.p, 0, sizeof(float) * nt_el, ctx->stream));
@@ -618,6 +639,7 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
 int xc_bfogc_destroy(xc_bfogc* bfogc) {
   return guarded([&] {
     if (!bfogc) return;
+    require(!bfogc->in_flight, XC_ERR_STATE, "xc_bfogc_destroy: a server build of this controller is in flight");
     (void)hipStreamSynchronize(bfogc->ctx->stream);
     --bfogc->ctx->live_handles;
     delete bfogc;
@@ -639,7 +661,8 @@ int xc_bfogc_stats(const xc_bfogc* bf, int64_t* sum_nb, double* sum_B_nb2, int64
     require(bf, XC_ERR_ARG, "null bfogc");
     if (sum_nb) *sum_nb = bf->sum_nb;
     if (sum_B_nb2) *sum_B_nb2 = bf->sum_B_nb2;
-    if (tile_bytes) *tile_bytes = bf->tile_elems * (int64_t)(sizeof(double) + (bf->d_t32.p ? sizeof(float) : 0));
+    if (tile_bytes)
+      *tile_bytes = bf->tile_elems * (int64_t)((bf->d_t64.p ? sizeof(double) : 0) + (bf->d_t32.p ? sizeof(float) : 0));
   });
 }
 
@@ -649,6 +672,7 @@ int xc_bfogc_block_data(const xc_bfogc* bf, int block, double* values, double* g
     require(bf, XC_ERR_ARG, "null bfogc");
     require(block >= 0 && block < bf->nblocks, XC_ERR_ARG, "block index out of range");
     require(!(gx || gy || gz) || bf->deriv >= 1, XC_ERR_STATE, "gradients were not evaluated (highest derivative 0)");
+    require(bf->d_t64.p, XC_ERR_STATE, "xc_bfogc_block_data: the controller keeps FP32 tiles only");
     std::lock_guard<std::recursive_mutex> lk(bf->ctx->mu);
     const DevBlock& bk = bf->blocks[block];
     const int n = bf->basis->nao;
@@ -680,13 +704,15 @@ int xc_bfogc_block_data(const xc_bfogc* bf, int block, double* values, double* g
 int xc_density(xc_bfogc* bf, const double* P, double* rho, double* grad, int* nonneg) {
   return guarded([&] {
     require(bf && P && rho, XC_ERR_ARG, "xc_density: null argument");
+    require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
     require(!grad || bf->deriv >= 1, XC_ERR_STATE, "xc_density: gradient requested but AO gradients not evaluated");
+    require(bf->d_t64.p, XC_ERR_STATE, "xc_density: the controller keeps FP32 tiles only");
     xc_ctx* ctx = bf->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     const int n = bf->basis->nao;
     const int64_t N = bf->grid->npts;
     HIPX(hipMemcpyAsync(bf->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
-    run_density<double>(bf, bf->d_t64.p, bf->d_P.p, grad ? 1 : 0);
+    run_density<double>(bf, bf->d_t64.p, bf->d_P.p, grad ? 1 : 0, ctx->stream);
     launch_reduce_partials(ctx->stream, bf->d_blocks.p, bf->bs, N, bf->d_part.p, grad ? 1 : 0, bf->d_rho.p,
                            bf->d_grad.p);
     HIPX(hipGetLastError());
@@ -739,7 +765,9 @@ int xc_functional(xc_ctx* ctx, int functional, int64_t npts, int block_size, con
 int xc_fock_add(xc_bfogc* bf, double thr, const double* v, const double* g, double* F) {
   return guarded([&] {
     require(bf && v && F, XC_ERR_ARG, "xc_fock_add: null argument");
+    require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
     require(!g || bf->deriv >= 1, XC_ERR_STATE, "xc_fock_add: gradient potential needs AO gradients");
+    require(bf->d_t64.p, XC_ERR_STATE, "xc_fock_add: the controller keeps FP32 tiles only");
     xc_ctx* ctx = bf->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     const int n = bf->basis->nao;
@@ -749,7 +777,7 @@ int xc_fock_add(xc_bfogc* bf, double thr, const double* v, const double* g, doub
     launch_weight_potential(ctx->stream, N, bf->grid->d_w.p, bf->d_v.p, g ? bf->d_g.p : nullptr, bf->d_vs.p,
                             bf->d_gw.p);
     launch_fill(ctx->stream, bf->d_U.p, (int64_t)n * n, 0.0);
-    run_fock<double>(bf, bf->d_t64.p, g ? 1 : 0, thr);
+    run_fock<double>(bf, bf->d_t64.p, g ? 1 : 0, thr, ctx->stream);
     HIPX(hipMemcpyAsync(bf->d_F.p, F, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
     launch_finalize(ctx->stream, bf->d_U.p, bf->d_F.p, n, 1);
     HIPX(hipGetLastError());
@@ -758,25 +786,29 @@ int xc_fock_add(xc_bfogc* bf, double thr, const double* v, const double* g, doub
   });
 }
 
-static void enqueue_build(xc_bfogc* bf, int functional, int precision, double thr, const double* dP, double* dF) {
+// One build enqueued on stream s, E_xc left in d_energy (device).  Stage
+// events are recorded only on the context stream.
+static void enqueue_build(xc_bfogc* bf, int functional, int precision, double thr, const double* dP, double* dF,
+                          hipStream_t s, double* d_energy) {
   xc_ctx* ctx = bf->ctx;
+  require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
   require(functional >= 0 && functional <= 6, XC_ERR_ARG, "unknown functional");
   require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
   const bool gga = functional_is_gga(functional);
   require(!gga || bf->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
   require(precision == XC_PREC_FP64 || bf->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+  require(precision == XC_PREC_FP32 || bf->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
   const int n = bf->basis->nao;
   const int64_t N = bf->grid->npts;
   const int nxc = (int)((N + kTile - 1) / kTile);
-  hipStream_t s = ctx->stream;
-  ctx->mark();
+  if (s == ctx->stream) ctx->mark();
   if (precision == XC_PREC_FP32) launch_to_float(s, dP, bf->d_P32.p, (int64_t)n * n);
-  ctx->mark();
+  if (s == ctx->stream) ctx->mark();
   if (precision == XC_PREC_FP64)
-    run_density<double>(bf, bf->d_t64.p, dP, gga ? 1 : 0);
+    run_density<double>(bf, bf->d_t64.p, dP, gga ? 1 : 0, s);
   else
-    run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0);
-  ctx->mark();
+    run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0, s);
+  if (s == ctx->stream) ctx->mark();
   if (bf->bs == kTile) {
     launch_functional(s, functional, ctx->prm, N, kTile, nullptr, nullptr, bf->d_blocks.p, bf->d_part.p,
                       bf->grid->d_w.p, nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
@@ -785,17 +817,17 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
     launch_functional(s, functional, ctx->prm, N, kTile, bf->d_rho.p, gga ? bf->d_grad.p : nullptr, nullptr,
                       nullptr, bf->grid->d_w.p, nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
   }
-  ctx->mark();
+  if (s == ctx->stream) ctx->mark();
   launch_fill(s, bf->d_U.p, (int64_t)n * n, 0.0);
-  ctx->mark();
+  if (s == ctx->stream) ctx->mark();
   if (precision == XC_PREC_FP64)
-    run_fock<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr);
+    run_fock<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr, s);
   else
-    run_fock<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr);
-  ctx->mark();
+    run_fock<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr, s);
+  if (s == ctx->stream) ctx->mark();
   launch_finalize(s, bf->d_U.p, dF, n, 0);
-  launch_sum(s, bf->d_eblk.p, nxc, ctx->d_energy);
-  ctx->mark();
+  launch_sum(s, bf->d_eblk.p, nxc, d_energy);
+  if (s == ctx->stream) ctx->mark();
   HIPX(hipGetLastError());
 }
 
@@ -827,6 +859,7 @@ static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int func
     require(b->bs == act->bs, XC_ERR_ARG, "non-additive build: controllers differ in block size");
     require(!gga || b->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
     require(!fp32 || b->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+    require(fp32 || b->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
   }
   const int n = act->basis->nao;
   const int64_t N = act->grid->npts;
@@ -845,9 +878,9 @@ static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int func
   for (size_t i = 0; i < sub.size(); ++i) {
     xc_bfogc* b = sub[i];
     if (fp32)
-      run_density<float>(b, b->d_t32.p, b->d_P32.p, gga ? 1 : 0);
+      run_density<float>(b, b->d_t32.p, b->d_P32.p, gga ? 1 : 0, s);
     else
-      run_density<double>(b, b->d_t64.p, Ps[i], gga ? 1 : 0);
+      run_density<double>(b, b->d_t64.p, Ps[i], gga ? 1 : 0, s);
     launch_reduce_partials(s, b->d_blocks.p, b->bs, N, b->d_part.p, gga ? 1 : 0, b->d_rho.p, b->d_grad.p, 0);
     launch_reduce_partials(s, b->d_blocks.p, b->bs, N, b->d_part.p, gga ? 1 : 0, act->d_rho_sup.p,
                            act->d_grad_sup.p, i == 0 ? 0 : 1);
@@ -867,13 +900,136 @@ static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int func
   launch_fill(s, act->d_U.p, (int64_t)n * n, 0.0);
   ctx->mark();
   if (fp32)
-    run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr);
+    run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr, s);
   else
-    run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr);
+    run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr, s);
   ctx->mark();
   launch_finalize(s, act->d_U.p, dF, n, 0);
   ctx->mark();
   HIPX(hipGetLastError());
+}
+
+int xc_bfogc_update_density(xc_bfogc* bf, int precision, const double* dP, int with_gradient) {
+  return guarded([&] {
+    require(bf && dP, XC_ERR_ARG, "xc_bfogc_update_density: null argument");
+    require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+    require(!with_gradient || bf->deriv >= 1, XC_ERR_STATE,
+            "xc_bfogc_update_density: gradient requested but AO gradients not evaluated");
+    require(precision == XC_PREC_FP64 || bf->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+    require(precision == XC_PREC_FP32 || bf->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
+    xc_ctx* ctx = bf->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const int n = bf->basis->nao;
+    const int64_t N = bf->grid->npts;
+    if (!bf->d_rho_c.p) {
+      bf->d_rho_c.alloc(ctx, N);
+      bf->d_grad_c.alloc(ctx, 3 * N);
+      bf->d_ecache.alloc(ctx, kNumFunctionals);
+    }
+    const int g = with_gradient ? 1 : 0;
+    if (precision == XC_PREC_FP32) {
+      launch_to_float(ctx->stream, dP, bf->d_P32.p, (int64_t)n * n);
+      run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, g, ctx->stream);
+    } else {
+      run_density<double>(bf, bf->d_t64.p, dP, g, ctx->stream);
+    }
+    launch_reduce_partials(ctx->stream, bf->d_blocks.p, bf->bs, N, bf->d_part.p, g, bf->d_rho_c.p, bf->d_grad_c.p,
+                           0);
+    HIPX(hipGetLastError());
+    bf->rho_c_valid = true;
+    bf->rho_c_grad = with_gradient != 0;
+    for (bool& v : bf->ecache_valid) v = false;
+  });
+}
+
+int xc_nadd_fock_build_cached(xc_bfogc* act, int nenv, xc_bfogc* const* env, int functional, int precision,
+                              double thr, double* dF, double* energy) {
+  return guarded([&] {
+    require(act && dF, XC_ERR_ARG, "xc_nadd_fock_build_cached: null argument");
+    require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
+    require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+    require(nenv >= 0 && (nenv == 0 || env), XC_ERR_ARG, "non-additive build: null environment list");
+    xc_ctx* ctx = act->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const bool gga = functional_is_gga(functional);
+    require(!gga || act->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
+    require(precision == XC_PREC_FP64 || act->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+    require(precision == XC_PREC_FP32 || act->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
+    std::vector<xc_bfogc*> sub(1, act);
+    for (int i = 0; i < nenv; ++i) {
+      require(env[i], XC_ERR_ARG, "non-additive build: null environment controller");
+      require(env[i] != act, XC_ERR_ARG, "non-additive build: the active controller is also listed as environment");
+      for (int j = 0; j < i; ++j)
+        require(env[j] != env[i], XC_ERR_ARG, "non-additive build: an environment controller is listed twice");
+      sub.push_back(env[i]);
+    }
+    for (xc_bfogc* b : sub) {
+      require(b->ctx == ctx, XC_ERR_ARG, "non-additive build: controllers live on different contexts");
+      require(b->grid == act->grid, XC_ERR_ARG,
+              "non-additive build: all subsystem controllers must be defined on the same (supersystem) grid");
+      require(b->rho_c_valid, XC_ERR_STATE,
+              "xc_nadd_fock_build_cached: a subsystem has no cached density (call xc_bfogc_update_density)");
+      require(!gga || b->rho_c_grad, XC_ERR_STATE,
+              "xc_nadd_fock_build_cached: GGA functional needs cached density gradients");
+    }
+    const int n = act->basis->nao;
+    const int64_t N = act->grid->npts;
+    const int nxc = (int)((N + kTile - 1) / kTile);
+    if (!act->d_rho_sup.p) {
+      act->d_rho_sup.alloc(ctx, N);
+      act->d_grad_sup.alloc(ctx, 3 * N);
+    }
+    hipStream_t s = ctx->stream;
+    const double* w = act->grid->d_w.p;
+    ctx->mark();
+    ctx->mark();
+    // rho_sup = rho_act + rho_env[0] + ... in list order, kListMax arrays per launch
+    for (size_t i0 = 0; i0 < sub.size(); i0 += kListMax) {
+      DensityList L{};
+      L.n = (int)std::min<size_t>(kListMax, sub.size() - i0);
+      for (int k = 0; k < L.n; ++k) {
+        L.rho[k] = sub[i0 + k]->d_rho_c.p;
+        L.grad[k] = sub[i0 + k]->d_grad_c.p;
+      }
+      launch_sum_densities(s, L, N, gga ? 1 : 0, i0 == 0 ? 0 : 1, act->d_rho_sup.p, act->d_grad_sup.p);
+    }
+    ctx->mark();
+    launch_functional(s, functional, ctx->prm, N, kTile, act->d_rho_sup.p, gga ? act->d_grad_sup.p : nullptr, nullptr,
+                      nullptr, w, nullptr, nullptr, nullptr, act->d_vs.p, act->d_gw.p, act->d_eblk.p, act->d_rho_c.p,
+                      gga ? act->d_grad_c.p : nullptr);
+    launch_sum(s, act->d_eblk.p, nxc, ctx->d_energy, 0);
+    // E[rho_env] per environment subsystem, kept until its density changes
+    for (size_t i = 1; i < sub.size(); ++i) {
+      xc_bfogc* b = sub[i];
+      if (b->ecache_valid[functional]) continue;
+      launch_functional(s, functional, ctx->prm, N, kTile, b->d_rho_c.p, gga ? b->d_grad_c.p : nullptr, nullptr,
+                        nullptr, w, nullptr, nullptr, nullptr, nullptr, nullptr, b->d_eblk.p);
+      launch_sum(s, b->d_eblk.p, nxc, b->d_ecache.p + functional, 0);
+      b->ecache_valid[functional] = true;
+    }
+    for (size_t i0 = 1; i0 < sub.size(); i0 += kListMax) {
+      DensityList L{};
+      L.n = (int)std::min<size_t>(kListMax, sub.size() - i0);
+      for (int k = 0; k < L.n; ++k) L.rho[k] = sub[i0 + k]->d_ecache.p + functional;
+      launch_subtract_scalars(s, L, ctx->d_energy);
+    }
+    ctx->mark();
+    launch_fill(s, act->d_U.p, (int64_t)n * n, 0.0);
+    ctx->mark();
+    if (precision == XC_PREC_FP32)
+      run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr, s);
+    else
+      run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr, s);
+    ctx->mark();
+    launch_finalize(s, act->d_U.p, dF, n, 0);
+    ctx->mark();
+    HIPX(hipGetLastError());
+    if (energy) {
+      HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPX(hipStreamSynchronize(s));
+      *energy = ctx->last_energy;
+    }
+  });
 }
 
 int xc_nadd_fock_build_device(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
@@ -923,7 +1079,7 @@ int xc_fock_build_device(xc_bfogc* bf, int functional, int precision, double thr
     require(bf && dP && dF, XC_ERR_ARG, "xc_fock_build_device: null argument");
     xc_ctx* ctx = bf->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    enqueue_build(bf, functional, precision, thr, dP, dF);
+    enqueue_build(bf, functional, precision, thr, dP, dF, ctx->stream, ctx->d_energy);
     if (energy) {
       HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
       HIPX(hipStreamSynchronize(ctx->stream));
@@ -950,7 +1106,7 @@ int xc_fock_build(xc_bfogc* bf, int functional, int precision, double thr, const
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     const int n = bf->basis->nao;
     HIPX(hipMemcpyAsync(bf->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
-    enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p);
+    enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p, ctx->stream, ctx->d_energy);
     HIPX(hipMemcpyAsync(F, bf->d_F.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPX(hipStreamSynchronize(ctx->stream));
@@ -1026,6 +1182,213 @@ int xc_device_copy(xc_ctx* ctx, void* dst, const void* src, size_t bytes) {
     launch_copy(ctx->stream, src, dst, (int64_t)(bytes / 32));
     HIPX(hipGetLastError());
     HIPX(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+// ---- serving ----------------------------------------------------------------
+struct xc_server {
+  struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    double* h_P = nullptr;  // pinned staging of P, F and E
+    double* h_F = nullptr;
+    double* h_E = nullptr;
+    size_t cap = 0;
+    bool busy = false;
+    int64_t ticket = -1;
+    xc_bfogc* bf = nullptr;
+    double* user_F = nullptr;
+    double* user_E = nullptr;
+  };
+  xc_ctx* ctx = nullptr;
+  std::vector<Slot> slots;
+  bool graphs = true;
+  int64_t next_ticket = 1;
+  int64_t submitted = 0, graph_launches = 0;
+};
+
+namespace {
+
+void server_slot_release(xc_server::Slot& sl) {
+  if (sl.h_P) (void)hipHostFree(sl.h_P);
+  if (sl.h_F) (void)hipHostFree(sl.h_F);
+  sl.h_P = sl.h_F = nullptr;
+  sl.cap = 0;
+}
+
+void server_finish(xc_server::Slot& sl, int n) {
+  std::memcpy(sl.user_F, sl.h_F, sizeof(double) * (size_t)n * n);
+  if (sl.user_E) *sl.user_E = *sl.h_E;
+  sl.bf->in_flight = false;
+  sl.busy = false;
+  sl.bf = nullptr;
+  sl.ticket = -1;
+}
+
+xc_server::Slot* server_slot_of(xc_server* srv, int64_t ticket) {
+  for (auto& sl : srv->slots)
+    if (sl.busy && sl.ticket == ticket) return &sl;
+  return nullptr;
+}
+
+}  // namespace
+
+int xc_server_create(xc_ctx* ctx, int nslots, int use_graphs, xc_server** out) {
+  return guarded([&] {
+    require(ctx && out, XC_ERR_ARG, "xc_server_create: null argument");
+    require(nslots >= 1 && nslots <= 16, XC_ERR_ARG, "xc_server_create: slots must be in [1, 16]");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipSetDevice(ctx->device));
+    auto* srv = new xc_server();
+    srv->ctx = ctx;
+    srv->graphs = use_graphs != 0;
+    srv->slots.resize(nslots);
+    try {
+      for (auto& sl : srv->slots) {
+        HIPX(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        HIPX(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        HIPX(hipHostMalloc((void**)&sl.h_E, sizeof(double), hipHostMallocDefault));
+      }
+    } catch (...) {
+      for (auto& sl : srv->slots) {
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.h_E) (void)hipHostFree(sl.h_E);
+      }
+      delete srv;
+      throw;
+    }
+    ++ctx->live_handles;
+    *out = srv;
+  });
+}
+
+int xc_server_destroy(xc_server* srv) {
+  return guarded([&] {
+    if (!srv) return;
+    std::lock_guard<std::recursive_mutex> lk(srv->ctx->mu);
+    for (auto& sl : srv->slots) {
+      (void)hipStreamSynchronize(sl.stream);
+      if (sl.busy) {
+        sl.bf->in_flight = false;
+        sl.busy = false;
+      }
+      server_slot_release(sl);
+      (void)hipHostFree(sl.h_E);
+      (void)hipEventDestroy(sl.done);
+      (void)hipStreamDestroy(sl.stream);
+    }
+    --srv->ctx->live_handles;
+    delete srv;
+  });
+}
+
+int xc_server_submit(xc_server* srv, xc_bfogc* bf, int functional, int precision, double thr, const double* P,
+                     double* F, double* energy, int64_t* ticket) {
+  return guarded([&] {
+    require(srv && bf && P && F && ticket, XC_ERR_ARG, "xc_server_submit: null argument");
+    require(bf->ctx == srv->ctx, XC_ERR_ARG, "xc_server_submit: controller of another context");
+    xc_ctx* ctx = srv->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    *ticket = 0;
+    if (bf->in_flight) throw XcError(XC_BUSY, "xc_server_submit: a build of this controller is in flight");
+    xc_server::Slot* sl = nullptr;
+    for (auto& c : srv->slots)
+      if (!c.busy) {
+        sl = &c;
+        break;
+      }
+    if (!sl) throw XcError(XC_BUSY, "xc_server_submit: every slot is busy");
+    const int n = bf->basis->nao;
+    const size_t nn = (size_t)n * n;
+    if (sl->cap < nn) {
+      server_slot_release(*sl);
+      HIPX(hipHostMalloc((void**)&sl->h_P, sizeof(double) * nn, hipHostMallocDefault));
+      HIPX(hipHostMalloc((void**)&sl->h_F, sizeof(double) * nn, hipHostMallocDefault));
+      sl->cap = nn;
+    }
+    std::memcpy(sl->h_P, P, sizeof(double) * nn);
+    hipStream_t s = sl->stream;
+    HIPX(hipMemcpyAsync(bf->d_P.p, sl->h_P, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+    if (srv->graphs) {
+      const auto key = std::make_pair(functional + 16 * precision, thr);
+      auto it = bf->graphs.find(key);
+      if (it == bf->graphs.end()) {
+        // capture one build on this controller's resident buffers (no memset nodes: U is
+        // cleared by a fill kernel)
+        hipGraph_t graph = nullptr;
+        HIPX(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p, s, bf->d_E.p);
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+          throw;
+        }
+        HIPX(hipStreamEndCapture(s, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPX(e);
+        it = bf->graphs.emplace(key, exec).first;
+      }
+      HIPX(hipGraphLaunch(it->second, s));
+      ++srv->graph_launches;
+    } else {
+      enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p, s, bf->d_E.p);
+    }
+    HIPX(hipMemcpyAsync(sl->h_F, bf->d_F.p, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(sl->h_E, bf->d_E.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPX(hipEventRecord(sl->done, s));
+    sl->busy = true;
+    sl->bf = bf;
+    sl->user_F = F;
+    sl->user_E = energy;
+    sl->ticket = srv->next_ticket++;
+    bf->in_flight = true;
+    ++srv->submitted;
+    *ticket = sl->ticket;
+  });
+}
+
+int xc_server_poll(xc_server* srv, int64_t ticket, int* done) {
+  return guarded([&] {
+    require(srv && done, XC_ERR_ARG, "xc_server_poll: null argument");
+    std::lock_guard<std::recursive_mutex> lk(srv->ctx->mu);
+    xc_server::Slot* sl = server_slot_of(srv, ticket);
+    require(sl != nullptr, XC_ERR_ARG, "xc_server_poll: unknown or finished ticket");
+    const hipError_t q = hipEventQuery(sl->done);
+    if (q == hipErrorNotReady) {
+      *done = 0;
+      return;
+    }
+    HIPX(q);
+    server_finish(*sl, sl->bf->basis->nao);
+    *done = 1;
+  });
+}
+
+int xc_server_wait(xc_server* srv, int64_t ticket) {
+  return guarded([&] {
+    require(srv, XC_ERR_ARG, "xc_server_wait: null argument");
+    std::lock_guard<std::recursive_mutex> lk(srv->ctx->mu);
+    xc_server::Slot* sl = server_slot_of(srv, ticket);
+    require(sl != nullptr, XC_ERR_ARG, "xc_server_wait: unknown or finished ticket");
+    HIPX(hipEventSynchronize(sl->done));
+    server_finish(*sl, sl->bf->basis->nao);
+  });
+}
+
+int xc_server_stats(const xc_server* srv, int* busy_slots, int64_t* submitted, int64_t* graph_launches) {
+  return guarded([&] {
+    require(srv, XC_ERR_ARG, "xc_server_stats: null argument");
+    if (busy_slots) {
+      int b = 0;
+      for (const auto& sl : srv->slots) b += sl.busy ? 1 : 0;
+      *busy_slots = b;
+    }
+    if (submitted) *submitted = srv->submitted;
+    if (graph_launches) *graph_launches = srv->graph_launches;
   });
 }
 

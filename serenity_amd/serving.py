@@ -1,0 +1,213 @@
+"""Serving stream of XC Fock builds on one MI355X.
+
+Request mix (seeded): half water clusters (H2O)_n with n uniform in 1..48, half
+linear alkanes CkH2k+2 with k uniform in 1..30; def2-SVP or def2-TZVP (50/50);
+LDA or PBE (50/50); Poisson arrivals at a fixed rate.  A system is a (molecule,
+basis) pair: its grid, basis, density matrix (P = 2 C C^T, C = S^-1/2 Q, seeded
+per system) and FP32 AO tiles are built once and stay resident in HBM, admitted
+against a device-memory budget by their measured footprint (least recently used
+systems without a build in flight are evicted first).  A request is one XC Fock
+build of its system with FP32-MFMA contractions.
+
+Continuous batching: arrived requests are dispatched in arrival order whenever a
+server slot is free and their system has no build in flight; builds of
+different systems overlap on the device (one stream per slot, HIP-graph replay
+of each system's build).  Latency = completion - arrival.
+
+    python -m serenity_amd.serving [--requests 2000] [--qps 400] [--slots 4]
+"""
+import argparse
+import collections
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import workloads as W
+from . import xc
+
+BASES = ("DEF2-SVP", "DEF2-TZVP")
+FUNCTIONALS = ("LDA", "PBE")
+
+
+def make_stream(n_requests, qps, seed=W.SEED):
+    """[(arrival_s, (kind, size, basis), functional)] with exponential inter-arrival times."""
+    rng = np.random.default_rng(seed + 5)
+    t, out = 0.0, []
+    for _ in range(n_requests):
+        t += rng.exponential(1.0 / qps)
+        if rng.random() < 0.5:
+            key = ("water", int(rng.integers(1, 49)))
+        else:
+            key = ("alkane", int(rng.integers(1, 31)))
+        basis = BASES[int(rng.integers(0, 2))]
+        func = FUNCTIONALS[int(rng.integers(0, 2))]
+        out.append((t, key + (basis,), func))
+    return out
+
+
+class HostSystems:
+    """Host-side data of every system: geometry, grid, basis and density matrix (built once)."""
+
+    def __init__(self, seed=W.SEED):
+        self.seed = seed
+        self.grids = {}
+        self.data = {}
+
+    def get(self, key):
+        if key not in self.data:
+            kind, size, basis = key
+            Z, X = W.water_chain(size, seed=self.seed) if kind == "water" else W.alkane(size, seed=self.seed)
+            if (kind, size) not in self.grids:
+                self.grids[(kind, size)] = xc.molecular_grid(Z, X)
+            b = xc.BasisController.from_file(basis, Z, X)
+            sys_seed = self.seed + 100000 * (kind == "alkane") + 100 * size + BASES.index(basis)
+            P = W.synthetic_density(b.overlap(), W.n_electrons(Z) // 2, seed=sys_seed)
+            self.data[key] = dict(basis=b, grid=self.grids[(kind, size)], P=P)
+        return self.data[key]
+
+
+class ResidentSystems:
+    """Device-resident controllers (FP32 tiles) admitted against a byte budget, LRU eviction."""
+
+    def __init__(self, ctx, host, budget_bytes, tiles="only"):
+        self.ctx, self.host, self.budget, self.tiles = ctx, host, budget_bytes, tiles
+        self.lru = collections.OrderedDict()  # key -> (bfogc, footprint bytes)
+        self.bytes = 0
+        self.created = self.evicted = 0
+
+    def get(self, key, busy):
+        if key in self.lru:
+            self.lru.move_to_end(key)
+            return self.lru[key][0]
+        d = self.host.get(key)
+        before = self.ctx.memory_in_use()
+        bf = xc.BasisFunctionOnGridController(self.ctx, d["basis"], d["grid"], 128, 1e-9, 1, keep_fp32=self.tiles)
+        foot = self.ctx.memory_in_use() - before
+        self.lru[key] = (bf, foot)
+        self.bytes += foot
+        self.created += 1
+        while self.bytes > self.budget:
+            victim = next((k for k in self.lru if k != key and k not in busy), None)
+            if victim is None:
+                break
+            vbf, vfoot = self.lru.pop(victim)
+            vbf._release()
+            self.bytes -= vfoot
+            self.evicted += 1
+        return bf
+
+
+def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="FP32", log=None):
+    """Serve the stream in real time; returns latency percentiles, throughput and counts."""
+    say = log or (lambda *a: None)
+    host = HostSystems()
+    res = ResidentSystems(ctx, host, budget_gb * 1e9, tiles="only" if precision == "FP32" else False)
+    keys = sorted({k for _, k, _ in stream})
+    t0 = time.perf_counter()
+    for k in keys:
+        host.get(k)
+    t_host = time.perf_counter() - t0
+    say("host setup of %d systems: %.1f s" % (len(keys), t_host))
+    t0 = time.perf_counter()
+    if prewarm:
+        for k in keys:
+            res.get(k, ())
+        ctx.synchronize()
+    t_dev = time.perf_counter() - t0
+    say("resident: %d systems, %.1f GB of device memory, setup %.1f s" % (len(res.lru), res.bytes / 1e9, t_dev))
+    srv = xc.Server(ctx, nslots=nslots, use_graphs=True)
+    # capture every resident system's build once (first-call graph capture is setup, not serving)
+    if prewarm:
+        for k in keys:
+            d = host.get(k)
+            for f in {f for _, kk, f in stream if kk == k}:
+                t = srv.submit(res.get(k, ()), f, precision, d["P"])
+                srv.wait(t)
+    n = len(stream)
+    arrive = np.array([a for a, _, _ in stream])
+    done_at = np.zeros(n)
+    pending = collections.deque()
+    inflight = {}  # ticket -> request index
+    busy = set()
+    energies = {}
+    i = nxt = 0
+    t0 = time.perf_counter()
+    while nxt < n:
+        now = time.perf_counter() - t0
+        while i < n and arrive[i] <= now:
+            pending.append(i)
+            i += 1
+        # dispatch in arrival order; a request whose system is busy waits, later ones may pass it
+        for j in list(pending):
+            _, key, func = stream[j]
+            if key in busy:
+                continue
+            bf = res.get(key, busy)
+            t = srv.submit(bf, func, precision, host.get(key)["P"])
+            if t is None:
+                break
+            inflight[t] = j
+            busy.add(key)
+            pending.remove(j)
+        for t in list(inflight):
+            out = srv.poll(t)
+            if out is not None:
+                j = inflight.pop(t)
+                done_at[j] = time.perf_counter() - t0
+                _, key, func = stream[j]
+                busy.discard(key)
+                energies[(key, func)] = out[1]
+                nxt += 1
+    wall = time.perf_counter() - t0
+    lat = (done_at - arrive) * 1e3
+    st = srv.stats()
+    srv._release()
+    for bf, _ in res.lru.values():
+        bf._release()
+    return dict(requests=n, qps_offered=round(n / arrive[-1], 2), slots=nslots, precision=precision,
+                p50_ms=round(float(np.percentile(lat, 50)), 3), p99_ms=round(float(np.percentile(lat, 99)), 3),
+                mean_ms=round(float(lat.mean()), 3), max_ms=round(float(lat.max()), 3),
+                builds_per_s=round(n / wall, 2), wall_s=round(wall, 3), distinct_systems=len(keys),
+                resident_gb=round(res.bytes / 1e9, 2), created=res.created, evicted=res.evicted,
+                graph_launches=st["graph_launches"], host_setup_s=round(t_host, 1), device_setup_s=round(t_dev, 1),
+                _energies=energies, _host=host)
+
+
+def fp64_check(ctx, host, energies, sample, seed=W.SEED):
+    """Max |dE_xc| of the served FP32 builds against FP64 builds on the GPU, over a seeded sample."""
+    keys = sorted(energies)
+    rng = np.random.default_rng(seed + 11)
+    pick = [keys[j] for j in rng.choice(len(keys), size=min(sample, len(keys)), replace=False)]
+    worst = 0.0
+    for key, func in pick:
+        d = host.get(key)
+        bf = xc.BasisFunctionOnGridController(ctx, d["basis"], d["grid"], 128, 1e-9, 1)
+        pot = xc.FuncPotential(bf, func, "FP64")
+        pot.getMatrix(d["P"])
+        worst = max(worst, abs(pot.getEnergy() - energies[(key, func)]))
+        bf._release()
+    return worst, len(pick)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--qps", type=float, default=400.0)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--fp64-sample", type=int, default=16)
+    args = ap.parse_args()
+    log = lambda *a: print("[serving]", *a, file=sys.stderr, flush=True)
+    ctx = xc.Context(0)
+    stream = make_stream(args.requests, args.qps)
+    out = run_stream(ctx, stream, nslots=args.slots, budget_gb=args.budget_gb, log=log)
+    energies, host = out.pop("_energies"), out.pop("_host")
+    out["max_dE_xc_fp32_vs_fp64"], out["fp64_sample"] = fp64_check(ctx, host, energies, args.fp64_sample)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -25,7 +25,7 @@ import weakref
 import numpy as np
 
 from . import _lib
-from ._lib import FUNCTIONALS, GGA, PRECISIONS, GRID_TYPES, RADIAL_TYPES, XCError, check
+from ._lib import FUNCTIONALS, GGA, PRECISIONS, GRID_TYPES, RADIAL_TYPES, XC_BUSY, XCError, check
 
 BASIS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "basis")
 ANGSTROM_TO_BOHR = 1.0 / 0.529177210544
@@ -107,11 +107,13 @@ class Context:
         return dict(zip(_lib.STAGES, ms.tolist())), nb.value
 
     def close(self):
-        """Release every handle created on this context (controllers first), then the context."""
+        """Release every handle created on this context (servers, then controllers, then the rest),
+        then the context."""
         if not self.handle:
             return
         kids = list(self._children)
-        for k in sorted(kids, key=lambda o: 0 if isinstance(o, BasisFunctionOnGridController) else 1):
+        order = lambda o: 0 if isinstance(o, Server) else 1 if isinstance(o, BasisFunctionOnGridController) else 2
+        for k in sorted(kids, key=order):
             k._release()
         check(_lib.lib().xc_ctx_destroy(self.handle))
         self.handle = None
@@ -274,7 +276,8 @@ class BasisFunctionOnGridController(_Handle):
     ``BasisFunctionOnGridController(basis, grid, maxBlockSize, radialThreshold,
     highestDerivative)`` as in the reference; ``highestDerivative`` must be 0 or
     1.  ``keep_fp32`` keeps an FP32 copy of the tiles for the mixed-precision
-    contractions."""
+    contractions; ``keep_fp32="only"`` keeps FP32 tiles alone (FP32 builds only,
+    a third of the footprint)."""
 
     def __init__(self, ctx, basis, grid, max_block_size=128, radial_threshold=1e-9, highest_derivative=1,
                  keep_fp32=False):
@@ -285,7 +288,8 @@ class BasisFunctionOnGridController(_Handle):
         self._dg = grid._device(ctx)
         h = C.c_void_p()
         check(_lib.lib().xc_bfogc_create(self._db.handle, self._dg.handle, int(max_block_size),
-                                         float(radial_threshold), int(highest_derivative), int(bool(keep_fp32)),
+                                         float(radial_threshold), int(highest_derivative),
+                                         2 if keep_fp32 == "only" else int(bool(keep_fp32)),
                                          C.byref(h)))
         self.handle = h
         ctx._register(self)
@@ -318,6 +322,11 @@ class BasisFunctionOnGridController(_Handle):
         a, b, c = C.c_int64(), C.c_double(), C.c_int64()
         check(_lib.lib().xc_bfogc_stats(self.handle, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def update_density(self, dP, precision="FP64", with_gradient=True):
+        """Compute and keep resident the grid density of the device density matrix ``dP``
+        (the cache read by NAddFuncPotential.build_cached)."""
+        check(_lib.lib().xc_bfogc_update_density(self.handle, PRECISIONS[precision], dP, int(bool(with_gradient))))
 
     def getBlockOnGridData(self, block):
         """dict(functionValues B x n, derivativeValues (x, y, z) or None, negligible n)."""
@@ -537,3 +546,68 @@ class NAddFuncPotential:
             self._energy = E.value
             return E.value
         return None
+
+    def build_cached(self, dF, sync=True):
+        """Build from the densities cached in every controller (update_density); dF: device pointer."""
+        E = C.c_double()
+        check(_lib.lib().xc_nadd_fock_build_cached(self.active.handle, len(self.environment), self._env_handles(),
+                                                   FUNCTIONALS[self.functional], PRECISIONS[self.precision],
+                                                   float(self.blockAveThreshold), dF,
+                                                   C.byref(E) if sync else None))
+        if sync:
+            self._energy = E.value
+            return E.value
+        return None
+
+
+class Server(_Handle):
+    """Asynchronous XC Fock builds on resident controllers: ``nslots`` builds overlap on the
+    device, each on its own stream; with ``use_graphs`` a controller's build is captured once
+    per (functional, precision, threshold) and replayed.  submit() returns a ticket, or None
+    when no slot is free or the controller already has a build in flight."""
+
+    _destroy = "xc_server_destroy"
+
+    def __init__(self, ctx, nslots=4, use_graphs=True):
+        h = C.c_void_p()
+        check(_lib.lib().xc_server_create(ctx.handle, int(nslots), int(bool(use_graphs)), C.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+        self._out = {}  # ticket -> (F, energy holder) kept alive until finished
+        ctx._register(self)
+
+    def submit(self, bfogc, functional, precision, P, block_ave_threshold=1e-11):
+        n = bfogc.getNBasisFunctions()
+        Pf = np.asfortranarray(P, dtype=np.float64)
+        if Pf.shape != (n, n):
+            raise XCError(1, "Server.submit: density matrix is %s, basis has %d functions" % (Pf.shape, n))
+        F = np.zeros((n, n), order="F")
+        E = (C.c_double * 1)()
+        t = C.c_int64()
+        rc = _lib.lib().xc_server_submit(self.handle, bfogc.handle, FUNCTIONALS[functional], PRECISIONS[precision],
+                                         float(block_ave_threshold), Pf.ctypes.data_as(_lib.dp),
+                                         F.ctypes.data_as(_lib.dp), E, C.byref(t))
+        if rc == XC_BUSY:
+            return None
+        check(rc)
+        self._out[t.value] = (F, E)
+        return t.value
+
+    def poll(self, ticket):
+        """(F, E) once the build of ``ticket`` has finished, else None."""
+        done = C.c_int()
+        check(_lib.lib().xc_server_poll(self.handle, C.c_int64(ticket), C.byref(done)))
+        if not done.value:
+            return None
+        F, E = self._out.pop(ticket)
+        return F, E[0]
+
+    def wait(self, ticket):
+        check(_lib.lib().xc_server_wait(self.handle, C.c_int64(ticket)))
+        F, E = self._out.pop(ticket)
+        return F, E[0]
+
+    def stats(self):
+        b, s, g = C.c_int(), C.c_int64(), C.c_int64()
+        check(_lib.lib().xc_server_stats(self.handle, C.byref(b), C.byref(s), C.byref(g)))
+        return dict(busy_slots=b.value, submitted=s.value, graph_launches=g.value)

@@ -126,3 +126,54 @@ def test_nadd_error_paths(ctx):
     # no environment: v[rho_act] - v[rho_act] = 0 (the two inlined evaluations may round differently)
     pot = xc.NAddFuncPotential(bfs[0], [], "PBE")
     assert np.abs(pot.getMatrix(P[0], [])).max() < 1e-15 and abs(pot.getEnergy()) < 1e-15
+
+
+def test_nadd_cached_densities_match_direct_build(ctx):
+    """Cached subsystem densities (update_density) give the same F and E as the direct build, through a
+    freeze-and-thaw style update of one subsystem's density, for PBE and TF; a missing cache is an error."""
+    emb = embedding(4, 3)
+    g, subs = emb["grid"], emb["subsystems"]
+    bfs = controllers(ctx, emb)
+    nsub = len(subs)
+    dummy = ctx.alloc(8)
+    with pytest.raises(xc.XCError, match="cached density"):
+        xc.NAddFuncPotential(bfs[0], bfs[1:], "PBE").build_cached(dummy)
+    ctx.free(dummy)
+    dP = []
+    for s_ in subs:
+        m = s_["basis"].nao
+        d = ctx.alloc(8 * m * m)
+        ctx.upload(d, np.asfortranarray(s_["P"]).ravel(order="F"))
+        dP.append(d)
+    for i in range(nsub):
+        bfs[i].update_density(dP[i])
+    n0 = subs[0]["basis"].nao
+    dF1, dF2 = ctx.alloc(8 * n0 * n0), ctx.alloc(8 * n0 * n0)
+
+    def both(func):
+        pot = xc.NAddFuncPotential(bfs[0], bfs[1:], func)
+        E1 = pot.build_device(dP[0], dP[1:], dF1)
+        E2 = pot.build_cached(dF2)
+        F1, F2 = np.zeros(n0 * n0), np.zeros(n0 * n0)
+        ctx.download(F1, dF1)
+        ctx.download(F2, dF2)
+        return E1, E2, F1, F2
+
+    for func in ("PBE", "TF"):
+        E1, E2, F1, F2 = both(func)
+        say("  cached", func, "E", E2, "|dE|", abs(E1 - E2), "max|dF|", np.abs(F1 - F2).max())
+        # same densities and energies; F differs only by the order of the FP64 atomic accumulation
+        assert E1 == E2 and np.abs(F1 - F2).max() < 1e-13
+    # a new density for environment subsystem 2: its cached density and energies are refreshed
+    m = subs[2]["basis"].nao
+    Pnew = W.synthetic_density(subs[2]["basis"].overlap(), 5, seed=7)
+    ctx.upload(dP[2], np.asfortranarray(Pnew).ravel(order="F"))
+    bfs[2].update_density(dP[2])
+    for func in ("PBE", "TF"):
+        E1, E2, F1, F2 = both(func)
+        assert E1 == E2 and np.abs(F1 - F2).max() < 1e-13
+        Fo, Eo = O.nadd_fock_build(subs[0]["basis"], [s_["basis"] for s_ in subs[1:]], g.points, g.weights,
+                                   subs[0]["P"], [s_["P"] if i != 1 else Pnew for i, s_ in enumerate(subs[1:])], func)
+        assert abs(E2 - Eo) <= TOL_E and np.abs(F2.reshape(n0, n0, order="F") - Fo).max() <= TOL_F
+    for d in dP + [dF1, dF2]:
+        ctx.free(d)

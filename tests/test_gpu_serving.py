@@ -1,0 +1,111 @@
+"""Serving: asynchronous builds on resident controllers overlap on the device
+(several slots, each its own stream, HIP-graph replay) and give the same F and
+E_xc as the same builds run alone, and as the oracle."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from serenity_amd import xc
+from serenity_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def _systems():
+    out = []
+    for name, (Z, X), basis, func in (("h2o2/svp/LDA", W.water_cluster(1, 1, 2), "DEF2-SVP", "LDA"),
+                                      ("butane/svp/PBE", W.alkane(4), "DEF2-SVP", "PBE"),
+                                      ("h2o4/tzvp/PBE", W.water_cluster(2, 2, 1), "DEF2-TZVP", "PBE"),
+                                      ("hexane/svp/PBE", W.alkane(6), "DEF2-SVP", "PBE")):
+        b = xc.BasisController.from_file(basis, Z, X)
+        g = xc.molecular_grid(Z, X, accuracy=3)
+        P = W.synthetic_density(b.overlap(), W.n_electrons(Z) // 2)
+        out.append(dict(name=name, basis=b, grid=g, P=P, func=func))
+    return out
+
+
+def test_server_same_alone_and_in_mix(ctx):
+    systems = _systems()
+    for s in systems:
+        s["bf"] = xc.BasisFunctionOnGridController(ctx, s["basis"], s["grid"], 128, 1e-9, 1, keep_fp32=True)
+    alone = {}
+    for i, s in enumerate(systems):
+        for prec in ("FP64", "FP32"):
+            pot = xc.FuncPotential(s["bf"], s["func"], prec)
+            alone[(i, prec)] = (pot.getMatrix(s["P"]), pot.getEnergy())
+        Fo, Eo, _, _ = O.fock_build(s["basis"], s["grid"].points, s["grid"].weights, s["P"], s["func"])
+        s["oracle"] = (Fo, Eo)
+    srv = xc.Server(ctx, nslots=3, use_graphs=True)
+    # three rounds over every (system, precision), interleaved so that builds of different
+    # systems overlap; a controller with a build in flight waits for its turn
+    queue = [(r, i, prec) for r in range(3) for prec in ("FP64", "FP32") for i in range(len(systems))]
+    tickets, results = {}, {}
+    t0 = time.time()
+    while queue or tickets:
+        for item in list(queue):
+            r, i, prec = item
+            t = srv.submit(systems[i]["bf"], systems[i]["func"], prec, systems[i]["P"])
+            if t is not None:
+                tickets[item] = t
+                queue.remove(item)
+        for key, t in list(tickets.items()):
+            out = srv.poll(t)
+            if out is not None:
+                results[key] = out
+                del tickets[key]
+        assert time.time() - t0 < 120, "server builds did not finish"
+    st = srv.stats()
+    say("server stats", st)
+    assert st["submitted"] == len(results) == 3 * 2 * len(systems)
+    assert st["graph_launches"] == st["submitted"]
+    for (r, i, prec), (F, E) in results.items():
+        Fa, Ea = alone[(i, prec)]
+        tol = 1e-12 if prec == "FP64" else 1e-10
+        dF, dE = np.abs(F - Fa).max(), abs(E - Ea)
+        assert dF <= tol and dE <= tol, (systems[i]["name"], prec, r, dF, dE)
+        if prec == "FP64":
+            Fo, Eo = systems[i]["oracle"]
+            assert np.abs(F - Fo).max() <= 1e-9 and abs(E - Eo) <= 1e-9, systems[i]["name"]
+    srv._release()
+
+
+def test_server_busy_and_errors(ctx):
+    s = _systems()[1]
+    bf = xc.BasisFunctionOnGridController(ctx, s["basis"], s["grid"], 128, 1e-9, 1)
+    bf2 = xc.BasisFunctionOnGridController(ctx, s["basis"], s["grid"], 128, 1e-9, 1)
+    srv = xc.Server(ctx, nslots=1, use_graphs=False)
+    t = srv.submit(bf, "PBE", "FP64", s["P"])
+    assert t is not None
+    assert srv.submit(bf, "PBE", "FP64", s["P"]) is None  # controller in flight
+    assert srv.submit(bf2, "PBE", "FP64", s["P"]) is None  # no free slot
+    with pytest.raises(xc.XCError, match="in flight"):
+        xc.FuncPotential(bf, "PBE").getMatrix(s["P"])
+    F, E = srv.wait(t)
+    Fs = xc.FuncPotential(bf2, "PBE").getMatrix(s["P"])
+    assert np.abs(F - Fs).max() < 1e-12
+    with pytest.raises(xc.XCError, match="ticket"):
+        srv.poll(t)
+    with pytest.raises(xc.XCError, match="FP32"):
+        srv.submit(bf, "PBE", "FP32", s["P"])
+    srv._release()
+
+
+def test_fp32_only_controller(ctx):
+    """keep_fp32="only": FP32 builds equal those of a controller holding both tile sets; FP64 paths refuse."""
+    s = _systems()[1]
+    both = xc.BasisFunctionOnGridController(ctx, s["basis"], s["grid"], 128, 1e-9, 1, keep_fp32=True)
+    only = xc.BasisFunctionOnGridController(ctx, s["basis"], s["grid"], 128, 1e-9, 1, keep_fp32="only")
+    assert only.stats()[2] * 3 == both.stats()[2]
+    Fb = xc.FuncPotential(both, "PBE", "FP32").getMatrix(s["P"])
+    Fo = xc.FuncPotential(only, "PBE", "FP32").getMatrix(s["P"])
+    assert np.abs(Fb - Fo).max() < 1e-13
+    with pytest.raises(xc.XCError, match="FP64"):
+        xc.FuncPotential(only, "PBE", "FP64").getMatrix(s["P"])
+    with pytest.raises(xc.XCError, match="FP32 tiles only"):
+        only.getBlockOnGridData(0)
