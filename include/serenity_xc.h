@@ -95,6 +95,13 @@ int xc_molecular_grid(int natoms, const int* Z, const double* coords_bohr, int g
  * weights, in place (HilbertRTreeSorting::sort, src/grid/HilbertRTreeSorting.cpp
  * :27-216; equal keys keep their input order). */
 int xc_hilbert_sort(int64_t npoints, double* points, double* weights);
+/* xc_molecular_grid with the fuzzy-cell partition (the O(N natoms^2) part) on
+ * the context's device; the candidates, the weight cut and the Hilbert order are
+ * those of the host build, so the result is the same grid.  No counterpart in the
+ * reference, whose grid is built on the CPU (GridFactory.cpp:51-300). */
+int xc_molecular_grid_device(xc_ctx* ctx, int natoms, const int* Z, const double* coords, int grid_type,
+                             int radial_type, int accuracy, int smoothing, double weight_threshold, int sort,
+                             int64_t* npoints, double** points, double** weights, int64_t* npoints_before_cut);
 
 /* Lebedev-Laikov rule with `npoints` points: x,y,z,w arrays of length npoints
  * (weights sum to 1). */

@@ -62,6 +62,8 @@ def lib():
     _sig("xc_free_host", None, vp)
     _sig("xc_molecular_grid", C.c_int, C.c_int, ip, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
          C.c_int, C.POINTER(i64), C.POINTER(dp), C.POINTER(dp), C.POINTER(i64))
+    _sig("xc_molecular_grid_device", C.c_int, vp, C.c_int, ip, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+         C.c_int, C.POINTER(i64), C.POINTER(dp), C.POINTER(dp), C.POINTER(i64))
     _sig("xc_lebedev", C.c_int, C.c_int, dp, dp, dp, dp)
     _sig("xc_overlap", C.c_int, C.c_int, ip, ip, ip, dp, dp, dp, dp)
     _sig("xc_basis_create", C.c_int, vp, C.c_int, ip, ip, ip, dp, dp, dp, C.POINTER(vp))

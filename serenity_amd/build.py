@@ -15,6 +15,7 @@ LIB = os.path.join(HERE, "libserenity_xc.so")
 SOURCES = [
     "xc_api.hip",
     "kernels/xc_kernels.hip",
+    "kernels/grid_kernels.hip",
     "host/basis.cpp",
     "host/grid.cpp",
     "host/overlap.cpp",

@@ -10,6 +10,7 @@
 #include <stdexcept>
 
 #include "host.h"
+#include "partition.h"
 
 namespace sxc {
 
@@ -321,7 +322,7 @@ void hilbert_sort(std::vector<double>& points, std::vector<double>& weights) {
   weights.swap(w2);
 }
 
-MolecularGrid build_molecular_grid(const std::vector<int>& Z, const std::vector<double>& xyz, const GridSettings& s) {
+GridProblem grid_problem(const std::vector<int>& Z, const std::vector<double>& xyz, const GridSettings& s) {
   const int nAtoms = (int)Z.size();
   if (nAtoms == 0) throw std::runtime_error("grid: empty geometry");
   if ((int)xyz.size() != 3 * nAtoms) throw std::runtime_error("grid: coordinate array size mismatch");
@@ -353,102 +354,60 @@ MolecularGrid build_molecular_grid(const std::vector<int>& Z, const std::vector<
   for (int k = 0; k < nAtoms; ++k)
     if (refW[Z[k]].empty()) atom_grid(Z[k], s, refPts[Z[k]], refW[Z[k]]);
 
-  std::vector<std::vector<double>> atomPts(nAtoms), atomW(nAtoms);
-  int64_t nBefore = 0;
-  for (int k = 0; k < nAtoms; ++k) nBefore += (int64_t)refW[Z[k]].size();
-
-#pragma omp parallel for schedule(dynamic)
+  // candidates: every atom's reference grid translated to the atom, atom order
+  GridProblem P;
+  P.natoms = nAtoms;
+  P.type = (int)s.type;
+  P.smoothing = s.smoothing;
+  P.xyz = xyz;
+  P.dist = dist;
+  P.apair = aPair;
+  P.min_dist.assign(nAtoms, 999999999.9);
+  for (int k = 0; k < nAtoms; ++k)
+    for (int l = 0; l < nAtoms; ++l)
+      if (l != k) P.min_dist[k] = std::min(P.min_dist[k], dist[l + (size_t)nAtoms * k]);
   for (int k = 0; k < nAtoms; ++k) {
-    std::vector<int> significant;
-    double minAtomDist = 999999999.9;
-    for (int l = 0; l < nAtoms; ++l) {
-      if (dist[l + (size_t)nAtoms * k] < 40.0) significant.push_back(l);
-      if (l != k) minAtomDist = std::min(minAtomDist, dist[l + (size_t)nAtoms * k]);
-    }
     const auto& rp = refPts[Z[k]];
     const auto& rw = refW[Z[k]];
-    auto& outP = atomPts[k];
-    auto& outW = atomW[k];
-    outP.reserve(rp.size());
-    outW.reserve(rw.size());
-    std::vector<double> rdist(nAtoms);
     for (size_t i = 0; i < rw.size(); ++i) {
-      const double gp[3] = {rp[3 * i] + xyz[3 * k], rp[3 * i + 1] + xyz[3 * k + 1], rp[3 * i + 2] + xyz[3 * k + 2]};
-      double weight = rw[i];
-      double cellSum = 0.0;
-      for (int a = 0; a < nAtoms; ++a) rdist[a] = dist3(&xyz[3 * a], gp);
-      if (s.type == GridType::BECKE) {
-        for (int l : significant) {
-          double cell = 1.0;
-          for (int j : significant) {
-            if (l == j) continue;
-            const double mu = (rdist[l] - rdist[j]) / dist[l + (size_t)nAtoms * j];
-            const double nu = mu + aPair[j + (size_t)nAtoms * l] * (1.0 - mu * mu);
-            cell *= becke_s(nu, s.smoothing);
-          }
-          if (l == k) weight *= cell;
-          cellSum += cell;
-        }
-        weight /= cellSum;
-      } else if (s.type == GridType::VORONOI) {
-        for (int l : significant) {
-          const double mu = (rdist[l] - rdist[k]) / dist[l + (size_t)nAtoms * k];
-          const double nu = mu + aPair[k + (size_t)nAtoms * l] * (1.0 - mu * mu);
-          if (nu < 0.0) {
-            weight = 0.0;
-            break;
-          }
-        }
-      } else {
-        // Stratmann-Scuseria-Frisch partition (CPL 257 (1996) 213), a = 0.64.
-        if (rdist[k] >= 0.5 * (1.0 - 0.64) * minAtomDist) {
-          bool zero = false;
-          for (int j = 0; j < nAtoms; ++j) {
-            const double nu = (rdist[k] - rdist[j]) / dist[k + (size_t)nAtoms * j];
-            if (nu >= 0.64) {
-              zero = true;
-              break;
-            }
-          }
-          if (zero) continue;
-          for (int a = 0; a < nAtoms; ++a) {
-            double cell = 1.0;
-            for (int j = 0; j < nAtoms; ++j) {
-              if (a == j) continue;
-              double nu = (rdist[a] - rdist[j]) / dist[a + (size_t)nAtoms * j];
-              if (nu <= -0.64) continue;
-              if (nu >= 0.64) {
-                cell = 0.0;
-                break;
-              }
-              nu /= 0.64;
-              const double n3 = nu * nu * nu;
-              const double poly = (-5.0 * n3 * n3 * nu + 21.0 * n3 * nu * nu - 35.0 * n3 + 35.0 * nu) / 16.0;
-              cell *= 0.5 * (1 - poly);
-            }
-            if (a == k) weight *= cell;
-            cellSum += cell;
-          }
-          weight /= cellSum;
-        }
-      }
-      if (weight > s.weight_threshold) {
-        outW.push_back(weight);
-        outP.push_back(gp[0]);
-        outP.push_back(gp[1]);
-        outP.push_back(gp[2]);
-      }
+      P.cand_xyz.push_back(rp[3 * i] + xyz[3 * k]);
+      P.cand_xyz.push_back(rp[3 * i + 1] + xyz[3 * k + 1]);
+      P.cand_xyz.push_back(rp[3 * i + 2] + xyz[3 * k + 2]);
+      P.cand_w.push_back(rw[i]);
+      P.cand_atom.push_back(k);
     }
   }
+  return P;
+}
 
-  MolecularGrid g;
-  g.n_before_cut = nBefore;
-  for (int k = 0; k < nAtoms; ++k) {
-    g.points.insert(g.points.end(), atomPts[k].begin(), atomPts[k].end());
-    g.weights.insert(g.weights.end(), atomW[k].begin(), atomW[k].end());
+void partition_weights_host(const GridProblem& P, std::vector<double>& w) {
+  const int64_t nc = (int64_t)P.cand_w.size();
+  w.resize(nc);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t c = 0; c < nc; ++c) {
+    const int k = P.cand_atom[c];
+    w[c] = partition_weight(P.type, P.smoothing, P.natoms, P.xyz.data(), P.dist.data(), P.apair.data(),
+                            P.min_dist[k], k, &P.cand_xyz[3 * c], P.cand_w[c]);
   }
+}
+
+MolecularGrid finish_grid(const GridProblem& P, const std::vector<double>& w, const GridSettings& s) {
+  MolecularGrid g;
+  g.n_before_cut = (int64_t)P.cand_w.size();
+  for (size_t c = 0; c < w.size(); ++c)
+    if (w[c] > s.weight_threshold) {
+      g.weights.push_back(w[c]);
+      g.points.insert(g.points.end(), &P.cand_xyz[3 * c], &P.cand_xyz[3 * c] + 3);
+    }
   if (s.sort) hilbert_sort(g.points, g.weights);
   return g;
+}
+
+MolecularGrid build_molecular_grid(const std::vector<int>& Z, const std::vector<double>& xyz, const GridSettings& s) {
+  const GridProblem P = grid_problem(Z, xyz, s);
+  std::vector<double> w;
+  partition_weights_host(P, w);
+  return finish_grid(P, w, s);
 }
 
 }  // namespace sxc

@@ -72,6 +72,23 @@ struct MolecularGrid {
 MolecularGrid build_molecular_grid(const std::vector<int>& Z, const std::vector<double>& xyz_bohr,
                                    const GridSettings& settings);
 
+// The partition problem of a molecular grid: atoms, interatomic distances,
+// Becke size adjustments, and every candidate point (each atom's reference grid
+// translated to the atom, in atom order) with its raw weight and atom.
+struct GridProblem {
+  int natoms = 0;
+  int type = 2;
+  unsigned smoothing = 3;
+  std::vector<double> xyz, dist, apair, min_dist;
+  std::vector<double> cand_xyz, cand_w;
+  std::vector<int> cand_atom;
+};
+GridProblem grid_problem(const std::vector<int>& Z, const std::vector<double>& xyz_bohr, const GridSettings& s);
+// Partitioned weight of every candidate (-1: dropped by the SSF screening), on the host.
+void partition_weights_host(const GridProblem& P, std::vector<double>& w);
+// Weight cut (w > threshold, candidates in order) and optional Hilbert ordering.
+MolecularGrid finish_grid(const GridProblem& P, const std::vector<double>& w, const GridSettings& s);
+
 // Lebedev-Laikov rule with `npoints` points (6..5810); unit-sphere points and
 // weights summing to 1.  Throws for an unknown order.
 void lebedev_rule(int npoints, std::vector<double>& x, std::vector<double>& y, std::vector<double>& z,

@@ -57,16 +57,8 @@ constexpr int kFockNumS = 12;         // slot-count classes of the sub-tile kern
 constexpr int kFockS[kFockNumS] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 17};
 inline int fock_smax(int fp64) { return fp64 ? 12 : 17; }  // register budget of the accumulators
 
-// Size classes of the macro-tile Fock kernel (blocks too wide for the pair
-// kernel), chosen from npad: class 0: npad <= 32, class 1: <= 64, class 2: 128-wide macro-tiles.
-constexpr int kNumClasses = 3;
-inline int size_class(int npad) { return npad <= 32 ? 0 : (npad <= 64 ? 1 : 2); }
-inline int class_width(int cls) { return cls == 0 ? 32 : (cls == 1 ? 64 : 128); }
-// Density passes hold NCT = 1..8 column tiles of 16 (pass width <= 128).
-constexpr int kMaxNct = 8;
-
-// Work item of the contraction kernels: block and two column offsets (density:
-// a = first column of the pass; fock: a, b = first rows of the macro-tile pair).
+// Work item of the density kernel: block, a = first column of the pass,
+// b = columns of the pass (multiple of 16), pass = partial-density slot.
 struct WorkItem {
   int block, a, b, pass;
 };
@@ -79,14 +71,11 @@ void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const in
                     const DevShell* shells, const double* alpha, const double* coef, const double* ang,
                     const double* pts, double radial_thr, double exp_thr, int ncomp, double* tiles64, float* tiles32);
 
+// Sub-tile density kernel: 8 wavefronts, S = pass columns / 16 (1..kDensMaxS).
 // Partial densities: part[(pass * 4 + c) * npts + q], c = rho, d/dx, d/dy, d/dz
 // (gradient slots hold sum T grad phi without the factor 2).
-template <typename T>
-void launch_density(hipStream_t s, int nct, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
-                    const T* tiles, const T* P, int n, int want_grad, double* part, int64_t npts);
-
-// Sub-tile density kernel: 8 wavefronts, S = pass columns / 16 (1..kDensMaxS).
 constexpr int kDensMaxS = 12;
+constexpr int kDensIdxLds = 2048;  // AO index lists up to this length are staged in LDS
 template <typename T>
 void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
                         const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
@@ -102,23 +91,16 @@ void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npt
                        double* v, double* g, double* vs, double* gw, double* e_blk,
                        const double* rho_sub = nullptr, const double* grad_sub = nullptr);
 
-template <typename T>
-void launch_fock(hipStream_t s, int cls, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
-                 const T* tiles, const double* vs, const double* gw, int gga, double block_ave_thr, double* U, int n,
-                 int64_t npts);
-
-// Pair-tile Fock kernel; work item: block, a = first pair, b = pair count.
-template <typename T>
-void launch_fock_pairs(hipStream_t s, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
-                       const T* tiles, const double* vs, const double* gw, int gga, double block_ave_thr, double* U,
-                       int n, int64_t npts);
 // Sub-tile Fock kernel: 8 wavefronts, slot class sc (kFockS[sc] sub-tiles per wavefront).
 template <typename T>
 void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,
                      const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
                      double block_ave_thr, double* U, int n, int64_t npts);
-int fock_pairs_rmax(int fp64);  // rows a pair-kernel work item may stage
-int fock_pairs_maxp(int fp64);  // pairs per wavefront
+
+// Molecular-grid partition weights of ncand candidate points (kernels/grid_kernels.hip).
+void launch_partition(hipStream_t s, int type, unsigned smoothing, int natoms, const double* xyz, const double* dist,
+                      const double* apair, const double* min_dist, int64_t ncand, const double* cand_xyz,
+                      const double* cand_w, const int* cand_atom, double* w);
 
 void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
                              double* vs, double* gw);

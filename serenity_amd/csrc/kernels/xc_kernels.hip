@@ -7,8 +7,8 @@
 //               (MatrixOperatorToGridTransformer.cpp:103-165)
 //   functional  per-point XC energy density and potential, block skip and E_xc partials
 //               (XCFun.cpp:129-158, 751-765)
-//   fock        F += phi_b^T G_b + G_b^T phi_b on matrix cores, scattered into the upper
-//               triangle with FP64 atomics (ScalarOperatorToMatrixAdder.cpp:179-303)
+//   fock        F += phi_b^T G_b + G_b^T phi_b on matrix cores, scattered into the lower
+//               triangle of U with FP64 atomics (ScalarOperatorToMatrixAdder.cpp:179-303)
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
@@ -150,145 +150,6 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
   }
 }
 
-// Density contraction.  One workgroup (4 wavefronts) per (block, column pass):
-// T[p][nu] = sum_mu phi[mu][p] P[mu][nu] for the pass's NC = 16 NCT columns,
-// wave w owning points 32w..32w+31 (2 x NCT MFMA tiles in registers).  The AO
-// (K) dimension streams through two LDS buffers 16 rows at a time: the global
-// loads of chunk k+1 are issued before the MFMAs of chunk k and written to the
-// other buffer after them.  The pass's partial rho and sum T grad phi are then
-// formed by dotting T (transposed through LDS) with phi / grad phi read
-// coalesced from the tiles, and written to the pass's partial slot.
-template <typename T, int NCT>
-__global__ __launch_bounds__(256) void density_kernel(const WorkItem* __restrict__ work,
-                                                      const DevBlock* __restrict__ blocks,
-                                                      const int* __restrict__ ao_idx, const T* __restrict__ tiles,
-                                                      const T* __restrict__ P, int n, int want_grad,
-                                                      double* __restrict__ part, int64_t npts) {
-  using M = Mfma<T>;
-  constexpr int NC = 16 * NCT;
-  constexpr int KC = 16;
-  constexpr int PHI_PER_T = KC * kTile / 256;
-  constexpr int P_PER_T = KC * NC / 256;
-  // rows padded by 16 elements (bank spread of the MFMA operand reads)
-  constexpr int PS = kTile + 16, QS = NC + 16;
-  __shared__ T sPhi[2][KC][PS];
-  __shared__ T sP[2][KC][QS];
-  __shared__ double red[4][kTile];
-  const WorkItem wk = work[blockIdx.x];
-  const DevBlock bk = blocks[wk.block];
-  const int col0 = wk.a;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int p = tid & (kTile - 1), h = tid >> 7;
-  const int npad = bk.npad;
-  const T* tb = tiles + bk.tile_off;
-  const int* idx = ao_idx + bk.idx_off;
-  const int64_t cstride = (int64_t)npad * kTile;
-
-  typename M::acc_t acc[2][NCT];
-#pragma unroll
-  for (int c = 0; c < NCT; ++c) acc[0][c] = acc[1][c] = M::zero();
-
-  // this thread's P-chunk elements: row i_loc of the chunk, columns tid/16 + 16 j
-  const int i_loc = tid & 15;
-  int nu_g[P_PER_T];
-#pragma unroll
-  for (int j = 0; j < P_PER_T; ++j) {
-    const int col = col0 + (tid >> 4) + 16 * j;
-    nu_g[j] = col < npad ? idx[col] : -1;
-  }
-  T rPhi[PHI_PER_T], rP[P_PER_T];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int j = 0; j < PHI_PER_T; ++j) {
-      const int e = tid + 256 * j;
-      rPhi[j] = tb[(int64_t)(k0 + e / kTile) * kTile + (e % kTile)];
-    }
-    const int mu = idx[k0 + i_loc];
-#pragma unroll
-    for (int j = 0; j < P_PER_T; ++j) rP[j] = (mu >= 0 && nu_g[j] >= 0) ? P[mu + (int64_t)n * nu_g[j]] : T(0);
-  };
-  auto store = [&](int b) {
-#pragma unroll
-    for (int j = 0; j < PHI_PER_T; ++j) {
-      const int e = tid + 256 * j;
-      sPhi[b][e / kTile][e % kTile] = rPhi[j];
-    }
-#pragma unroll
-    for (int j = 0; j < P_PER_T; ++j) sP[b][i_loc][(tid >> 4) + 16 * j] = rP[j];
-  };
-
-  const int nch = npad / KC;
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int ch = 0; ch < nch; ++ch) {
-    const int b = ch & 1;
-    if (ch + 1 < nch) load((ch + 1) * KC);
-#pragma unroll
-    for (int ks = 0; ks < KC / 4; ++ks) {
-      const int kk = ks * 4 + (lane >> 4);
-      const T a0 = sPhi[b][kk][32 * w + (lane & 15)];
-      const T a1 = sPhi[b][kk][32 * w + 16 + (lane & 15)];
-#pragma unroll
-      for (int c = 0; c < NCT; ++c) {
-        const T bv = sP[b][kk][16 * c + (lane & 15)];
-        acc[0][c] = M::step(a0, bv, acc[0][c]);
-        acc[1][c] = M::step(a1, bv, acc[1][c]);
-      }
-    }
-    if (ch + 1 < nch) store(b ^ 1);
-    __syncthreads();
-  }
-
-  // dots with phi and grad phi, 32 columns at a time through LDS (aliases sPhi)
-  T(*sT)[PS] = reinterpret_cast<T(*)[PS]>(&sPhi[0][0][0]);
-  double racc = 0.0, gxa = 0.0, gya = 0.0, gza = 0.0;
-#pragma unroll
-  for (int cg = 0; cg < NCT; cg += 2) {
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int cc = 0; cc < 2; ++cc) {
-        if (cg + cc >= NCT) continue;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) sT[16 * cc + (lane & 15)][32 * w + 16 * r + M::row(lane, i)] = acc[r][cg + cc][i];
-      }
-    __syncthreads();
-    for (int j = 0; j < 16; ++j) {
-      const int nl = 16 * h + j;
-      const int col = col0 + 16 * cg + nl;
-      if (16 * cg + nl < NC && col < npad) {
-        const double tv = (double)sT[nl][p];
-        const int64_t row = (int64_t)col * kTile + p;
-        racc += tv * (double)tb[row];
-        if (want_grad) {
-          gxa += tv * (double)tb[row + cstride];
-          gya += tv * (double)tb[row + 2 * cstride];
-          gza += tv * (double)tb[row + 3 * cstride];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  if (h == 1) {
-    red[0][p] = racc;
-    red[1][p] = gxa;
-    red[2][p] = gya;
-    red[3][p] = gza;
-  }
-  __syncthreads();
-  if (h == 0 && p < bk.npts) {
-    const int64_t q = bk.first + p;
-    double* dst = part + (int64_t)wk.pass * 4 * npts + q;
-    dst[0] = racc + red[0][p];
-    if (want_grad) {
-      dst[npts] = gxa + red[1][p];
-      dst[2 * npts] = gya + red[2][p];
-      dst[3 * npts] = gza + red[3][p];
-    }
-  }
-}
-
 // Density contraction on 16 x 16 sub-tiles.  One workgroup of 8 wavefronts
 // per (block, column pass): T[p][nu] = sum_mu phi[mu][p] P[mu][nu] for the
 // pass's 16 S columns, wavefront w owning points 16 w .. 16 w + 15 and all S
@@ -321,6 +182,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
   typedef T vec_t __attribute__((ext_vector_type(KV)));
   __shared__ __attribute__((aligned(16))) unsigned char smem[STAGE_BYTES > DOT_BYTES ? STAGE_BYTES : DOT_BYTES];
   __shared__ double redv[4][4][kTile];
+  __shared__ int sIdx[kDensIdxLds];
   T(*sA)[kTile][RS] = reinterpret_cast<T(*)[kTile][RS]>(smem);
   T(*sB)[NC][RS] = reinterpret_cast<T(*)[NC][RS]>(smem + 2 * kTile * RS * sizeof(T));
 
@@ -334,6 +196,13 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
   const T* tb = tiles + bk.tile_off;
   const int* idx = ao_idx + bk.idx_off;
   const int64_t cstride = (int64_t)npad * kTile;
+  // the block's AO index list in LDS (P rows of a chunk are then gathered without a
+  // dependent global load); wider blocks read it from global memory
+  const bool idx_lds = npad <= kDensIdxLds;
+  if (idx_lds)
+    for (int i = tid; i < npad; i += 512) sIdx[i] = idx[i];
+  __syncthreads();
+  const int* lidx = idx_lds ? sIdx : idx;
 
   // staging: phi task = (point pa, AO quad qa); P tasks = (column cb, AO quad qb)
   const int pa = tid & (kTile - 1), qa = tid >> 7;
@@ -341,7 +210,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
 #pragma unroll
   for (int m = 0; m < NB; ++m) {
     const int task = tid + 512 * m, cb = task >> 2;
-    nu[m] = (cb < ncols) ? idx[col0 + cb] : -1;
+    nu[m] = (cb < ncols) ? lidx[col0 + cb] : -1;
   }
   vec_t rA, rB[NB];
   auto load = [&](int k0) {
@@ -352,7 +221,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
       const int qb = (tid + 512 * m) & 3;
 #pragma unroll
       for (int e = 0; e < KV; ++e) {
-        const int mu = idx[k0 + KV * qb + e];
+        const int mu = lidx[k0 + KV * qb + e];
         rB[m][e] = (mu >= 0 && nu[m] >= 0) ? P[mu + (int64_t)n * nu[m]] : T(0);
       }
     }
@@ -577,326 +446,6 @@ __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm,
   if (t == 0) e_blk[b] = eb;
 }
 
-// Fock contraction.  One workgroup per (block, macro-tile pair I <= J) of edge
-// MS = 32 WT AOs; wave w owns the (w / 2, w % 2) quadrant as WT x WT MFMA
-// tiles in registers.  The block's points (K = 128) stream through two LDS
-// buffers 8 at a time, G = (w v / 2) phi + sum_k (w g_k) d_k phi being formed
-// while staging.  S_IJ = phi_I^T G_J + G_I^T phi_J is added with FP64 atomics
-// into the lower triangle of U: entry (row, col) with row <= col goes to
-// U[col + n row], so the 16 lanes of a row group hit consecutive addresses
-// (diagonal macro-tiles: only entries with row <= col).
-template <typename T, int WT>
-__global__ __launch_bounds__(256) void fock_kernel(const WorkItem* __restrict__ work, const DevBlock* __restrict__ blocks,
-                                                   const int* __restrict__ ao_idx, const T* __restrict__ tiles,
-                                                   const double* __restrict__ vs, const double* __restrict__ gw,
-                                                   int gga, double thr, double* __restrict__ U, int n, int64_t npts) {
-  using M = Mfma<T>;
-  constexpr int MS = 32 * WT;
-  constexpr int KC = 8;
-  constexpr int EPS = MS * KC / 256;
-  constexpr int MSP = MS + 16;  // padded rows (LDS bank spread)
-  __shared__ T sPI[2][KC][MSP], sGI[2][KC][MSP], sPJ[2][KC][MSP], sGJ[2][KC][MSP];
-  __shared__ double spot[4][kTile];
-  __shared__ double red[kTile];
-  const WorkItem wk = work[blockIdx.x];
-  const DevBlock bk = blocks[wk.block];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int npad = bk.npad;
-  const T* tb = tiles + bk.tile_off;
-  const int64_t cstride = (int64_t)npad * kTile;
-  const int I0 = wk.a, J0 = wk.b;
-  const bool diag = I0 == J0;
-  double part = 0.0;
-  if (tid < kTile) {
-    const bool in = tid < bk.npts;
-    const int64_t q = bk.first + tid;
-    spot[0][tid] = in ? vs[q] : 0.0;
-    spot[1][tid] = (in && gga) ? gw[q] : 0.0;
-    spot[2][tid] = (in && gga) ? gw[npts + q] : 0.0;
-    spot[3][tid] = (in && gga) ? gw[2 * npts + q] : 0.0;
-    part = fabs(spot[0][tid]) + fabs(spot[1][tid]) + fabs(spot[2][tid]) + fabs(spot[3][tid]);
-    red[tid] = part;
-  }
-  __syncthreads();
-  for (int s = 64; s > 0; s >>= 1) {
-    if (tid < s) red[tid] += red[tid + s];
-    __syncthreads();
-  }
-  // block average of |w v| (+ |w g|) for the prescreening; uniform over the workgroup
-  if (red[0] / bk.npts < thr) return;
-
-  const int pp = tid & (KC - 1);
-  T rI[EPS][4], rJ[EPS][4];
-  auto load = [&](int p0) {
-#pragma unroll
-    for (int j = 0; j < EPS; ++j) {
-      const int i = (tid >> 3) + 32 * j;
-#pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        if (side == 1 && diag) break;
-        T(&r)[4] = side ? rJ[j] : rI[j];
-        const int mu = (side ? J0 : I0) + i;
-        if (mu < npad) {
-          const T* src = tb + (int64_t)mu * kTile + p0 + pp;
-          r[0] = src[0];
-          if (gga) {
-            r[1] = src[cstride];
-            r[2] = src[2 * cstride];
-            r[3] = src[3 * cstride];
-          }
-        } else {
-          r[0] = r[1] = r[2] = r[3] = T(0);
-        }
-      }
-    }
-  };
-  auto store = [&](int b, int p0) {
-    const int pt = p0 + pp;
-    const double sv = 0.5 * spot[0][pt];
-#pragma unroll
-    for (int j = 0; j < EPS; ++j) {
-      const int i = (tid >> 3) + 32 * j;
-#pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        if (side == 1 && diag) break;
-        const T(&r)[4] = side ? rJ[j] : rI[j];
-        double gv = sv * (double)r[0];
-        if (gga) gv += spot[1][pt] * (double)r[1] + spot[2][pt] * (double)r[2] + spot[3][pt] * (double)r[3];
-        if (side) {
-          sPJ[b][pp][i] = r[0];
-          sGJ[b][pp][i] = (T)gv;
-        } else {
-          sPI[b][pp][i] = r[0];
-          sGI[b][pp][i] = (T)gv;
-        }
-      }
-    }
-  };
-
-  typename M::acc_t acc[WT][WT];
-#pragma unroll
-  for (int r = 0; r < WT; ++r)
-#pragma unroll
-    for (int c = 0; c < WT; ++c) acc[r][c] = M::zero();
-  const int rowbase = (w >> 1) * 16 * WT, colbase = (w & 1) * 16 * WT;
-  constexpr int NCH = kTile / KC;
-  load(0);
-  store(0, 0);
-  __syncthreads();
-  for (int ch = 0; ch < NCH; ++ch) {
-    const int b = ch & 1;
-    if (ch + 1 < NCH) load((ch + 1) * KC);
-#pragma unroll
-    for (int ks = 0; ks < KC / 4; ++ks) {
-      const int kk = ks * 4 + (lane >> 4);
-      T aP[WT], aG[WT], bP[WT], bG[WT];
-#pragma unroll
-      for (int r = 0; r < WT; ++r) {
-        aP[r] = sPI[b][kk][rowbase + 16 * r + (lane & 15)];
-        aG[r] = sGI[b][kk][rowbase + 16 * r + (lane & 15)];
-      }
-#pragma unroll
-      for (int c = 0; c < WT; ++c) {
-        bP[c] = diag ? sPI[b][kk][colbase + 16 * c + (lane & 15)] : sPJ[b][kk][colbase + 16 * c + (lane & 15)];
-        bG[c] = diag ? sGI[b][kk][colbase + 16 * c + (lane & 15)] : sGJ[b][kk][colbase + 16 * c + (lane & 15)];
-      }
-#pragma unroll
-      for (int r = 0; r < WT; ++r)
-#pragma unroll
-        for (int c = 0; c < WT; ++c) {
-          acc[r][c] = M::step(aP[r], bG[c], acc[r][c]);
-          acc[r][c] = M::step(aG[r], bP[c], acc[r][c]);
-        }
-    }
-    if (ch + 1 < NCH) store(b ^ 1, (ch + 1) * KC);
-    __syncthreads();
-  }
-  const int* idx = ao_idx + bk.idx_off;
-#pragma unroll
-  for (int r = 0; r < WT; ++r)
-#pragma unroll
-    for (int c = 0; c < WT; ++c)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = I0 + rowbase + 16 * r + M::row(lane, i);
-        const int col = J0 + colbase + 16 * c + (lane & 15);
-        if (row >= npad || col >= npad) continue;
-        const int gr = idx[row], gc = idx[col];
-        if (gr < 0 || gc < 0) continue;
-        if (diag && gr > gc) continue;
-        unsafeAtomicAdd(U + gc + (int64_t)n * gr, (double)acc[r][c][i]);
-      }
-}
-
-// Fock contraction over the upper triangle of 32 x 32 AO tile pairs (I <= J,
-// enumerated row by row).  A work item holds up to 4 MAXP consecutive pairs of
-// one block; wave w takes pairs w, w + 4, ... with 2 x 2 MFMA tiles each in
-// registers.  Per chunk of 8 points the workgroup stages phi and G for the AO
-// rows [row0, npad) of the block (row0: first row tile of the item) through
-// two LDS buffers, G = (w v / 2) phi + sum_k (w g_k) d_k phi formed while
-// staging.  Only blocks with npad - row0 <= RMAX rows use this kernel.
-template <typename T>
-struct FockPairsCfg {
-  static constexpr int RMAX = sizeof(T) == 8 ? 256 : 512;
-  static constexpr int MAXP = sizeof(T) == 8 ? 4 : 8;
-};
-
-__device__ inline void decode_pair(int idx, int nt, int& I, int& J) {
-  // row I starts at I * nt - I (I - 1) / 2
-  I = 0;
-  int start = 0;
-  while (start + (nt - I) <= idx) {
-    start += nt - I;
-    ++I;
-  }
-  J = I + (idx - start);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void fock_pairs_kernel(const WorkItem* __restrict__ work,
-                                                         const DevBlock* __restrict__ blocks,
-                                                         const int* __restrict__ ao_idx, const T* __restrict__ tiles,
-                                                         const double* __restrict__ vs, const double* __restrict__ gw,
-                                                         int gga, double thr, double* __restrict__ U, int n,
-                                                         int64_t npts) {
-  using M = Mfma<T>;
-  constexpr int RMAX = FockPairsCfg<T>::RMAX;
-  constexpr int MAXP = FockPairsCfg<T>::MAXP;
-  constexpr int KC = 8;
-  constexpr int JMAX = RMAX / 32;
-  // rows padded by 16 elements: the four lane groups of an MFMA operand read
-  // (consecutive rows) then fall on different LDS banks
-  constexpr int RS = RMAX + 16;
-  __shared__ T sPhi[2][KC][RS], sG[2][KC][RS];
-  __shared__ double spot[4][kTile];
-  __shared__ double red[kTile];
-  const WorkItem wk = work[blockIdx.x];
-  const DevBlock bk = blocks[wk.block];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int npad = bk.npad, nt = npad / 32;
-  const T* tb = tiles + bk.tile_off;
-  const int64_t cstride = (int64_t)npad * kTile;
-  if (tid < kTile) {
-    const bool in = tid < bk.npts;
-    const int64_t q = bk.first + tid;
-    spot[0][tid] = in ? vs[q] : 0.0;
-    spot[1][tid] = (in && gga) ? gw[q] : 0.0;
-    spot[2][tid] = (in && gga) ? gw[npts + q] : 0.0;
-    spot[3][tid] = (in && gga) ? gw[2 * npts + q] : 0.0;
-    red[tid] = fabs(spot[0][tid]) + fabs(spot[1][tid]) + fabs(spot[2][tid]) + fabs(spot[3][tid]);
-  }
-  __syncthreads();
-  for (int st = 64; st > 0; st >>= 1) {
-    if (tid < st) red[tid] += red[tid + st];
-    __syncthreads();
-  }
-  if (red[0] / bk.npts < thr) return;
-
-  int I0, J0;
-  decode_pair(wk.a, nt, I0, J0);
-  const int row0 = 32 * I0;
-  const int nrows = npad - row0;  // multiple of 32, <= RMAX
-  const int nj = nrows / 32;
-  int pI[MAXP], pJ[MAXP];
-#pragma unroll
-  for (int q = 0; q < MAXP; ++q) {
-    const int idx = wk.a + w + 4 * q;
-    if (idx < wk.a + wk.b) {
-      decode_pair(idx, nt, pI[q], pJ[q]);
-    } else {
-      pI[q] = -1;
-      pJ[q] = -1;
-    }
-  }
-  const int pp = tid & (KC - 1);
-  T raw[JMAX][4];
-  auto load = [&](int p0) {
-#pragma unroll
-    for (int j = 0; j < JMAX; ++j) {
-      if (j < nj) {
-        const T* src = tb + (int64_t)(row0 + (tid >> 3) + 32 * j) * kTile + p0 + pp;
-        raw[j][0] = src[0];
-        if (gga) {
-          raw[j][1] = src[cstride];
-          raw[j][2] = src[2 * cstride];
-          raw[j][3] = src[3 * cstride];
-        }
-      }
-    }
-  };
-  auto store = [&](int b, int p0) {
-    const int pt = p0 + pp;
-    const double sv = 0.5 * spot[0][pt];
-#pragma unroll
-    for (int j = 0; j < JMAX; ++j) {
-      if (j < nj) {
-        double gv = sv * (double)raw[j][0];
-        if (gga) gv += spot[1][pt] * (double)raw[j][1] + spot[2][pt] * (double)raw[j][2] + spot[3][pt] * (double)raw[j][3];
-        const int lr = (tid >> 3) + 32 * j;
-        sPhi[b][pp][lr] = raw[j][0];
-        sG[b][pp][lr] = (T)gv;
-      }
-    }
-  };
-  typename M::acc_t acc[MAXP][2][2];
-#pragma unroll
-  for (int q = 0; q < MAXP; ++q)
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) acc[q][r][c] = M::zero();
-  constexpr int NCH = kTile / KC;
-  load(0);
-  store(0, 0);
-  __syncthreads();
-  for (int ch = 0; ch < NCH; ++ch) {
-    const int b = ch & 1;
-    if (ch + 1 < NCH) load((ch + 1) * KC);
-#pragma unroll
-    for (int ks = 0; ks < KC / 4; ++ks) {
-      const int kk = ks * 4 + (lane >> 4);
-#pragma unroll
-      for (int q = 0; q < MAXP; ++q) {
-        if (pI[q] < 0) continue;
-        const int li = 32 * pI[q] - row0 + (lane & 15), lj = 32 * pJ[q] - row0 + (lane & 15);
-        const T aP0 = sPhi[b][kk][li], aP1 = sPhi[b][kk][li + 16];
-        const T aG0 = sG[b][kk][li], aG1 = sG[b][kk][li + 16];
-        const T bP0 = sPhi[b][kk][lj], bP1 = sPhi[b][kk][lj + 16];
-        const T bG0 = sG[b][kk][lj], bG1 = sG[b][kk][lj + 16];
-        acc[q][0][0] = M::step(aP0, bG0, acc[q][0][0]);
-        acc[q][0][1] = M::step(aP0, bG1, acc[q][0][1]);
-        acc[q][1][0] = M::step(aP1, bG0, acc[q][1][0]);
-        acc[q][1][1] = M::step(aP1, bG1, acc[q][1][1]);
-        acc[q][0][0] = M::step(aG0, bP0, acc[q][0][0]);
-        acc[q][0][1] = M::step(aG0, bP1, acc[q][0][1]);
-        acc[q][1][0] = M::step(aG1, bP0, acc[q][1][0]);
-        acc[q][1][1] = M::step(aG1, bP1, acc[q][1][1]);
-      }
-    }
-    if (ch + 1 < NCH) store(b ^ 1, (ch + 1) * KC);
-    __syncthreads();
-  }
-  const int* idx = ao_idx + bk.idx_off;
-#pragma unroll
-  for (int q = 0; q < MAXP; ++q) {
-    if (pI[q] < 0) continue;
-    const bool diag = pI[q] == pJ[q];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int row = 32 * pI[q] + 16 * r + M::row(lane, i);
-          const int col = 32 * pJ[q] + 16 * c + (lane & 15);
-          const int gr = idx[row], gc = idx[col];
-          if (gr < 0 || gc < 0) continue;
-          if (diag && gr > gc) continue;
-          unsafeAtomicAdd(U + gc + (int64_t)n * gr, (double)acc[q][r][c][i]);
-        }
-  }
-}
-
 // Sub-tile Fock contraction.  One workgroup of 8 wavefronts per work item; the
 // item's 16 x 16 AO sub-tiles (i, j) are dealt to the wavefronts round-robin,
 // S per wavefront, each accumulating S_ij = phi_i^T G_j + G_i^T phi_j over the
@@ -926,7 +475,7 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
   typedef T vec_t __attribute__((ext_vector_type(KV)));
   __shared__ __attribute__((aligned(16))) T sPhi[kFockRows][RS];
   __shared__ __attribute__((aligned(16))) T sG[kFockRows][RS];
-  __shared__ double spot[4][kTile];
+  __shared__ __attribute__((aligned(16))) double spot[kTile][4];  // w v, w g_x, w g_y, w g_z of each point
   __shared__ double red[2];
 
   const FockItem it = work[blockIdx.x];
@@ -936,11 +485,15 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
   if (tid < kTile) {
     const bool in = tid < bk.npts;
     const int64_t q = bk.first + tid;
-    spot[0][tid] = in ? vs[q] : 0.0;
-    spot[1][tid] = (in && gga) ? gw[q] : 0.0;
-    spot[2][tid] = (in && gga) ? gw[npts + q] : 0.0;
-    spot[3][tid] = (in && gga) ? gw[2 * npts + q] : 0.0;
-    double a = fabs(spot[0][tid]) + fabs(spot[1][tid]) + fabs(spot[2][tid]) + fabs(spot[3][tid]);
+    const double v0 = in ? vs[q] : 0.0;
+    const double v1 = (in && gga) ? gw[q] : 0.0;
+    const double v2 = (in && gga) ? gw[npts + q] : 0.0;
+    const double v3 = (in && gga) ? gw[2 * npts + q] : 0.0;
+    spot[tid][0] = 0.5 * v0;
+    spot[tid][1] = v1;
+    spot[tid][2] = v2;
+    spot[tid][3] = v3;
+    double a = fabs(v0) + fabs(v1) + fabs(v2) + fabs(v3);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
     if (lane == 0) red[w] = a;
@@ -1001,7 +554,19 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
     }
   };
   auto store = [&](int c) {
-    const int p0 = c * KC;
+    // the KV points of this thread's tasks are the same for every task (task & 3 == tid & 3)
+    const int p0 = c * KC + KV * (tid & 3);
+    typedef double dvec2 __attribute__((ext_vector_type(2)));
+    double wt[KV][4];
+#pragma unroll
+    for (int e = 0; e < KV; ++e) {
+      const dvec2 lo = *reinterpret_cast<const dvec2*>(&spot[p0 + e][0]);
+      const dvec2 hi = *reinterpret_cast<const dvec2*>(&spot[p0 + e][2]);
+      wt[e][0] = lo[0];
+      wt[e][1] = lo[1];
+      wt[e][2] = hi[0];
+      wt[e][3] = hi[1];
+    }
 #pragma unroll
     for (int m = 0; m < NTASK; ++m) {
       const int task = tid + 512 * m;
@@ -1010,11 +575,9 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
         vec_t gv;
 #pragma unroll
         for (int e = 0; e < KV; ++e) {
-          const int pt = p0 + KV * q + e;
-          double v = 0.5 * spot[0][pt] * (double)raw[m][0][e];
+          double v = wt[e][0] * (double)raw[m][0][e];
           if (gga)
-            v += spot[1][pt] * (double)raw[m][1][e] + spot[2][pt] * (double)raw[m][2][e] +
-                 spot[3][pt] * (double)raw[m][3][e];
+            v += wt[e][1] * (double)raw[m][1][e] + wt[e][2] * (double)raw[m][2][e] + wt[e][3] * (double)raw[m][3][e];
           gv[e] = (T)v;
         }
         *reinterpret_cast<vec_t*>(&sPhi[lr][KV * q]) = raw[m][0];
@@ -1176,35 +739,6 @@ void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const in
 }
 
 template <typename T>
-void launch_density(hipStream_t s, int nct, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
-                    const T* tiles, const T* P, int n, int want_grad, double* part, int64_t npts) {
-  if (nwork <= 0) return;
-#define SXC_DENSITY_CASE(K)                                                                                   \
-  case K:                                                                                                     \
-    hipLaunchKernelGGL((density_kernel<T, K>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, P, n, \
-                       want_grad, part, npts);                                                                \
-    break;
-  switch (nct) {
-    SXC_DENSITY_CASE(1)
-    SXC_DENSITY_CASE(2)
-    SXC_DENSITY_CASE(3)
-    SXC_DENSITY_CASE(4)
-    SXC_DENSITY_CASE(5)
-    SXC_DENSITY_CASE(6)
-    SXC_DENSITY_CASE(7)
-    default:
-      hipLaunchKernelGGL((density_kernel<T, 8>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, P, n,
-                         want_grad, part, npts);
-      break;
-  }
-#undef SXC_DENSITY_CASE
-}
-template void launch_density<double>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
-                                     const double*, const double*, int, int, double*, int64_t);
-template void launch_density<float>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*, const float*,
-                                    const float*, int, int, double*, int64_t);
-
-template <typename T>
 void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
                         const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
                         int64_t npts) {
@@ -1252,46 +786,6 @@ void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npt
   hipLaunchKernelGGL(functional_kernel, dim3(nb), dim3(128), 0, s, func, prm, npts, bs, rho, grad, blocks, part, w,
                      eps, v, g, vs, gw, e_blk, rho_sub, grad_sub);
 }
-
-template <typename T>
-void launch_fock(hipStream_t s, int cls, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
-                 const T* tiles, const double* vs, const double* gw, int gga, double thr, double* U, int n,
-                 int64_t npts) {
-  if (nwork <= 0) return;
-  switch (cls) {
-    case 0:
-      hipLaunchKernelGGL((fock_kernel<T, 1>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
-                         thr, U, n, npts);
-      break;
-    case 1:
-      hipLaunchKernelGGL((fock_kernel<T, 2>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
-                         thr, U, n, npts);
-      break;
-    default:
-      hipLaunchKernelGGL((fock_kernel<T, 4>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
-                         thr, U, n, npts);
-      break;
-  }
-}
-template void launch_fock<double>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*, const double*,
-                                  const double*, const double*, int, double, double*, int, int64_t);
-template void launch_fock<float>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*, const float*,
-                                 const double*, const double*, int, double, double*, int, int64_t);
-
-template <typename T>
-void launch_fock_pairs(hipStream_t s, const WorkItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
-                       const T* tiles, const double* vs, const double* gw, int gga, double thr, double* U, int n,
-                       int64_t npts) {
-  if (nwork <= 0) return;
-  hipLaunchKernelGGL((fock_pairs_kernel<T>), dim3(nwork), dim3(256), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
-                     thr, U, n, npts);
-}
-template void launch_fock_pairs<double>(hipStream_t, const WorkItem*, int, const DevBlock*, const int*, const double*,
-                                        const double*, const double*, int, double, double*, int, int64_t);
-template void launch_fock_pairs<float>(hipStream_t, const WorkItem*, int, const DevBlock*, const int*, const float*,
-                                       const double*, const double*, int, double, double*, int, int64_t);
-int fock_pairs_rmax(int fp64) { return fp64 ? FockPairsCfg<double>::RMAX : FockPairsCfg<float>::RMAX; }
-int fock_pairs_maxp(int fp64) { return fp64 ? FockPairsCfg<double>::MAXP : FockPairsCfg<float>::MAXP; }
 
 template <typename T, int S>
 static void launch_fock_sub_s(hipStream_t s, const FockItem* work, int nwork, const DevBlock* blocks,

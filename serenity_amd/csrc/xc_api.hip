@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -58,9 +59,50 @@ int guarded(F&& f) {
 
 }  // namespace
 
+// Execution lanes of a build: its stream plus kAux helper streams.  The launches
+// of one stage (one per work class) are independent, so they are spread over
+// the lanes after a fork and joined before the next stage.
+constexpr int kAux = 3;
+struct Lanes {
+  hipStream_t main = nullptr;
+  hipStream_t aux[kAux] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kAux] = {};
+  void create(hipStream_t m) {
+    main = m;
+    for (int i = 0; i < kAux; ++i) {
+      HIPX(hipStreamCreateWithFlags(&aux[i], hipStreamNonBlocking));
+      HIPX(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
+    }
+    HIPX(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  }
+  void destroy() {
+    for (int i = 0; i < kAux; ++i) {
+      if (aux[i]) (void)hipStreamDestroy(aux[i]);
+      if (join[i]) (void)hipEventDestroy(join[i]);
+      aux[i] = nullptr;
+      join[i] = nullptr;
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    fork = nullptr;
+  }
+  hipStream_t lane(int i) const { return i % (kAux + 1) == 0 ? main : aux[i % (kAux + 1) - 1]; }
+  void fork_all() const {
+    HIPX(hipEventRecord(fork, main));
+    for (int i = 0; i < kAux; ++i) HIPX(hipStreamWaitEvent(aux[i], fork, 0));
+  }
+  void join_all() const {
+    for (int i = 0; i < kAux; ++i) {
+      HIPX(hipEventRecord(join[i], aux[i]));
+      HIPX(hipStreamWaitEvent(main, join[i], 0));
+    }
+  }
+};
+
 struct xc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  Lanes lanes;  // helper lanes of builds on the context stream
   std::recursive_mutex mu;
   int64_t bytes = 0;
   double last_energy = 0.0;
@@ -184,20 +226,37 @@ struct xc_bfogc {
 
 namespace {
 
-template <typename T>
-void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad, hipStream_t s) {
-  for (int c = kDensMaxS; c >= 1; --c)
-    launch_density_sub<T>(s, c, bf->d_dens_work[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
-                      bf->d_ao_idx.p, tiles, P, bf->basis->nao, want_grad, bf->d_part.p, bf->grid->npts);
+// Launch order and lanes of a stage's work classes: the non-empty classes, the
+// largest first, dealt round-robin over the lanes (fork / join only when more
+// than one class has work).
+template <typename F>
+void over_lanes(const Lanes& L, int nclass, const std::function<int(int)>& count, F&& launch) {
+  int nz = 0;
+  for (int c = nclass - 1; c >= 0; --c) nz += count(c) > 0 ? 1 : 0;
+  if (nz > 1) L.fork_all();
+  int k = 0;
+  for (int c = nclass - 1; c >= 0; --c)
+    if (count(c) > 0) launch(c, nz > 1 ? L.lane(k++) : L.main);
+  if (nz > 1) L.join_all();
 }
 
 template <typename T>
-void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr, hipStream_t s) {
+void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad, const Lanes& L) {
+  over_lanes(L, kDensMaxS + 1, [&](int c) { return c >= 1 ? (int)bf->dens_work[c].size() : 0; },
+             [&](int c, hipStream_t s) {
+    launch_density_sub<T>(s, c, bf->d_dens_work[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
+                      bf->d_ao_idx.p, tiles, P, bf->basis->nao, want_grad, bf->d_part.p, bf->grid->npts);
+  });
+}
+
+template <typename T>
+void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr, const Lanes& L) {
   const int f = sizeof(T) == 8 ? 1 : 0;
-  for (int c = kFockNumS - 1; c >= 0; --c)
+  over_lanes(L, kFockNumS, [&](int c) { return (int)bf->fock_items[f][c].size(); }, [&](int c, hipStream_t s) {
     launch_fock_sub<T>(s, c, bf->d_fock_items[f][c].p, (int)bf->fock_items[f][c].size(),
                        bf->d_blocks.p, bf->d_ao_idx.p, tiles, bf->d_vs.p, bf->d_gw.p, gga, thr, bf->d_U.p,
                        bf->basis->nao, bf->grid->npts);
+  });
 }
 
 }  // namespace
@@ -218,6 +277,7 @@ int xc_ctx_create(int device, xc_ctx** out) {
     c->device = device;
     HIPX(hipSetDevice(device));
     HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->lanes.create(c->stream);
     c->d_energy = (double*)c->alloc(sizeof(double));
     *out = c;
   });
@@ -232,6 +292,7 @@ int xc_ctx_destroy(xc_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     ctx->release(ctx->d_energy, sizeof(double));
+    ctx->lanes.destroy();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });
@@ -323,6 +384,55 @@ int xc_molecular_grid(int natoms, const int* Z, const double* coords, int grid_t
     s.weight_threshold = weight_threshold;
     s.sort = sort != 0;
     MolecularGrid g = build_molecular_grid(std::vector<int>(Z, Z + natoms), std::vector<double>(coords, coords + 3 * natoms), s);
+    const int64_t n = (int64_t)g.weights.size();
+    *npoints = n;
+    *points = (double*)std::malloc(sizeof(double) * 3 * std::max<int64_t>(n, 1));
+    *weights = (double*)std::malloc(sizeof(double) * std::max<int64_t>(n, 1));
+    std::memcpy(*points, g.points.data(), sizeof(double) * 3 * n);
+    std::memcpy(*weights, g.weights.data(), sizeof(double) * n);
+    if (npoints_before_cut) *npoints_before_cut = g.n_before_cut;
+  });
+}
+
+int xc_molecular_grid_device(xc_ctx* ctx, int natoms, const int* Z, const double* coords, int grid_type,
+                             int radial_type, int accuracy, int smoothing, double weight_threshold, int sort,
+                             int64_t* npoints, double** points, double** weights, int64_t* npoints_before_cut) {
+  return guarded([&] {
+    require(ctx && natoms > 0 && Z && coords && npoints && points && weights, XC_ERR_ARG,
+            "xc_molecular_grid_device: bad arguments");
+    require(grid_type >= 0 && grid_type <= 2 && radial_type >= 0 && radial_type <= 4, XC_ERR_ARG,
+            "xc_molecular_grid_device: unknown grid or radial type");
+    GridSettings s;
+    s.type = (GridType)grid_type;
+    s.radial = (RadialType)radial_type;
+    s.accuracy = accuracy;
+    s.smoothing = (unsigned)smoothing;
+    s.weight_threshold = weight_threshold;
+    s.sort = sort != 0;
+    const GridProblem P =
+        grid_problem(std::vector<int>(Z, Z + natoms), std::vector<double>(coords, coords + 3 * natoms), s);
+    const int64_t nc = (int64_t)P.cand_w.size();
+    std::vector<double> w((size_t)nc);
+    {
+      std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+      HIPX(hipSetDevice(ctx->device));
+      DBuf<double> d_xyz, d_dist, d_apair, d_min, d_cxyz, d_cw, d_w;
+      DBuf<int> d_catom;
+      d_xyz.upload(ctx, P.xyz.data(), P.xyz.size());
+      d_dist.upload(ctx, P.dist.data(), P.dist.size());
+      d_apair.upload(ctx, P.apair.data(), P.apair.size());
+      d_min.upload(ctx, P.min_dist.data(), P.min_dist.size());
+      d_cxyz.upload(ctx, P.cand_xyz.data(), P.cand_xyz.size());
+      d_cw.upload(ctx, P.cand_w.data(), P.cand_w.size());
+      d_catom.upload(ctx, P.cand_atom.data(), P.cand_atom.size());
+      d_w.alloc(ctx, (size_t)std::max<int64_t>(nc, 1));
+      launch_partition(ctx->stream, P.type, P.smoothing, P.natoms, d_xyz.p, d_dist.p, d_apair.p, d_min.p, nc,
+                       d_cxyz.p, d_cw.p, d_catom.p, d_w.p);
+      HIPX(hipGetLastError());
+      if (nc) HIPX(hipMemcpyAsync(w.data(), d_w.p, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
+      HIPX(hipStreamSynchronize(ctx->stream));
+    }
+    MolecularGrid g = finish_grid(P, w, s);
     const int64_t n = (int64_t)g.weights.size();
     *npoints = n;
     *points = (double*)std::malloc(sizeof(double) * 3 * std::max<int64_t>(n, 1));
@@ -712,7 +822,7 @@ int xc_density(xc_bfogc* bf, const double* P, double* rho, double* grad, int* no
     const int n = bf->basis->nao;
     const int64_t N = bf->grid->npts;
     HIPX(hipMemcpyAsync(bf->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
-    run_density<double>(bf, bf->d_t64.p, bf->d_P.p, grad ? 1 : 0, ctx->stream);
+    run_density<double>(bf, bf->d_t64.p, bf->d_P.p, grad ? 1 : 0, ctx->lanes);
     launch_reduce_partials(ctx->stream, bf->d_blocks.p, bf->bs, N, bf->d_part.p, grad ? 1 : 0, bf->d_rho.p,
                            bf->d_grad.p);
     HIPX(hipGetLastError());
@@ -777,7 +887,7 @@ int xc_fock_add(xc_bfogc* bf, double thr, const double* v, const double* g, doub
     launch_weight_potential(ctx->stream, N, bf->grid->d_w.p, bf->d_v.p, g ? bf->d_g.p : nullptr, bf->d_vs.p,
                             bf->d_gw.p);
     launch_fill(ctx->stream, bf->d_U.p, (int64_t)n * n, 0.0);
-    run_fock<double>(bf, bf->d_t64.p, g ? 1 : 0, thr, ctx->stream);
+    run_fock<double>(bf, bf->d_t64.p, g ? 1 : 0, thr, ctx->lanes);
     HIPX(hipMemcpyAsync(bf->d_F.p, F, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
     launch_finalize(ctx->stream, bf->d_U.p, bf->d_F.p, n, 1);
     HIPX(hipGetLastError());
@@ -789,8 +899,9 @@ int xc_fock_add(xc_bfogc* bf, double thr, const double* v, const double* g, doub
 // One build enqueued on stream s, E_xc left in d_energy (device).  Stage
 // events are recorded only on the context stream.
 static void enqueue_build(xc_bfogc* bf, int functional, int precision, double thr, const double* dP, double* dF,
-                          hipStream_t s, double* d_energy) {
+                          const Lanes& L, double* d_energy) {
   xc_ctx* ctx = bf->ctx;
+  hipStream_t s = L.main;
   require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
   require(functional >= 0 && functional <= 6, XC_ERR_ARG, "unknown functional");
   require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
@@ -805,9 +916,9 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   if (precision == XC_PREC_FP32) launch_to_float(s, dP, bf->d_P32.p, (int64_t)n * n);
   if (s == ctx->stream) ctx->mark();
   if (precision == XC_PREC_FP64)
-    run_density<double>(bf, bf->d_t64.p, dP, gga ? 1 : 0, s);
+    run_density<double>(bf, bf->d_t64.p, dP, gga ? 1 : 0, L);
   else
-    run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0, s);
+    run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0, L);
   if (s == ctx->stream) ctx->mark();
   if (bf->bs == kTile) {
     launch_functional(s, functional, ctx->prm, N, kTile, nullptr, nullptr, bf->d_blocks.p, bf->d_part.p,
@@ -821,9 +932,9 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   launch_fill(s, bf->d_U.p, (int64_t)n * n, 0.0);
   if (s == ctx->stream) ctx->mark();
   if (precision == XC_PREC_FP64)
-    run_fock<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr, s);
+    run_fock<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr, L);
   else
-    run_fock<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr, s);
+    run_fock<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr, L);
   if (s == ctx->stream) ctx->mark();
   launch_finalize(s, bf->d_U.p, dF, n, 0);
   launch_sum(s, bf->d_eblk.p, nxc, d_energy);
@@ -878,9 +989,9 @@ static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int func
   for (size_t i = 0; i < sub.size(); ++i) {
     xc_bfogc* b = sub[i];
     if (fp32)
-      run_density<float>(b, b->d_t32.p, b->d_P32.p, gga ? 1 : 0, s);
+      run_density<float>(b, b->d_t32.p, b->d_P32.p, gga ? 1 : 0, ctx->lanes);
     else
-      run_density<double>(b, b->d_t64.p, Ps[i], gga ? 1 : 0, s);
+      run_density<double>(b, b->d_t64.p, Ps[i], gga ? 1 : 0, ctx->lanes);
     launch_reduce_partials(s, b->d_blocks.p, b->bs, N, b->d_part.p, gga ? 1 : 0, b->d_rho.p, b->d_grad.p, 0);
     launch_reduce_partials(s, b->d_blocks.p, b->bs, N, b->d_part.p, gga ? 1 : 0, act->d_rho_sup.p,
                            act->d_grad_sup.p, i == 0 ? 0 : 1);
@@ -900,9 +1011,9 @@ static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int func
   launch_fill(s, act->d_U.p, (int64_t)n * n, 0.0);
   ctx->mark();
   if (fp32)
-    run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr, s);
+    run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr, ctx->lanes);
   else
-    run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr, s);
+    run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr, ctx->lanes);
   ctx->mark();
   launch_finalize(s, act->d_U.p, dF, n, 0);
   ctx->mark();
@@ -929,9 +1040,9 @@ int xc_bfogc_update_density(xc_bfogc* bf, int precision, const double* dP, int w
     const int g = with_gradient ? 1 : 0;
     if (precision == XC_PREC_FP32) {
       launch_to_float(ctx->stream, dP, bf->d_P32.p, (int64_t)n * n);
-      run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, g, ctx->stream);
+      run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, g, ctx->lanes);
     } else {
-      run_density<double>(bf, bf->d_t64.p, dP, g, ctx->stream);
+      run_density<double>(bf, bf->d_t64.p, dP, g, ctx->lanes);
     }
     launch_reduce_partials(ctx->stream, bf->d_blocks.p, bf->bs, N, bf->d_part.p, g, bf->d_rho_c.p, bf->d_grad_c.p,
                            0);
@@ -1017,9 +1128,9 @@ int xc_nadd_fock_build_cached(xc_bfogc* act, int nenv, xc_bfogc* const* env, int
     launch_fill(s, act->d_U.p, (int64_t)n * n, 0.0);
     ctx->mark();
     if (precision == XC_PREC_FP32)
-      run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr, s);
+      run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr, ctx->lanes);
     else
-      run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr, s);
+      run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr, ctx->lanes);
     ctx->mark();
     launch_finalize(s, act->d_U.p, dF, n, 0);
     ctx->mark();
@@ -1079,7 +1190,7 @@ int xc_fock_build_device(xc_bfogc* bf, int functional, int precision, double thr
     require(bf && dP && dF, XC_ERR_ARG, "xc_fock_build_device: null argument");
     xc_ctx* ctx = bf->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    enqueue_build(bf, functional, precision, thr, dP, dF, ctx->stream, ctx->d_energy);
+    enqueue_build(bf, functional, precision, thr, dP, dF, ctx->lanes, ctx->d_energy);
     if (energy) {
       HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
       HIPX(hipStreamSynchronize(ctx->stream));
@@ -1106,7 +1217,7 @@ int xc_fock_build(xc_bfogc* bf, int functional, int precision, double thr, const
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     const int n = bf->basis->nao;
     HIPX(hipMemcpyAsync(bf->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
-    enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p, ctx->stream, ctx->d_energy);
+    enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p, ctx->lanes, ctx->d_energy);
     HIPX(hipMemcpyAsync(F, bf->d_F.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPX(hipStreamSynchronize(ctx->stream));
@@ -1189,6 +1300,7 @@ int xc_device_copy(xc_ctx* ctx, void* dst, const void* src, size_t bytes) {
 struct xc_server {
   struct Slot {
     hipStream_t stream = nullptr;
+    Lanes lanes;
     hipEvent_t done = nullptr;
     double* h_P = nullptr;  // pinned staging of P, F and E
     double* h_F = nullptr;
@@ -1246,11 +1358,13 @@ int xc_server_create(xc_ctx* ctx, int nslots, int use_graphs, xc_server** out) {
     try {
       for (auto& sl : srv->slots) {
         HIPX(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        sl.lanes.create(sl.stream);
         HIPX(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
         HIPX(hipHostMalloc((void**)&sl.h_E, sizeof(double), hipHostMallocDefault));
       }
     } catch (...) {
       for (auto& sl : srv->slots) {
+        sl.lanes.destroy();
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.h_E) (void)hipHostFree(sl.h_E);
@@ -1276,6 +1390,7 @@ int xc_server_destroy(xc_server* srv) {
       server_slot_release(sl);
       (void)hipHostFree(sl.h_E);
       (void)hipEventDestroy(sl.done);
+      sl.lanes.destroy();
       (void)hipStreamDestroy(sl.stream);
     }
     --srv->ctx->live_handles;
@@ -1319,7 +1434,7 @@ int xc_server_submit(xc_server* srv, xc_bfogc* bf, int functional, int precision
         hipGraph_t graph = nullptr;
         HIPX(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-          enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p, s, bf->d_E.p);
+          enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p, sl->lanes, bf->d_E.p);
         } catch (...) {
           (void)hipStreamEndCapture(s, &graph);
           if (graph) (void)hipGraphDestroy(graph);
@@ -1335,7 +1450,7 @@ int xc_server_submit(xc_server* srv, xc_bfogc* bf, int functional, int precision
       HIPX(hipGraphLaunch(it->second, s));
       ++srv->graph_launches;
     } else {
-      enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p, s, bf->d_E.p);
+      enqueue_build(bf, functional, precision, thr, bf->d_P.p, bf->d_F.p, sl->lanes, bf->d_E.p);
     }
     HIPX(hipMemcpyAsync(sl->h_F, bf->d_F.p, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
     HIPX(hipMemcpyAsync(sl->h_E, bf->d_E.p, sizeof(double), hipMemcpyDeviceToHost, s));

@@ -48,10 +48,12 @@ def make_stream(n_requests, qps, seed=W.SEED):
 
 
 class HostSystems:
-    """Host-side data of every system: geometry, grid, basis and density matrix (built once)."""
+    """Host-side data of every system: geometry, grid (partitioned on the GPU when a context is
+    given), basis and density matrix (built once)."""
 
-    def __init__(self, seed=W.SEED):
+    def __init__(self, seed=W.SEED, ctx=None):
         self.seed = seed
+        self.ctx = ctx
         self.grids = {}
         self.data = {}
 
@@ -60,7 +62,7 @@ class HostSystems:
             kind, size, basis = key
             Z, X = W.water_chain(size, seed=self.seed) if kind == "water" else W.alkane(size, seed=self.seed)
             if (kind, size) not in self.grids:
-                self.grids[(kind, size)] = xc.molecular_grid(Z, X)
+                self.grids[(kind, size)] = xc.molecular_grid(Z, X, ctx=self.ctx)
             b = xc.BasisController.from_file(basis, Z, X)
             sys_seed = self.seed + 100000 * (kind == "alkane") + 100 * size + BASES.index(basis)
             P = W.synthetic_density(b.overlap(), W.n_electrons(Z) // 2, seed=sys_seed)
@@ -102,7 +104,7 @@ class ResidentSystems:
 def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="FP32", log=None):
     """Serve the stream in real time; returns latency percentiles, throughput and counts."""
     say = log or (lambda *a: None)
-    host = HostSystems()
+    host = HostSystems(ctx=ctx)
     res = ResidentSystems(ctx, host, budget_gb * 1e9, tiles="only" if precision == "FP32" else False)
     keys = sorted({k for _, k, _ in stream})
     t0 = time.perf_counter()

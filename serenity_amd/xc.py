@@ -201,17 +201,22 @@ class GridController:
 
 
 def molecular_grid(Z, coords_bohr, grid_type="SSF", radial_type="AHLRICHS", accuracy=4, smoothing=3,
-                   weight_threshold=1e-14, sort=True, return_before_cut=False):
-    """Atom-centred integration grid (GridFactory::produce + Hilbert sort)."""
+                   weight_threshold=1e-14, sort=True, return_before_cut=False, ctx=None):
+    """Atom-centred integration grid (GridFactory::produce + Hilbert sort).  With ``ctx`` the
+    fuzzy-cell partition runs on that context's GPU (same grid as the host build)."""
     Z = np.ascontiguousarray(Z, dtype=np.int32)
     xyz = _f64(np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3))
     L = _lib.lib()
     n = C.c_int64()
     nb = C.c_int64()
     pp, pw = _lib.dp(), _lib.dp()
-    check(L.xc_molecular_grid(len(Z), _i(Z), _d(xyz), GRID_TYPES[grid_type], RADIAL_TYPES[radial_type],
-                              int(accuracy), int(smoothing), float(weight_threshold), int(bool(sort)),
-                              C.byref(n), C.byref(pp), C.byref(pw), C.byref(nb)))
+    args = (len(Z), _i(Z), _d(xyz), GRID_TYPES[grid_type], RADIAL_TYPES[radial_type], int(accuracy),
+            int(smoothing), float(weight_threshold), int(bool(sort)), C.byref(n), C.byref(pp), C.byref(pw),
+            C.byref(nb))
+    if ctx is None:
+        check(L.xc_molecular_grid(*args))
+    else:
+        check(L.xc_molecular_grid_device(ctx.handle, *args))
     try:
         N = n.value
         pts = np.ctypeslib.as_array(pp, shape=(N * 3,)).copy().reshape(N, 3)

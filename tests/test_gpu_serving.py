@@ -109,3 +109,17 @@ def test_fp32_only_controller(ctx):
         xc.FuncPotential(only, "PBE", "FP64").getMatrix(s["P"])
     with pytest.raises(xc.XCError, match="FP32 tiles only"):
         only.getBlockOnGridData(0)
+
+
+@pytest.mark.parametrize("grid_type", ["SSF", "BECKE", "VORONOI"])
+def test_device_grid_partition_matches_host(ctx, grid_type):
+    """The fuzzy-cell partition on the GPU gives the host grid: same points in the same (Hilbert)
+    order, weights equal to rounding."""
+    for Z, X in (W.water_cluster(2, 2, 2), W.alkane(6)):
+        gh, nh = xc.molecular_grid(Z, X, grid_type=grid_type, return_before_cut=True)
+        t0 = time.time()
+        gd, nd = xc.molecular_grid(Z, X, grid_type=grid_type, return_before_cut=True, ctx=ctx)
+        say("device grid", grid_type, len(Z), "atoms", gd.getNGridPoints(), "points %.3f s" % (time.time() - t0))
+        assert nh == nd and gh.getNGridPoints() == gd.getNGridPoints()
+        assert np.array_equal(gh.points, gd.points)
+        assert np.abs(gh.weights - gd.weights).max() <= 1e-14 * np.abs(gh.weights).max()
