@@ -64,12 +64,15 @@ int guarded(F&& f) {
 // the lanes after a fork and joined before the next stage.
 constexpr int kAux = 3;
 struct Lanes {
+  int n = 1;  // lanes in use: 1 (the build's stream only) or kAux + 1
   hipStream_t main = nullptr;
   hipStream_t aux[kAux] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[kAux] = {};
-  void create(hipStream_t m) {
+  void create(hipStream_t m, bool helpers) {
     main = m;
+    n = helpers ? kAux + 1 : 1;
+    if (!helpers) return;
     for (int i = 0; i < kAux; ++i) {
       HIPX(hipStreamCreateWithFlags(&aux[i], hipStreamNonBlocking));
       HIPX(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
@@ -233,11 +236,12 @@ template <typename F>
 void over_lanes(const Lanes& L, int nclass, const std::function<int(int)>& count, F&& launch) {
   int nz = 0;
   for (int c = nclass - 1; c >= 0; --c) nz += count(c) > 0 ? 1 : 0;
-  if (nz > 1) L.fork_all();
+  const bool par = nz > 1 && L.n > 1;
+  if (par) L.fork_all();
   int k = 0;
   for (int c = nclass - 1; c >= 0; --c)
-    if (count(c) > 0) launch(c, nz > 1 ? L.lane(k++) : L.main);
-  if (nz > 1) L.join_all();
+    if (count(c) > 0) launch(c, par ? L.lane(k++) : L.main);
+  if (par) L.join_all();
 }
 
 template <typename T>
@@ -277,7 +281,7 @@ int xc_ctx_create(int device, xc_ctx** out) {
     c->device = device;
     HIPX(hipSetDevice(device));
     HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    c->lanes.create(c->stream);
+    c->lanes.create(c->stream, true);
     c->d_energy = (double*)c->alloc(sizeof(double));
     *out = c;
   });
@@ -1358,7 +1362,8 @@ int xc_server_create(xc_ctx* ctx, int nslots, int use_graphs, xc_server** out) {
     try {
       for (auto& sl : srv->slots) {
         HIPX(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-        sl.lanes.create(sl.stream);
+        // concurrency comes from the slots: a slot's build stays on its own stream
+        sl.lanes.create(sl.stream, false);
         HIPX(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
         HIPX(hipHostMalloc((void**)&sl.h_E, sizeof(double), hipHostMallocDefault));
       }
