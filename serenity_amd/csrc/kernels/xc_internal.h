@@ -53,9 +53,9 @@ struct FockItem {
 };
 constexpr int kFockRows = 256;        // rows a sub-tile item may stage
 constexpr int kFockPanel = 128;       // panel height of blocks split into panel pairs
-constexpr int kFockNumS = 12;         // slot-count classes of the sub-tile kernel
-constexpr int kFockS[kFockNumS] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 17};
-inline int fock_smax(int fp64) { return fp64 ? 12 : 17; }  // register budget of the accumulators
+constexpr int kFockNumS = 9;          // slot-count classes of the sub-tile kernel
+constexpr int kFockS[kFockNumS] = {1, 2, 3, 4, 5, 6, 7, 8, 10};
+inline int fock_smax(int) { return 10; }  // accumulator budget without spills
 
 // Work item of the density kernel: block, a = first column of the pass,
 // b = columns of the pass (multiple of 16), pass = partial-density slot.

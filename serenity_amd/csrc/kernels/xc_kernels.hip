@@ -154,10 +154,10 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
 // per (block, column pass): T[p][nu] = sum_mu phi[mu][p] P[mu][nu] for the
 // pass's 16 S columns, wavefront w owning points 16 w .. 16 w + 15 and all S
 // column tiles (S independent accumulators).  The AO (K) dimension streams in
-// chunks of KC (16 FP32 / 8 FP64) through two LDS buffers that hold phi
+// chunks of KC (32 FP32 / 16 FP64) through an LDS buffer that holds phi
 // point-major and the pass's P columns, both with the AO index fastest: with
 // k-step t of lane group g taking AO KV g + t (KV = KC / 4) every MFMA operand
-// of a chunk is one 16-byte LDS read.  The global loads of chunk k + 1 are
+// of a chunk is one 32-byte LDS read.  The global loads of chunk k + 1 are
 // issued before the MFMAs of chunk k.  The pass's partial rho and
 // sum_nu T grad phi are then formed by dotting T (transposed through LDS)
 // with phi / grad phi read coalesced from the tiles.
@@ -169,22 +169,22 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
                                                           int n, int want_grad, double* __restrict__ part,
                                                           int64_t npts) {
   using M = Mfma<T>;
-  constexpr int KC = sizeof(T) == 8 ? 8 : 16;
+  constexpr int KC = 128 / (int)sizeof(T);      // AOs per chunk (128 bytes of each LDS row)
   constexpr int KV = KC / 4;
-  constexpr int RS = KC + 16 / (int)sizeof(T);  // 80-byte rows
+  constexpr int RS = KC + 16 / (int)sizeof(T);  // 144-byte rows
   constexpr int NC = 16 * S;
   constexpr int NB = (4 * NC + 511) / 512;      // P staging tasks per thread
-  constexpr int GS = 4;
+  constexpr int GS = 2;
   constexpr int CG = 32;                        // columns per transposition round of the dot phase
   constexpr int PS = kTile + 1;                 // odd row stride of the transposed T
-  constexpr int STAGE_BYTES = 2 * (kTile + NC) * RS * (int)sizeof(T);
+  constexpr int STAGE_BYTES = (kTile + NC) * RS * (int)sizeof(T);
   constexpr int DOT_BYTES = CG * PS * (int)sizeof(T);
   typedef T vec_t __attribute__((ext_vector_type(KV)));
   __shared__ __attribute__((aligned(16))) unsigned char smem[STAGE_BYTES > DOT_BYTES ? STAGE_BYTES : DOT_BYTES];
   __shared__ double redv[4][4][kTile];
   __shared__ int sIdx[kDensIdxLds];
-  T(*sA)[kTile][RS] = reinterpret_cast<T(*)[kTile][RS]>(smem);
-  T(*sB)[NC][RS] = reinterpret_cast<T(*)[NC][RS]>(smem + 2 * kTile * RS * sizeof(T));
+  T(*sA)[RS] = reinterpret_cast<T(*)[RS]>(smem);
+  T(*sB)[RS] = reinterpret_cast<T(*)[RS]>(smem + kTile * RS * sizeof(T));
 
   const WorkItem wk = work[blockIdx.x];
   const DevBlock bk = blocks[wk.block];
@@ -226,32 +226,32 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
       }
     }
   };
-  auto store = [&](int b) {
-    *reinterpret_cast<vec_t*>(&sA[b][pa][KV * qa]) = rA;
+  auto store = [&]() {
+    *reinterpret_cast<vec_t*>(&sA[pa][KV * qa]) = rA;
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
       const int task = tid + 512 * m;
-      if ((task >> 2) < NC) *reinterpret_cast<vec_t*>(&sB[b][task >> 2][KV * (task & 3)]) = rB[m];
+      if ((task >> 2) < NC) *reinterpret_cast<vec_t*>(&sB[task >> 2][KV * (task & 3)]) = rB[m];
     }
   };
 
   typename M::acc_t acc[S];
 #pragma unroll
   for (int c = 0; c < S; ++c) acc[c] = M::zero();
+  // one LDS buffer; the global loads of chunk k + 1 are in flight during the MFMAs of chunk k
   const int nch = npad / KC;
   load(0);
-  store(0);
-  __syncthreads();
   for (int ch = 0; ch < nch; ++ch) {
-    const int b = ch & 1;
+    store();
+    __syncthreads();
     if (ch + 1 < nch) load((ch + 1) * KC);
-    const vec_t av = *reinterpret_cast<const vec_t*>(&sA[b][16 * w + r][KV * g]);
+    const vec_t av = *reinterpret_cast<const vec_t*>(&sA[16 * w + r][KV * g]);
 #pragma unroll
     for (int s0 = 0; s0 < S; s0 += GS) {
       vec_t bv[GS];
 #pragma unroll
       for (int u = 0; u < GS; ++u)
-        if (s0 + u < S) bv[u] = *reinterpret_cast<const vec_t*>(&sB[b][16 * (s0 + u) + r][KV * g]);
+        if (s0 + u < S) bv[u] = *reinterpret_cast<const vec_t*>(&sB[16 * (s0 + u) + r][KV * g]);
 #pragma unroll
       for (int t = 0; t < KV; ++t)
 #pragma unroll
@@ -259,7 +259,6 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
           if (s0 + u < S) acc[s0 + u] = M::step(av[t], bv[u][t], acc[s0 + u]);
       __builtin_amdgcn_sched_barrier(0);
     }
-    if (ch + 1 < nch) store(b ^ 1);
     __syncthreads();
   }
 
@@ -450,14 +449,14 @@ __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm,
 // item's 16 x 16 AO sub-tiles (i, j) are dealt to the wavefronts round-robin,
 // S per wavefront, each accumulating S_ij = phi_i^T G_j + G_i^T phi_j over the
 // block's 128 points in registers (dummy slots of a short item compute on row
-// 0 and are dropped).  The points stream in chunks of KC (16 FP32 / 8 FP64):
-// the 16-byte global loads of chunk c + 1 (along the point axis of the
-// AO-major tiles) are issued before the MFMAs of chunk c, and a chunk is
+// 0 and are dropped).  The points stream in chunks of KC (32 FP32 / 16 FP64,
+// one 128-byte line of each AO row): the global loads of chunk c + 1 (along
+// the point axis of the AO-major tiles) are issued before the MFMAs of chunk c, and a chunk is
 // turned into phi and G = (w v / 2) phi + sum_k (w g_k) d_k phi rows of LDS
 // on its way in.  Inside a chunk, k-step t of lane group g takes point
-// KV g + t (KV = KC / 4), so each MFMA operand of a chunk is one 16-byte LDS
+// KV g + t (KV = KC / 4), so each MFMA operand of a chunk is one 32-byte LDS
 // read of KV consecutive points of one AO row.  MFMAs are issued in groups of
-// four independent accumulators.
+// two independent accumulators.
 template <typename T, int S>
 __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restrict__ work,
                                                        const DevBlock* __restrict__ blocks,
@@ -466,12 +465,12 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
                                                        int gga, double thr, double* __restrict__ U, int n,
                                                        int64_t npts) {
   using M = Mfma<T>;
-  constexpr int KC = sizeof(T) == 8 ? 8 : 16;
+  constexpr int KC = 128 / (int)sizeof(T);       // points per chunk: one 128-byte line of each AO row
   constexpr int KV = KC / 4;
-  constexpr int RS = KC + 16 / (int)sizeof(T);  // 80-byte LDS rows: the 16 rows of an operand read hit distinct banks
+  constexpr int RS = KC + 16 / (int)sizeof(T);  // 144-byte LDS rows: the 16 rows of an operand read hit distinct banks
   constexpr int NCH = kTile / KC;
   constexpr int NTASK = kFockRows * 4 / 512;
-  constexpr int GS = sizeof(T) == 8 ? 2 : 4;  // independent accumulators per MFMA group
+  constexpr int GS = 2;                          // independent accumulators per MFMA group
   typedef T vec_t __attribute__((ext_vector_type(KV)));
   __shared__ __attribute__((aligned(16))) T sPhi[kFockRows][RS];
   __shared__ __attribute__((aligned(16))) T sG[kFockRows][RS];
@@ -813,19 +812,7 @@ void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, con
     SXC_FOCK_S(7)
     SXC_FOCK_S(8)
     SXC_FOCK_S(10)
-    SXC_FOCK_S(12)
 #undef SXC_FOCK_S
-    case 14:
-    case 17:
-      if constexpr (sizeof(T) == 4) {
-        switch (kFockS[sc]) {
-          case 14:
-            return launch_fock_sub_s<T, 14>(s, work, nwork, blocks, ao_idx, tiles, vs, gw, gga, thr, U, n, npts);
-          case 17:
-            return launch_fock_sub_s<T, 17>(s, work, nwork, blocks, ao_idx, tiles, vs, gw, gga, thr, U, n, npts);
-        }
-      }
-      break;
   }
   throw std::runtime_error("launch_fock_sub: slot class beyond the accumulator budget of this precision");
 }
