@@ -11,9 +11,12 @@ of each against the other and against the CPU oracle are reported.
 Beside the headline line the run reports (in the same JSON object) the
 embedding sweep of C6H6 + 24 H2O (non-additive PBE + TF potentials) and a
 serving stream of 2000 Poisson-arriving builds at a fixed QPS (p50/p99 latency,
-builds/s; setup of the resident systems is not timed).
+builds/s; setup of the resident systems is not timed), and the full restricted
+SCF of (H2O)8/def2-SVP/LDA with RI-J (SMALL-grid iterations, DEFAULT-grid final
+energy) in FP64 and in the mixed form (FP32 iterations + FP64 finish), beside the
+same SCF driven by the CPU oracle's XC build.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-fde] [--no-serving]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-fde] [--no-serving] [--no-scf]
 """
 import argparse
 import json
@@ -146,6 +149,44 @@ def fde_sweep(ctx, steps, warmup, check_oracle):
     return out
 
 
+def scf_leg(ctx, check_oracle):
+    """Full SCF of (H2O)8/def2-SVP/LDA (RI-J): FP64, mixed with FP64 finish, and the oracle-driven SCF."""
+    from serenity_amd import scf as SCF
+    Z, X = W.geometry(W.CONFIGS[1]["system"])
+    t0 = time.perf_counter()
+    task = SCF.ScfTask(ctx, Z, X, basis="DEF2-SVP", functional="LDA")
+    out = dict(workload="(H2O)8/def2-SVP/LDA, RI-J (RI_J_WEIGEND), SCF on the accuracy-2 grid, final energy on "
+                        "the accuracy-4 grid", nao=task.basis.nao, naux=task.auxbasis.nao,
+               npts_small=task.grid_small.getNGridPoints(), npts_default=task.grid_default.getNGridPoints(),
+               setup_s=round(time.perf_counter() - t0, 3))
+    task.run("FP64")  # first run builds the resident tiles of both grids
+    r64 = task.run("FP64")
+    rmx = task.run("MIXED")
+    out["fp64"] = dict(energy=r64["energy"], cycles=r64["cycles"], wall_s=r64["wall_s"], xc_s=r64["t_xc_s"])
+    out["mixed"] = dict(energy_before_fp64_finish=rmx["energy_before_fp64_finish"], energy=rmx["energy"],
+                        cycles_fp32=rmx["cycles_fp32"], cycles_fp64_finish=rmx["cycles_fp64_finish"],
+                        wall_s=rmx["wall_s"])
+    out["mixed"]["dE_before_finish_vs_fp64"] = abs(rmx["energy_before_fp64_finish"] - r64["energy"])
+    out["mixed"]["dE_after_finish_vs_fp64"] = abs(rmx["energy"] - r64["energy"])
+    if check_oracle:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        b, gs, gd = task.basis, task.grid_small, task.grid_default
+
+        def build(P):
+            F, E, _, _ = O.fock_build(b, gs.points, gs.weights, P, "LDA")
+            return E, F
+        r = SCF.perform(task.ints, task.nocc, build)
+        _, e_xc, _, _ = O.fock_build(b, gd.points, gd.weights, r["P"], "LDA")
+        e = SCF.total_energy(task.ints, r["P"], e_xc)
+        out["cpu_oracle"] = dict(energy=e, cycles=r["cycles"], wall_s=round(time.perf_counter() - t0, 3),
+                                 threads=O.max_threads())
+        out["fp64"]["dE_vs_cpu_oracle"] = abs(r64["energy"] - e)
+        out["mixed"]["dE_after_finish_vs_cpu_oracle"] = abs(rmx["energy"] - e)
+    task.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +197,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fde", action="store_true", help="skip the embedding (non-additive potential) sweep")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-scf", action="store_true", help="skip the full SCF of (H2O)8/def2-SVP/LDA")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--serving-qps", type=float, default=400.0)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -257,6 +299,17 @@ def main():
             (fde["ms_per_sweep"], fde["density_updates_per_sweep"], fde["nadd_builds_per_sweep"],
              fde["ms_direct_nadd_build"]))
         out_extra["fde"] = fde
+
+    if not args.no_scf:
+        sc = scf_leg(ctx, not args.no_cpu_baseline)
+        log("SCF (H2O)8/SVP/LDA: FP64 E %.10f (%d cycles, %.2f s); mixed before finish dE %.2e, after %.2e "
+            "(%d+%d cycles, %.2f s)%s" %
+            (sc["fp64"]["energy"], sc["fp64"]["cycles"], sc["fp64"]["wall_s"],
+             sc["mixed"]["dE_before_finish_vs_fp64"], sc["mixed"]["dE_after_finish_vs_fp64"],
+             sc["mixed"]["cycles_fp32"], sc["mixed"]["cycles_fp64_finish"], sc["mixed"]["wall_s"],
+             "" if "cpu_oracle" not in sc else "; CPU oracle SCF %.2f s, |dE| %.2e" %
+             (sc["cpu_oracle"]["wall_s"], sc["fp64"]["dE_vs_cpu_oracle"])))
+        out_extra["scf"] = sc
 
     if not args.no_serving:
         from serenity_amd import serving as SV

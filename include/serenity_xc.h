@@ -36,9 +36,11 @@ enum {
 
 /* Functionals available on the device.  LDA = Slater + VWN5, PBE = PBE
  * exchange + PBE correlation (PW92 LDA part), TF = Thomas-Fermi kinetic.
- * Serenity: functional_definitions.dat:44,47,80. */
+ * Serenity: functional_definitions.dat:44,47,80.  XC_FUNC_PBE carries the
+ * constants of XCFun (Serenity's default evaluator); XC_FUNC_PBE_LIBXC those of
+ * LibXC (customFunc.impl = LIBXC), which differ in the exchange mu. */
 enum { XC_FUNC_LDA = 0, XC_FUNC_PBE = 1, XC_FUNC_TF = 2, XC_FUNC_SLATER = 3, XC_FUNC_VWN5 = 4,
-       XC_FUNC_PBEX = 5, XC_FUNC_PBEC = 6 };
+       XC_FUNC_PBEX = 5, XC_FUNC_PBEC = 6, XC_FUNC_PBE_LIBXC = 7 };
 
 /* Precision of the two dense contractions (phi.P and phi^T.(w v phi)).  The
  * functional, the E_xc reduction and the F accumulation are FP64 in every mode. */
@@ -110,6 +112,35 @@ int xc_lebedev(int npoints, double* x, double* y, double* z, double* w);
 /* Analytic overlap matrix S (n x n column-major) of a shell list. */
 int xc_overlap(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
                const double* shell_center, const double* alpha, const double* coef, double* S);
+
+/* ---- integrals of the SCF driver (host, OpenMP) --------------------------
+ * The full SCF around the XC build needs the one-electron Hamiltonian and the
+ * Coulomb matrix, which Serenity takes from libint2 through
+ * OneElectronIntegralController (src/integrals/OneElectronIntegralController.cpp)
+ * and ERIPotential / RI_J_IntegralController
+ * (src/integrals/RI_J_IntegralController.cpp:41-121).  Shells as in
+ * xc_basis_create, l <= 4.  AO pairs are packed as
+ * pair(mu, nu) = mu (mu + 1) / 2 + nu, mu >= nu; npair = n (n + 1) / 2. */
+/* Boys function F_m(T), 0 <= m <= 17. */
+int xc_boys(int m, double T, double* F);
+/* Kinetic energy T and nuclear attraction V of point nuclei (charges Z, coords
+ * 3 x natoms in bohr), n x n column-major. */
+int xc_one_electron(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
+                    const double* shell_center, const double* alpha, const double* coef, int natoms, const int* Z,
+                    const double* coords, double* T, double* V);
+/* Four-centre Coulomb integrals (mu nu|la si): eri is npair x npair; quartets
+ * whose Schwarz bound is below schwarz_threshold are zero.  At most 2^31
+ * entries (XC_ERR_INPUT beyond). */
+int xc_eri_packed(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
+                  const double* shell_center, const double* alpha, const double* coef, double schwarz_threshold,
+                  double* eri);
+/* Density-fitted Coulomb integrals over an auxiliary shell list: B3 is
+ * naux x npair (row P holds (P|mu nu)), M the naux x naux metric (P|Q). */
+int xc_ri_j_integrals(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
+                      const double* shell_center, const double* alpha, const double* coef, int naux_shell,
+                      const int* aux_l, const int* aux_pure, const int* aux_nprim, const double* aux_center,
+                      const double* aux_alpha, const double* aux_coef, double schwarz_threshold, double* B3,
+                      double* M);
 
 /* ---- resident objects --------------------------------------------------- */
 /* Shells in AO order.  shell_center: 3 x nshell; alpha/coef: concatenated

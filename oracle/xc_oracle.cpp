@@ -243,10 +243,16 @@ void ao_block(const Basis& bas, const double* pts, int B, double thr, int deriv,
 }
 
 // ---- functionals: closed forms with hand-derived first derivatives -------
+// XCFun's PBE (Serenity's default evaluator): mu = 0.066725 pi^2 / 3; LibXC's:
+// mu = 0.2195149727645171 (= beta pi^2 / 3 with the accurate beta).  Both: PW92
+// A = 0.0310907, accurate beta in the correlation.  Decided by the full-SCF
+// energies of qcserenity/tests/test_tasks.py:50-76 (water/STO-3G PBE, XCFun
+// and LibXC runs).
 struct FParams {
   double vwn_A = 0.0310907, vwn_b = 3.72744, vwn_c = 12.9352, vwn_x0 = -0.10498;
-  double pw_A = 0.031091, pw_a1 = 0.21370, pw_b1 = 7.5957, pw_b2 = 3.5876, pw_b3 = 1.6382, pw_b4 = 0.49294;
-  double kappa = 0.804, mu = 0.2195149727645171, beta = 0.06672455060314922;
+  double pw_A = 0.0310907, pw_a1 = 0.21370, pw_b1 = 7.5957, pw_b2 = 3.5876, pw_b3 = 1.6382, pw_b4 = 0.49294;
+  double kappa = 0.804, mu = 0.066725 * M_PI * M_PI / 3.0, beta = 0.06672455060314922;
+  double mu_libxc = 0.2195149727645171;
 };
 
 // e, de/drho, de/dsigma
@@ -325,7 +331,7 @@ EV tf(double r) {
   return {cf * r * r23, (5.0 / 3.0) * cf * r23, 0.0};
 }
 
-bool is_gga(int f) { return f == 1 || f == 5 || f == 6; }
+bool is_gga(int f) { return f == 1 || f == 5 || f == 6 || f == 7; }
 
 EV eval(int f, double r, double sig, const FParams& P) {
   EV a{0, 0, 0}, b{0, 0, 0};
@@ -337,6 +343,12 @@ EV eval(int f, double r, double sig, const FParams& P) {
     case 4: a = vwn5(r, P); break;
     case 5: a = pbex(r, sig, P); break;
     case 6: a = pbec(r, sig, P); break;
+    case 7: {
+      FParams L = P;
+      L.mu = P.mu_libxc;
+      a = pbex(r, sig, L);
+      b = pbec(r, sig, P);
+    } break;
     default: break;
   }
   return {a.e + b.e, a.dr + b.dr, a.ds + b.ds};
@@ -473,7 +485,7 @@ int oracle_density(int nshell, const int* l, const int* pure, const int* nprim, 
 }
 
 // Functional on N points in blocks of bs: eps, v, g (3 x N, nullable), energy.
-// params: NULL or {vwn_A, pw_A, kappa, mu, beta}.
+// params: NULL or {vwn_A, pw_A, kappa, mu, beta} (mu of the XCFun form; id 7 keeps LibXC's mu).
 int oracle_functional(int func, int64_t N, int bs, const double* rho, const double* grad, const double* w,
                       const double* params, double* eps, double* v, double* g, double* energy) {
   const FParams P = params_from(params);

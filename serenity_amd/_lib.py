@@ -11,8 +11,8 @@ LIB_PATH = os.path.join(HERE, "libserenity_xc.so")
 
 XC_OK = 0
 XC_BUSY = 6
-FUNCTIONALS = {"LDA": 0, "PBE": 1, "TF": 2, "SLATER": 3, "VWN5": 4, "PBEX": 5, "PBEC": 6}
-GGA = {"PBE", "PBEX", "PBEC"}
+FUNCTIONALS = {"LDA": 0, "PBE": 1, "TF": 2, "SLATER": 3, "VWN5": 4, "PBEX": 5, "PBEC": 6, "PBE_LIBXC": 7}
+GGA = {"PBE", "PBEX", "PBEC", "PBE_LIBXC"}
 PRECISIONS = {"FP64": 0, "FP32": 1}
 STAGES = ["convert", "density", "functional", "clear", "fock", "finalize"]
 GRID_TYPES = {"BECKE": 0, "VORONOI": 1, "SSF": 2}
@@ -66,6 +66,11 @@ def lib():
          C.c_int, C.POINTER(i64), C.POINTER(dp), C.POINTER(dp), C.POINTER(i64))
     _sig("xc_lebedev", C.c_int, C.c_int, dp, dp, dp, dp)
     _sig("xc_overlap", C.c_int, C.c_int, ip, ip, ip, dp, dp, dp, dp)
+    _sig("xc_boys", C.c_int, C.c_int, C.c_double, dp)
+    _sig("xc_one_electron", C.c_int, C.c_int, ip, ip, ip, dp, dp, dp, C.c_int, ip, dp, dp, dp)
+    _sig("xc_eri_packed", C.c_int, C.c_int, ip, ip, ip, dp, dp, dp, C.c_double, dp)
+    _sig("xc_ri_j_integrals", C.c_int, C.c_int, ip, ip, ip, dp, dp, dp, C.c_int, ip, ip, ip, dp, dp, dp,
+         C.c_double, dp, dp)
     _sig("xc_basis_create", C.c_int, vp, C.c_int, ip, ip, ip, dp, dp, dp, C.POINTER(vp))
     _sig("xc_basis_destroy", C.c_int, vp)
     _sig("xc_basis_nao", C.c_int, vp, ip)

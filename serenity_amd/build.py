@@ -19,6 +19,7 @@ SOURCES = [
     "host/basis.cpp",
     "host/grid.cpp",
     "host/overlap.cpp",
+    "host/integrals.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

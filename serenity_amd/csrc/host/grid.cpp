@@ -224,11 +224,6 @@ void atom_grid(int Z, const GridSettings& s, std::vector<double>& pts, std::vect
   }
 }
 
-constexpr double becke_p(double nu, unsigned k) {
-  return (k > 1) ? becke_p((3.0 - nu * nu) * nu / 2.0, k - 1) : (3.0 - nu * nu) * nu / 2.0;
-}
-constexpr double becke_s(double nu, unsigned k) { return 0.5 * (1 - becke_p(nu, k)); }
-
 double dist3(const double* a, const double* b) {
   const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
   return std::sqrt(dx * dx + dy * dy + dz * dz);

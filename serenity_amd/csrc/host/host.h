@@ -102,4 +102,20 @@ void hilbert_sort(std::vector<double>& points, std::vector<double>& weights);
 // Analytic overlap matrix S (n x n, column-major) of a shell list.
 void overlap_matrix(const std::vector<Shell>& shells, std::vector<double>& S);
 
+// ---- integrals of the SCF driver (integrals.cpp) ----------------------------
+// AO pairs are packed as pair(mu, nu) = mu (mu + 1) / 2 + nu for mu >= nu.
+// Boys function F_m(T), m <= 17.
+double boys_function(int m, double T);
+// Kinetic energy T and nuclear attraction V (point nuclei of charge Z at xyz,
+// bohr), both n x n column-major.
+void one_electron_matrices(const std::vector<Shell>& shells, const std::vector<int>& Z, const std::vector<double>& xyz,
+                           std::vector<double>& T, std::vector<double>& V);
+// Four-centre Coulomb integrals (mu nu|la si) as an npair x npair matrix
+// (npair = n (n + 1) / 2); shell quartets below the Schwarz threshold are zero.
+void eri_packed(const std::vector<Shell>& shells, double schwarz_threshold, std::vector<double>& eri);
+// Density fitting of the Coulomb matrix: three-centre integrals (P|mu nu) as
+// an naux x npair matrix B3 (row P) and the metric (P|Q), naux x naux.
+void ri_j_integrals(const std::vector<Shell>& shells, const std::vector<Shell>& aux, double schwarz_threshold,
+                    std::vector<double>& B3, std::vector<double>& M);
+
 }  // namespace sxc

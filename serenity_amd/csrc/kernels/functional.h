@@ -60,13 +60,18 @@ SXC_HD D2 dpow(D2 a, double p) {
   return chain(a, f, p * f / a.v);
 }
 
-// Constants of the functionals.  XCFun (the evaluator Serenity prefers for
-// LDA/PBE) is not available here; the values below are the published ones,
-// chosen by the stored E_xc of Serenity's He and water PBE test systems.
+// Constants of the functionals (the published VWN5, PW92 and PBE values).
+// The two PBE evaluators Serenity offers differ in the exchange mu: XCFun
+// (its default for LDA/PBE) takes mu = 0.066725 pi^2 / 3 from the PBE paper's
+// rounded beta, LibXC takes mu = beta pi^2 / 3 with the accurate beta.  Both
+// use A = 0.0310907 in PW92 and the accurate beta in the correlation.  The
+// full-SCF energies of Serenity's water/STO-3G PBE tests decide these values
+// (qcserenity/tests/test_tasks.py:50-76: 2.8e-8 for XCFun, 2.2e-10 for LibXC).
 struct XcParams {
   double vwn_A = 0.0310907, vwn_b = 3.72744, vwn_c = 12.9352, vwn_x0 = -0.10498;
-  double pw_A = 0.031091, pw_a1 = 0.21370, pw_b1 = 7.5957, pw_b2 = 3.5876, pw_b3 = 1.6382, pw_b4 = 0.49294;
-  double pbe_kappa = 0.804, pbe_mu = 0.2195149727645171, pbe_beta = 0.06672455060314922;
+  double pw_A = 0.0310907, pw_a1 = 0.21370, pw_b1 = 7.5957, pw_b2 = 3.5876, pw_b3 = 1.6382, pw_b4 = 0.49294;
+  double pbe_kappa = 0.804, pbe_mu = 0.066725 * M_PI * M_PI / 3.0, pbe_beta = 0.06672455060314922;
+  double pbe_mu_libxc = 0.2195149727645171;
 };
 
 SXC_HD D2 slater_x(D2 rho) {
@@ -99,11 +104,11 @@ SXC_HD D2 pw92_eps(D2 rs, const XcParams& p) {
   return -2.0 * p.pw_A * (1.0 + p.pw_a1 * rs) * dlog(1.0 + 1.0 / den);
 }
 
-SXC_HD D2 pbe_x(D2 rho, D2 sigma, const XcParams& p) {
+SXC_HD D2 pbe_x(D2 rho, D2 sigma, const XcParams& p, double mu) {
   // s^2 = sigma / (4 (3 pi^2)^(2/3) rho^(8/3))
   const double k = 4.0 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
   D2 s2 = sigma / (k * dpow(rho, 8.0 / 3.0));
-  D2 Fx = (1.0 + p.pbe_kappa) - p.pbe_kappa / (1.0 + (p.pbe_mu / p.pbe_kappa) * s2);
+  D2 Fx = (1.0 + p.pbe_kappa) - p.pbe_kappa / (1.0 + (mu / p.pbe_kappa) * s2);
   return slater_x(rho) * Fx;
 }
 
@@ -127,15 +132,15 @@ SXC_HD D2 tf_k(D2 rho) {
 }
 
 // Functional ids as in serenity_xc.h.
-constexpr int kNumFunctionals = 7;  // LDA, PBE, TF, SLATER, VWN5, PBEX, PBEC
-SXC_HD bool functional_is_gga(int f) { return f == 1 || f == 5 || f == 6; }
+constexpr int kNumFunctionals = 8;  // LDA, PBE, TF, SLATER, VWN5, PBEX, PBEC, PBE (LibXC constants)
+SXC_HD bool functional_is_gga(int f) { return f == 1 || f == 5 || f == 6 || f == 7; }
 
 SXC_HD D2 eval_functional(int f, D2 rho, D2 sigma, const XcParams& p) {
   switch (f) {
     case 0:
       return slater_x(rho) + vwn5_c(rho, p);
     case 1:
-      return pbe_x(rho, sigma, p) + pbe_c(rho, sigma, p);
+      return pbe_x(rho, sigma, p, p.pbe_mu) + pbe_c(rho, sigma, p);
     case 2:
       return tf_k(rho);
     case 3:
@@ -143,9 +148,11 @@ SXC_HD D2 eval_functional(int f, D2 rho, D2 sigma, const XcParams& p) {
     case 4:
       return vwn5_c(rho, p);
     case 5:
-      return pbe_x(rho, sigma, p);
+      return pbe_x(rho, sigma, p, p.pbe_mu);
     case 6:
       return pbe_c(rho, sigma, p);
+    case 7:
+      return pbe_x(rho, sigma, p, p.pbe_mu_libxc) + pbe_c(rho, sigma, p);
     default:
       return mk(0.0);
   }

@@ -265,6 +265,36 @@ void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr, const Lanes& L)
 
 }  // namespace
 
+namespace {
+// Shell list of the host integral entry points (flat arrays as in xc_basis_create).
+std::vector<Shell> host_shells(const char* who, int nshell, const int* shell_l, const int* shell_pure,
+                               const int* shell_nprim, const double* shell_center, const double* alpha,
+                               const double* coef, int lmax) {
+  require(nshell > 0 && shell_l && shell_pure && shell_nprim && shell_center && alpha && coef, XC_ERR_ARG,
+          std::string(who) + ": bad shell arguments");
+  std::vector<Shell> sh(nshell);
+  int k = 0;
+  for (int s = 0; s < nshell; ++s) {
+    sh[s].l = shell_l[s];
+    sh[s].pure = shell_pure[s] != 0;
+    require(sh[s].l >= 0 && sh[s].l <= lmax, XC_ERR_INPUT,
+            std::string(who) + ": angular momentum above " + std::to_string(lmax));
+    require(shell_nprim[s] > 0, XC_ERR_INPUT, std::string(who) + ": shell without primitives");
+    sh[s].center = {shell_center[3 * s], shell_center[3 * s + 1], shell_center[3 * s + 2]};
+    for (int q = 0; q < shell_nprim[s]; ++q, ++k) {
+      sh[s].alpha.push_back(alpha[k]);
+      sh[s].coef.push_back(coef[k]);
+    }
+  }
+  return sh;
+}
+int nao_of(const std::vector<Shell>& sh) {
+  int n = 0;
+  for (const auto& s : sh) n += s.nfunc();
+  return n;
+}
+}  // namespace
+
 extern "C" {
 
 const char* xc_last_error(void) { return g_last_error.c_str(); }
@@ -469,26 +499,67 @@ int xc_lebedev(int npoints, double* x, double* y, double* z, double* w) {
   });
 }
 
+
 int xc_overlap(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim, const double* shell_center,
                const double* alpha, const double* coef, double* S) {
   return guarded([&] {
-    require(nshell > 0 && shell_l && shell_pure && shell_nprim && shell_center && alpha && coef && S, XC_ERR_ARG,
-            "xc_overlap: bad arguments");
-    std::vector<Shell> sh(nshell);
-    int k = 0;
-    for (int s = 0; s < nshell; ++s) {
-      sh[s].l = shell_l[s];
-      sh[s].pure = shell_pure[s] != 0;
-      require(sh[s].l >= 0 && sh[s].l <= 6, XC_ERR_INPUT, "xc_overlap: angular momentum above 6");
-      sh[s].center = {shell_center[3 * s], shell_center[3 * s + 1], shell_center[3 * s + 2]};
-      for (int q = 0; q < shell_nprim[s]; ++q, ++k) {
-        sh[s].alpha.push_back(alpha[k]);
-        sh[s].coef.push_back(coef[k]);
-      }
-    }
+    require(S, XC_ERR_ARG, "xc_overlap: null output");
+    auto sh = host_shells("xc_overlap", nshell, shell_l, shell_pure, shell_nprim, shell_center, alpha, coef, 6);
     std::vector<double> out;
     overlap_matrix(sh, out);
     std::copy(out.begin(), out.end(), S);
+  });
+}
+
+int xc_boys(int m, double T, double* F) {
+  return guarded([&] {
+    require(F && m >= 0 && m <= 17 && T >= 0.0, XC_ERR_ARG, "xc_boys: bad arguments");
+    *F = boys_function(m, T);
+  });
+}
+
+int xc_one_electron(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
+                    const double* shell_center, const double* alpha, const double* coef, int natoms, const int* Z,
+                    const double* coords, double* T, double* V) {
+  return guarded([&] {
+    require(natoms >= 0 && (natoms == 0 || (Z && coords)) && T && V, XC_ERR_ARG, "xc_one_electron: bad arguments");
+    auto sh = host_shells("xc_one_electron", nshell, shell_l, shell_pure, shell_nprim, shell_center, alpha, coef, 4);
+    std::vector<int> z(Z, Z + natoms);
+    std::vector<double> x(coords, coords + 3 * natoms), t, v;
+    one_electron_matrices(sh, z, x, t, v);
+    std::copy(t.begin(), t.end(), T);
+    std::copy(v.begin(), v.end(), V);
+  });
+}
+
+int xc_eri_packed(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
+                  const double* shell_center, const double* alpha, const double* coef, double schwarz_threshold,
+                  double* eri) {
+  return guarded([&] {
+    require(eri && schwarz_threshold >= 0.0, XC_ERR_ARG, "xc_eri_packed: bad arguments");
+    auto sh = host_shells("xc_eri_packed", nshell, shell_l, shell_pure, shell_nprim, shell_center, alpha, coef, 4);
+    const int64_t n = nao_of(sh), np = n * (n + 1) / 2;
+    require(np * np <= (int64_t)1 << 31, XC_ERR_INPUT, "xc_eri_packed: more than 2^31 integrals; use density fitting");
+    std::vector<double> out;
+    eri_packed(sh, schwarz_threshold, out);
+    std::copy(out.begin(), out.end(), eri);
+  });
+}
+
+int xc_ri_j_integrals(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
+                      const double* shell_center, const double* alpha, const double* coef, int naux_shell,
+                      const int* aux_l, const int* aux_pure, const int* aux_nprim, const double* aux_center,
+                      const double* aux_alpha, const double* aux_coef, double schwarz_threshold, double* B3,
+                      double* M) {
+  return guarded([&] {
+    require(B3 && M && schwarz_threshold >= 0.0, XC_ERR_ARG, "xc_ri_j_integrals: bad arguments");
+    auto sh = host_shells("xc_ri_j_integrals", nshell, shell_l, shell_pure, shell_nprim, shell_center, alpha, coef, 4);
+    auto ax = host_shells("xc_ri_j_integrals (auxiliary)", naux_shell, aux_l, aux_pure, aux_nprim, aux_center,
+                          aux_alpha, aux_coef, 4);
+    std::vector<double> b, m;
+    ri_j_integrals(sh, ax, schwarz_threshold, b, m);
+    std::copy(b.begin(), b.end(), B3);
+    std::copy(m.begin(), m.end(), M);
   });
 }
 

@@ -1,0 +1,311 @@
+"""Restricted Kohn-Sham SCF around the XC Fock build.
+
+Mirrors the callers of FuncPotential in Serenity:
+
+* ``ScfIntegrals`` -- the one-electron Hamiltonian h = T + V, the overlap S
+  and the Coulomb matrix J (exact four-centre integrals or density fitting,
+  Serenity's default ``densFitJ = RI`` with the RI_J_WEIGEND auxiliary basis,
+  Settings.h:128-138; RI_J_IntegralController.cpp:41-121).
+* ``perform`` -- ``Scf<RESTRICTED>::perform`` (src/scf/Scf.cpp:38-148): build
+  F = h + J[P] + V_xc[P] from the current density, extrapolate it by DIIS,
+  diagonalise in the canonically orthogonalised basis, occupy the lowest
+  orbitals, repeat until the energy, the density and the DIIS error are
+  converged.
+* ``ScfTask.run`` -- ``ScfTask<RESTRICTED>::run`` (src/tasks/ScfTask.cpp:368-)
+  with Serenity's grid protocol: the SCF iterations use the SMALL grid
+  (accuracy 2, ScfTask.cpp:358-359) and the final energy is evaluated once on
+  the DEFAULT grid (accuracy 4) from the converged density
+  (finalDFTEnergyEvaluation, ScfTask.cpp:127-153).
+
+Precision modes of the XC build inside the SCF: "FP64" runs every build on
+FP64 matrix cores.  "MIXED" iterates with FP32-MFMA contractions until the
+FP32-level convergence criteria hold, then restarts the DIIS and finishes with
+FP64 builds from that density (iterative refinement); the energies before and
+after the FP64 finish are both reported.
+
+The linear algebra (eigh, DIIS, the density-fitted J contractions) is numpy on
+the host: at the sizes the SCF targets it is small beside the XC builds.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib
+from . import xc
+from .xc import check
+
+_d = lambda a: a.ctypes.data_as(_lib.dp)
+_i = lambda a: a.ctypes.data_as(_lib.ip)
+
+
+def nuclear_repulsion(Z, coords_bohr):
+    Z = np.asarray(Z, dtype=np.float64)
+    X = np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
+    e = 0.0
+    for a in range(len(Z)):
+        for b in range(a):
+            e += Z[a] * Z[b] / np.linalg.norm(X[a] - X[b])
+    return e
+
+
+def one_electron(basis, Z, coords_bohr):
+    """(T, V): kinetic energy and nuclear attraction, n x n."""
+    n = basis.nao
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    X = np.ascontiguousarray(np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3))
+    T = np.zeros((n, n), order="F")
+    V = np.zeros((n, n), order="F")
+    check(_lib.lib().xc_one_electron(*basis.as_args(), len(Z), _i(Z), _d(X), _d(T.reshape(-1, order="F")),
+                                     _d(V.reshape(-1, order="F"))))
+    return T, V
+
+
+def eri_packed(basis, schwarz=1e-14):
+    """(mu nu|la si) over packed AO pairs pair(mu, nu) = mu (mu + 1) / 2 + nu, mu >= nu."""
+    npair = basis.nao * (basis.nao + 1) // 2
+    E = np.zeros((npair, npair))
+    check(_lib.lib().xc_eri_packed(*basis.as_args(), C.c_double(schwarz), _d(E)))
+    return E
+
+
+def ri_j_integrals(basis, aux, schwarz=1e-14):
+    """(B3, M): (P|mu nu) as naux x npair and the metric (P|Q)."""
+    npair = basis.nao * (basis.nao + 1) // 2
+    B3 = np.zeros((aux.nao, npair))
+    M = np.zeros((aux.nao, aux.nao))
+    check(_lib.lib().xc_ri_j_integrals(*basis.as_args(), *aux.as_args(), C.c_double(schwarz), _d(B3), _d(M)))
+    return B3, M
+
+
+def pack_density(P):
+    """Packed pair vector of a symmetric matrix with the off-diagonal pairs doubled."""
+    n = P.shape[0]
+    r, c = np.tril_indices(n)
+    v = P[r, c].copy()
+    v[r != c] *= 2.0
+    return v
+
+
+def unpack_symmetric(v, n):
+    out = np.zeros((n, n))
+    r, c = np.tril_indices(n)
+    out[r, c] = v
+    out[c, r] = v
+    return out
+
+
+class ScfIntegrals:
+    """S, h, E_nuc and the Coulomb matrix of one system (restricted).
+
+    ``aux`` = None: exact four-centre J (Serenity's densFitJ NONE);
+    otherwise density-fitted J over the auxiliary basis (densFitJ RI)."""
+
+    def __init__(self, basis, Z, coords_bohr, aux=None, schwarz=1e-14):
+        t0 = time.perf_counter()
+        self.basis, self.aux = basis, aux
+        self.n = basis.nao
+        self.S = basis.overlap()
+        self.T, self.V = one_electron(basis, Z, coords_bohr)
+        self.h = self.T + self.V
+        self.e_nuc = nuclear_repulsion(Z, coords_bohr)
+        if aux is None:
+            self.eri = eri_packed(basis, schwarz)
+            self.D = None
+        else:
+            B3, M = ri_j_integrals(basis, aux, schwarz)
+            Lm = np.linalg.cholesky(M)
+            # D = L^-1 B3, so that J = D^T (D p) = B3^T M^-1 B3 p
+            self.D = np.linalg.solve(Lm, B3)
+            self.eri = None
+        self.setup_s = time.perf_counter() - t0
+
+    def coulomb(self, P):
+        p = pack_density(P)
+        if self.D is not None:
+            j = self.D.T @ (self.D @ p)
+        else:
+            j = self.eri @ p
+        return unpack_symmetric(j, self.n)
+
+
+def canonical_orthogonalization(S, thr=1e-7):
+    """X with X^T S X = 1 (eigenvectors of S with eigenvalue above thr; canOrthThreshold)."""
+    e, U = np.linalg.eigh(S)
+    keep = e > thr
+    return U[:, keep] / np.sqrt(e[keep])
+
+
+class Diis:
+    """Pulay DIIS on Fock matrices with the orthonormal-basis commutator error."""
+
+    def __init__(self, max_store=10):
+        self.max_store = max_store
+        self.F, self.E = [], []
+
+    def reset(self):
+        self.F, self.E = [], []
+
+    def extrapolate(self, F, err):
+        self.F.append(F)
+        self.E.append(err)
+        if len(self.F) > self.max_store:
+            self.F.pop(0)
+            self.E.pop(0)
+        m = len(self.F)
+        if m < 2:
+            return F
+        B = -np.ones((m + 1, m + 1))
+        B[m, m] = 0.0
+        for i in range(m):
+            for j in range(i + 1):
+                B[i, j] = B[j, i] = float(np.vdot(self.E[i], self.E[j]))
+        rhs = np.zeros(m + 1)
+        rhs[m] = -1.0
+        try:
+            c = np.linalg.solve(B, rhs)[:m]
+        except np.linalg.LinAlgError:
+            return F
+        return sum(ci * Fi for ci, Fi in zip(c, self.F))
+
+
+def perform(ints, nocc, xc_build, P0=None, max_cycles=100, energy_thr=1e-10, rmsd_thr=1e-9, diis_thr=1e-8,
+            diis_store=10, log=None):
+    """Scf<RESTRICTED>::perform.  xc_build(P) -> (E_xc, V_xc).  Returns a dict with the energy of the
+    last density, P, orbitals, orbital energies, cycles and the energy history."""
+    say = log or (lambda *a: None)
+    X = canonical_orthogonalization(ints.S)
+    S, h = ints.S, ints.h
+    if P0 is None:
+        e, Cp = np.linalg.eigh(X.T @ h @ X)
+        Co = X @ Cp[:, :nocc]
+        P = 2.0 * Co @ Co.T
+    else:
+        P = np.array(P0, dtype=np.float64)
+    diis = Diis(diis_store)
+    e_old = None
+    hist = []
+    t_xc = t_j = 0.0
+    for cycle in range(1, max_cycles + 1):
+        t0 = time.perf_counter()
+        J = ints.coulomb(P)
+        t1 = time.perf_counter()
+        e_xc, Vxc = xc_build(P)
+        t2 = time.perf_counter()
+        t_j += t1 - t0
+        t_xc += t2 - t1
+        F = h + J + Vxc
+        energy = ints.e_nuc + float(np.vdot(P, h)) + 0.5 * float(np.vdot(P, J)) + e_xc
+        FPS = F @ P @ S
+        err = X.T @ (FPS - FPS.T) @ X
+        err_norm = float(np.sqrt(np.mean(err * err)))
+        Fx = diis.extrapolate(F, err)
+        eps, Cp = np.linalg.eigh(X.T @ Fx @ X)
+        Cmo = X @ Cp
+        Co = Cmo[:, :nocc]
+        P_new = 2.0 * Co @ Co.T
+        rmsd = float(np.sqrt(np.mean((P_new - P) ** 2)))
+        de = None if e_old is None else energy - e_old
+        hist.append(dict(cycle=cycle, energy=energy, dE=de, rmsd=rmsd, diis_error=err_norm, e_xc=e_xc))
+        say("cycle %3d  E = %.12f  dE = %s  rmsd = %.2e  diis = %.2e" %
+            (cycle, energy, "      -     " if de is None else "%.2e" % de, rmsd, err_norm))
+        if de is not None and abs(de) < energy_thr and rmsd < rmsd_thr and err_norm < diis_thr:
+            return dict(energy=energy, P=P, C=Cmo, eps=eps, cycles=cycle, converged=True, history=hist,
+                        e_xc=e_xc, t_xc=t_xc, t_j=t_j)
+        e_old = energy
+        P = P_new
+    return dict(energy=energy, P=P, C=Cmo, eps=eps, cycles=max_cycles, converged=False, history=hist, e_xc=e_xc,
+                t_xc=t_xc, t_j=t_j)
+
+
+def total_energy(ints, P, e_xc):
+    """E = E_nuc + tr(P h) + 1/2 tr(P J[P]) + E_xc[P]."""
+    J = ints.coulomb(P)
+    return ints.e_nuc + float(np.vdot(P, ints.h)) + 0.5 * float(np.vdot(P, J)) + e_xc
+
+
+# FP32-level convergence of the mixed-precision iterations (the FP32 contractions leave
+# about 1e-7 relative noise in F, below which the DIIS error does not fall).
+MIXED_THRESHOLDS = dict(energy_thr=1e-6, rmsd_thr=1e-5, diis_thr=1e-4)
+
+
+def gpu_xc_build(pot):
+    def build(P):
+        F = pot.getMatrix(P)
+        return pot.getEnergy(), F
+    return build
+
+
+class ScfTask:
+    """ScfTask<RESTRICTED>::run for a KS-DFT system on the GPU XC build.
+
+    Grids: SCF iterations on the SMALL grid (accuracy ``small_accuracy``), the final
+    energy on the DEFAULT grid (accuracy ``accuracy``) from the converged density,
+    as Serenity does (ScfTask.cpp:127-153, 358-359)."""
+
+    def __init__(self, ctx, Z, coords_bohr, basis="DEF2-SVP", functional="LDA", ri=True, aux="RI_J_WEIGEND",
+                 accuracy=4, small_accuracy=2, spherical=True, schwarz=1e-14, fp32_tiles=True, log=None):
+        self.ctx = ctx
+        self.Z = np.ascontiguousarray(Z, dtype=np.int32)
+        self.X = np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
+        self.functional = functional
+        self.say = log or (lambda *a: None)
+        t0 = time.perf_counter()
+        self.basis = xc.BasisController.from_file(basis, self.Z, self.X, spherical=spherical)
+        self.auxbasis = xc.BasisController.from_file(aux, self.Z, self.X, spherical=spherical) if ri else None
+        self.ints = ScfIntegrals(self.basis, self.Z, self.X, self.auxbasis, schwarz)
+        self.nocc = int(self.Z.sum()) // 2
+        self.grid_small = xc.molecular_grid(self.Z, self.X, accuracy=small_accuracy, ctx=ctx)
+        self.grid_default = (self.grid_small if accuracy == small_accuracy
+                             else xc.molecular_grid(self.Z, self.X, accuracy=accuracy, ctx=ctx))
+        self.setup_s = time.perf_counter() - t0
+        self.fp32_tiles = fp32_tiles
+        self._bf = {}
+
+    def _potential(self, which, precision):
+        """FuncPotential on the small or default grid; one controller per grid serves both precisions
+        (FP64 tiles plus, with fp32_tiles, their FP32 copy)."""
+        grid = self.grid_small if which == "small" else self.grid_default
+        if which not in self._bf:
+            self._bf[which] = xc.BasisFunctionOnGridController(self.ctx, self.basis, grid, 128, 1e-9, 1,
+                                                               keep_fp32=self.fp32_tiles)
+        return xc.FuncPotential(self._bf[which], self.functional, precision)
+
+    def final_energy(self, P, precision="FP64"):
+        """finalDFTEnergyEvaluation: E_xc on the default grid from the converged P."""
+        pot = self._potential("default", precision)
+        pot.getMatrix(P)
+        return total_energy(self.ints, P, pot.getEnergy())
+
+    def run(self, mode="FP64", max_cycles=100, P0=None):
+        """mode "FP64" or "MIXED"; returns energies, cycle counts and timings."""
+        t0 = time.perf_counter()
+        out = dict(mode=mode, nao=self.basis.nao, naux=None if self.auxbasis is None else self.auxbasis.nao,
+                   npts_small=self.grid_small.getNGridPoints(), npts_default=self.grid_default.getNGridPoints())
+        if mode == "FP64":
+            r = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "FP64")), P0=P0,
+                        max_cycles=max_cycles, log=self.say)
+            out.update(cycles=r["cycles"], converged=r["converged"], energy_small_grid=r["energy"])
+            out["energy"] = self.final_energy(r["P"], "FP64")
+        elif mode == "MIXED":
+            r32 = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "FP32")), P0=P0,
+                          max_cycles=max_cycles, log=self.say, **MIXED_THRESHOLDS)
+            out["cycles_fp32"] = r32["cycles"]
+            out["energy_before_fp64_finish"] = self.final_energy(r32["P"], "FP32")
+            r = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "FP64")), P0=r32["P"],
+                        max_cycles=max_cycles, log=self.say)
+            out.update(cycles_fp64_finish=r["cycles"], converged=r32["converged"] and r["converged"],
+                       energy_small_grid=r["energy"])
+            out["cycles"] = r32["cycles"] + r["cycles"]
+            out["energy"] = self.final_energy(r["P"], "FP64")
+        else:
+            raise xc.XCError(1, "ScfTask: unknown mode " + str(mode))
+        out["P"] = r["P"]
+        out["t_xc_s"] = round(r["t_xc"], 3)
+        out["wall_s"] = round(time.perf_counter() - t0, 3)
+        return out
+
+    def close(self):
+        for bf in self._bf.values():
+            bf._release()
+        self._bf = {}

@@ -1,0 +1,99 @@
+"""Full-SCF energies of the SCF driver (serenity_amd.scf) against the reference's own tests,
+with the oracle's XC build standing in for the GPU build (the GPU form is in test_gpu_scf.py).
+
+Pins (total energies of Serenity's ScfTask: SMALL-grid iterations, final DEFAULT-grid
+energy from the converged density, ScfTask.cpp:127-153, 358-359):
+
+* H2/STO-6G LDA without density fitting, src/scf/Scf_test.cpp:169-190: -1.1298876147476988
+  (tolerance 1e-6 as there).
+* H2/STO-6G LDA with RI-J, Scf_test.cpp:144-167: -1.1299388663978864.  The value matches
+  a Cartesian RI_J_WEIGEND basis (H carries a d fitting function): the test geometry's
+  atoms are shared objects that keep the first auxiliary basis loaded on them, and the
+  stored TestSystem_H2_MINBAS settings (data/testresources) have MAKESPHERICALBASIS false.
+  A spherical auxiliary basis lands 6.9e-7 lower.
+* water/STO-3G PBE with RI-J, qcserenity/tests/test_tasks.py:50-76: -75.2218044178 with
+  LibXC's PBE and -75.22180876683022 with XCFun's (the unrestricted run of a closed shell;
+  places=7 there, i.e. 5e-8).
+"""
+import numpy as np
+import pytest
+
+from serenity_amd import scf, xc
+from oracle import oracle as O
+
+from test_integrals import water_geometry
+
+
+def oracle_xc(basis, grid, functional):
+    def build(P):
+        F, E, _, _ = O.fock_build(basis, grid.points, grid.weights, P, functional)
+        return E, F
+    return build
+
+
+def scf_energy(Z, X, basis, functional, aux=None, aux_spherical=True, spherical=True):
+    b = xc.BasisController.from_file(basis, Z, X, spherical=spherical)
+    a = None if aux is None else xc.BasisController.from_file(aux, Z, X, spherical=aux_spherical)
+    ints = scf.ScfIntegrals(b, Z, X, a)
+    small = xc.molecular_grid(Z, X, accuracy=2)
+    default = xc.molecular_grid(Z, X, accuracy=4)
+    r = scf.perform(ints, int(np.sum(Z)) // 2, oracle_xc(b, small, functional))
+    assert r["converged"]
+    _, e_xc, _, _ = O.fock_build(b, default.points, default.weights, r["P"], functional)
+    return scf.total_energy(ints, r["P"], e_xc), r
+
+
+def h2():
+    """TestSystem_H2_MINBAS geometry: H at z = -0.7 and +0.7 bohr (SystemController__TEST_SUPPLY.cpp:1744-1745)."""
+    return np.array([1, 1]), np.array([[0.0, 0.0, -0.7], [0.0, 0.0, 0.7]])
+
+
+def test_h2_sto6g_lda_no_ri():
+    Z, X = h2()
+    e, r = scf_energy(Z, X, "STO-6G", "LDA")
+    assert abs(e - (-1.1298876147476988)) < 1e-7
+    assert r["cycles"] < 20
+
+
+def test_h2_sto6g_lda_ri():
+    Z, X = h2()
+    e, _ = scf_energy(Z, X, "STO-6G", "LDA", aux="RI_J_WEIGEND", aux_spherical=False)
+    assert abs(e - (-1.1299388663978864)) < 1e-7
+    e_sph, _ = scf_energy(Z, X, "STO-6G", "LDA", aux="RI_J_WEIGEND", aux_spherical=True)
+    assert 5e-7 < (-1.1299388663978864) - e_sph < 9e-7
+
+
+@pytest.mark.parametrize("functional,ref", [("PBE", -75.22180876683022), ("PBE_LIBXC", -75.2218044178)])
+def test_water_sto3g_pbe(functional, ref):
+    Z, X = water_geometry()
+    e, r = scf_energy(Z, X, "STO-3G", functional, aux="RI_J_WEIGEND")
+    assert abs(e - ref) < 5e-8
+    # the SMALL-grid energy differs from the final DEFAULT-grid one: the protocol matters
+    assert abs(r["energy"] - e) > 1e-6
+
+
+def test_scf_converges_from_any_guess_to_the_same_energy():
+    Z, X = water_geometry()
+    b = xc.BasisController.from_file("STO-3G", Z, X)
+    ints = scf.ScfIntegrals(b, Z, X, xc.BasisController.from_file("RI_J_WEIGEND", Z, X))
+    g = xc.molecular_grid(Z, X, accuracy=2)
+    build = oracle_xc(b, g, "LDA")
+    r0 = scf.perform(ints, 5, build)
+    rng = np.random.default_rng(3)
+    Q, _ = np.linalg.qr(rng.standard_normal((b.nao, 5)))
+    Xo = scf.canonical_orthogonalization(ints.S)
+    Co = Xo @ Q
+    r1 = scf.perform(ints, 5, build, P0=2.0 * Co @ Co.T)
+    assert r0["converged"] and r1["converged"]
+    assert abs(r0["energy"] - r1["energy"]) < 1e-9
+    # idempotency and electron count of the converged density
+    P, S = r0["P"], ints.S
+    assert abs(np.trace(P @ S) - 10.0) < 1e-10
+    assert np.abs(P @ S @ P - 2.0 * P).max() < 1e-10
+
+
+def test_nuclear_repulsion():
+    Z, X = water_geometry()
+    e = scf.nuclear_repulsion(Z, X)
+    d = lambda a, b: np.linalg.norm(X[a] - X[b])
+    assert abs(e - (1 / d(0, 1) + 8 / d(0, 2) + 8 / d(1, 2))) < 1e-13
