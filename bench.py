@@ -151,7 +151,7 @@ def fde_sweep(ctx, steps, warmup, check_oracle):
 
 def scf_leg(ctx, check_oracle):
     """Full SCF of (H2O)8/def2-SVP/LDA (RI-J): FP64, mixed with FP64 finish, and the oracle-driven SCF."""
-    from serenity_amd import scf as SCF
+    from serenity_amd import scf as SCF, workloads as W
     Z, X = W.geometry(W.CONFIGS[1]["system"])
     t0 = time.perf_counter()
     task = SCF.ScfTask(ctx, Z, X, basis="DEF2-SVP", functional="LDA")

@@ -50,7 +50,7 @@ struct Matrix {
   void setZero() { std::fill(data.begin(), data.end(), 0.0); }
 };
 
-enum class Functional { LDA = XC_FUNC_LDA, PBE = XC_FUNC_PBE, TF = XC_FUNC_TF };
+enum class Functional { LDA = XC_FUNC_LDA, PBE = XC_FUNC_PBE, TF = XC_FUNC_TF, PBE_LIBXC = XC_FUNC_PBE_LIBXC };
 enum class Precision { FP64 = XC_PREC_FP64, FP32 = XC_PREC_FP32 };
 
 /* One HIP device and its stream. */

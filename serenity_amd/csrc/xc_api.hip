@@ -913,7 +913,7 @@ int xc_functional(xc_ctx* ctx, int functional, int64_t npts, int block_size, con
                   const double* weights, double* eps, double* v, double* g, double* energy) {
   return guarded([&] {
     require(ctx && rho && weights && npts > 0, XC_ERR_ARG, "xc_functional: bad arguments");
-    require(functional >= 0 && functional <= 6, XC_ERR_ARG, "xc_functional: unknown functional");
+    require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "xc_functional: unknown functional");
     require(block_size >= 1 && block_size <= kTile, XC_ERR_ARG, "xc_functional: block size must be in [1,128]");
     const bool gga = functional_is_gga(functional);
     require(!gga || grad, XC_ERR_ARG, "xc_functional: GGA functional needs the density gradient");
@@ -978,7 +978,7 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   xc_ctx* ctx = bf->ctx;
   hipStream_t s = L.main;
   require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
-  require(functional >= 0 && functional <= 6, XC_ERR_ARG, "unknown functional");
+  require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
   require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
   const bool gga = functional_is_gga(functional);
   require(!gga || bf->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
@@ -1023,7 +1023,7 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
 static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int functional, int precision, double thr,
                          const double* dP_act, const double* const* dP_env, double* dF) {
   xc_ctx* ctx = act->ctx;
-  require(functional >= 0 && functional <= 6, XC_ERR_ARG, "unknown functional");
+  require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
   require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
   require(nenv >= 0 && (nenv == 0 || (env && dP_env)), XC_ERR_ARG, "non-additive build: null environment list");
   const bool gga = functional_is_gga(functional);
