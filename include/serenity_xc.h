@@ -158,9 +158,11 @@ int xc_grid_destroy(xc_grid* grid);
 
 /* BasisFunctionOnGridController(basis, grid, maxBlockSize, radialThreshold,
  * highestDerivative) (BasisFunctionOnGridController.h:105-107).  Runs the
- * block screening and evaluates the AO values (and gradients if
- * highest_derivative >= 1) of every block into resident HBM tiles.
- * highest_derivative must be 0 or 1 (the Fock build needs at most 1).
+ * block screening and evaluates the AO values (gradients if
+ * highest_derivative >= 1, second derivatives if 2) of every block into
+ * resident HBM tiles.  highest_derivative is 0, 1 or 2 (the Fock build needs
+ * at most 1; the GGA nuclear gradient needs 2, as getGeomGradients sets it,
+ * FuncPotential.cpp:123-128).
  * keep_fp32: 0 = FP64 tiles only, 1 = FP64 and FP32 tiles (XC_PREC_FP32 builds),
  * 2 = FP32 tiles only (XC_PREC_FP32 builds; a third of the footprint, for
  * serving many resident systems). */
@@ -178,6 +180,11 @@ int xc_bfogc_stats(const xc_bfogc* bfogc, int64_t* sum_nb, double* sum_B_nb2, in
  * negligible flags (1 = screened out) to the host. */
 int xc_bfogc_block_data(const xc_bfogc* bfogc, int block, double* values, double* grad_x, double* grad_y,
                         double* grad_z, int* negligible, int* block_size);
+/* Second derivatives of the block's AOs (blockOnGridData->secondDerivativeValues,
+ * BasisFunctionOnGridController.h:73-79): hess is 6 x n x B (planes xx, xy, xz,
+ * yy, yz, zz; each n x B with B the block size), zero for negligible AOs.
+ * Needs highest_derivative 2. */
+int xc_bfogc_block_hessian(const xc_bfogc* bfogc, int block, double* hess);
 
 /* ---- the path ----------------------------------------------------------- */
 /* DensityOnGridCalculator::calcDensityAndGradientOnGrid(P, rho, grad)
@@ -213,6 +220,16 @@ int xc_fock_build(xc_bfogc* bfogc, int functional, int precision, double block_a
  * with xc_last_energy after xc_ctx_synchronize). */
 int xc_fock_build_device(xc_bfogc* bfogc, int functional, int precision, double block_ave_threshold,
                          const double* dP, double* dF, double* energy);
+
+/* FuncPotential::getGeomGradients (FuncPotential.cpp:113-239): the XC part of
+ * the nuclear gradient of the restricted density P (host, n x n column-major)
+ * at a fixed grid (no weight derivatives, as the reference), written to grad
+ * as natoms x 3 column-major (Eigen's nAtoms x 3 layout).  ao_atom[nu] is the
+ * atom of AO nu (BasisController::getAtomIndicesOfBasis).  The controller
+ * needs highest_derivative 1 (LDA) or 2 (GGA) and FP64 tiles; energy gets E_xc
+ * of P (may be NULL). */
+int xc_geom_gradient(xc_bfogc* bfogc, int functional, const double* P, int natoms, const int* ao_atom,
+                     double* grad, double* energy);
 int xc_last_energy(xc_ctx* ctx, double* energy);
 
 /* NAddFuncPotential::getMatrix + getEnergy, restricted (src/potentials/

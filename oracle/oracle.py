@@ -163,5 +163,21 @@ def nadd_fock_build(active_basis, env_bases, points, weights, P_active, P_envs, 
     return F, E
 
 
+def xc_gradient(basis, points, weights, P, name, ao_atom, natoms, block_size=128, thr=1e-9, params=None):
+    """FuncPotential::getGeomGradients: (natoms x 3 gradient, E_xc)."""
+    pts = _pts(points)
+    N = len(pts)
+    Pf = np.asfortranarray(P, dtype=np.float64)
+    G = np.zeros((natoms, 3), order="F")
+    E = C.c_double()
+    prm = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    atom = np.ascontiguousarray(ao_atom, dtype=np.int32)
+    lib().oracle_xc_gradient(*_basis_args(basis), _d(pts), _d(np.ascontiguousarray(weights, dtype=np.float64)),
+                             C.c_int64(N), C.c_int(block_size), C.c_double(thr), C.c_int(FUNCTIONALS[name]),
+                             _d(prm), Pf.ctypes.data_as(dp), C.c_int(natoms), _i(atom), G.ctypes.data_as(dp),
+                             C.byref(E))
+    return np.array(G), E.value
+
+
 def max_threads():
     return lib().oracle_max_threads()

@@ -16,6 +16,7 @@
 //                        (src/data/grid/ScalarOperatorToMatrixAdder.cpp:51-113, 179-303)
 //   oracle_fock_build    FuncPotential::getMatrix / getEnergy (src/potentials/FuncPotential.cpp:66-111),
 //                        with the reference's three AO evaluations per build
+//   oracle_xc_gradient   FuncPotential::getGeomGradients (src/potentials/FuncPotential.cpp:113-239)
 //
 // Parity pins (tests/test_oracle.py): the reference's Mathematica known-answer
 // values for AO values/gradients/Hessians, density/gradient/Hessian and the
@@ -74,16 +75,20 @@ double dfact(int k) {
   return r;
 }
 
-// Real solid harmonics S_lm (Racah normalisation) and their gradients by the
-// HJO recurrences.  out[(l)][m+l] for one point; value and d/dx,d/dy,d/dz.
+// Real solid harmonics S_lm (Racah normalisation) and their first and second
+// derivatives by the HJO recurrences (each recurrence differentiated once and
+// twice).  out[(l)][m+l] for one point.
 struct Harm {
   double v[8][15], x[8][15], y[8][15], z[8][15];
+  double xx[8][15], xy[8][15], xz[8][15], yy[8][15], yz[8][15], zz[8][15];
 };
 
 void solid_harmonics(int lmax, double X, double Y, double Z, Harm& H) {
   const double r2 = X * X + Y * Y + Z * Z;
   for (int l = 0; l <= lmax; ++l)
-    for (int k = 0; k < 2 * l + 1; ++k) H.v[l][k] = H.x[l][k] = H.y[l][k] = H.z[l][k] = 0.0;
+    for (int k = 0; k < 2 * l + 1; ++k)
+      H.v[l][k] = H.x[l][k] = H.y[l][k] = H.z[l][k] = H.xx[l][k] = H.xy[l][k] = H.xz[l][k] = H.yy[l][k] =
+          H.yz[l][k] = H.zz[l][k] = 0.0;
   H.v[0][0] = 1.0;
   for (int l = 0; l < lmax; ++l) {
     // diagonal terms m = +-(l+1)
@@ -99,6 +104,23 @@ void solid_harmonics(int lmax, double X, double Y, double Z, Harm& H) {
     H.x[l + 1][M0] = c * (Y * H.x[l][2 * l] + f * (slm + X * H.x[l][0]));
     H.y[l + 1][M0] = c * (sll + Y * H.y[l][2 * l] + f * X * H.y[l][0]);
     H.z[l + 1][M0] = c * (Y * H.z[l][2 * l] + f * X * H.z[l][0]);
+    {
+      // second derivatives of S_{l+1,l+1} = c (X s - f Y t) and S_{l+1,-l-1} = c (Y s + f X t),
+      // s = S_{l,l}, t = S_{l,-l}
+      const int a = 2 * l, b = 0;
+      H.xx[l + 1][P] = c * (2.0 * H.x[l][a] + X * H.xx[l][a] - f * Y * H.xx[l][b]);
+      H.xy[l + 1][P] = c * (H.y[l][a] + X * H.xy[l][a] - f * (H.x[l][b] + Y * H.xy[l][b]));
+      H.xz[l + 1][P] = c * (H.z[l][a] + X * H.xz[l][a] - f * Y * H.xz[l][b]);
+      H.yy[l + 1][P] = c * (X * H.yy[l][a] - f * (2.0 * H.y[l][b] + Y * H.yy[l][b]));
+      H.yz[l + 1][P] = c * (X * H.yz[l][a] - f * (H.z[l][b] + Y * H.yz[l][b]));
+      H.zz[l + 1][P] = c * (X * H.zz[l][a] - f * Y * H.zz[l][b]);
+      H.xx[l + 1][M0] = c * (Y * H.xx[l][a] + f * (2.0 * H.x[l][b] + X * H.xx[l][b]));
+      H.xy[l + 1][M0] = c * (H.x[l][a] + Y * H.xy[l][a] + f * (H.y[l][b] + X * H.xy[l][b]));
+      H.xz[l + 1][M0] = c * (Y * H.xz[l][a] + f * (H.z[l][b] + X * H.xz[l][b]));
+      H.yy[l + 1][M0] = c * (2.0 * H.y[l][a] + Y * H.yy[l][a] + f * X * H.yy[l][b]);
+      H.yz[l + 1][M0] = c * (H.z[l][a] + Y * H.yz[l][a] + f * X * H.yz[l][b]);
+      H.zz[l + 1][M0] = c * (Y * H.zz[l][a] + f * X * H.zz[l][b]);
+    }
     // |m| <= l by the z recurrence
     for (int m = -l; m <= l; ++m) {
       const double a = (2.0 * l + 1.0);
@@ -107,17 +129,34 @@ void solid_harmonics(int lmax, double X, double Y, double Z, Harm& H) {
       const int k1 = m + l + 1;  // index in l+1
       const int k0 = m + l;      // index in l
       double sm1 = 0.0, sm1x = 0.0, sm1y = 0.0, sm1z = 0.0;
+      double w2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // xx xy xz yy yz zz of S_{l-1,m}
       if (b != 0.0) {
         const int km = m + l - 1;
         sm1 = H.v[l - 1][km];
         sm1x = H.x[l - 1][km];
         sm1y = H.y[l - 1][km];
         sm1z = H.z[l - 1][km];
+        w2[0] = H.xx[l - 1][km];
+        w2[1] = H.xy[l - 1][km];
+        w2[2] = H.xz[l - 1][km];
+        w2[3] = H.yy[l - 1][km];
+        w2[4] = H.yz[l - 1][km];
+        w2[5] = H.zz[l - 1][km];
       }
       H.v[l + 1][k1] = (a * Z * H.v[l][k0] - b * r2 * sm1) / d;
       H.x[l + 1][k1] = (a * Z * H.x[l][k0] - b * (2.0 * X * sm1 + r2 * sm1x)) / d;
       H.y[l + 1][k1] = (a * Z * H.y[l][k0] - b * (2.0 * Y * sm1 + r2 * sm1y)) / d;
       H.z[l + 1][k1] = (a * (H.v[l][k0] + Z * H.z[l][k0]) - b * (2.0 * Z * sm1 + r2 * sm1z)) / d;
+      // S = (a Z u - b r^2 w) / d:  S_ij = (a (dz_i u_j + dz_j u_i + Z u_ij)
+      //                                   - b (2 d_ij w + 2 x_i w_j + 2 x_j w_i + r^2 w_ij)) / d
+      const double u = H.v[l][k0], ux = H.x[l][k0], uy = H.y[l][k0], uz = H.z[l][k0];
+      H.xx[l + 1][k1] = (a * Z * H.xx[l][k0] - b * (2.0 * sm1 + 4.0 * X * sm1x + r2 * w2[0])) / d;
+      H.xy[l + 1][k1] = (a * Z * H.xy[l][k0] - b * (2.0 * X * sm1y + 2.0 * Y * sm1x + r2 * w2[1])) / d;
+      H.xz[l + 1][k1] = (a * (ux + Z * H.xz[l][k0]) - b * (2.0 * X * sm1z + 2.0 * Z * sm1x + r2 * w2[2])) / d;
+      H.yy[l + 1][k1] = (a * Z * H.yy[l][k0] - b * (2.0 * sm1 + 4.0 * Y * sm1y + r2 * w2[3])) / d;
+      H.yz[l + 1][k1] = (a * (uy + Z * H.yz[l][k0]) - b * (2.0 * Y * sm1z + 2.0 * Z * sm1y + r2 * w2[4])) / d;
+      H.zz[l + 1][k1] = (a * (2.0 * uz + Z * H.zz[l][k0]) - b * (2.0 * sm1 + 4.0 * Z * sm1z + r2 * w2[5])) / d;
+      (void)u;
     }
   }
 }
@@ -125,7 +164,7 @@ void solid_harmonics(int lmax, double X, double Y, double Z, Harm& H) {
 struct BlockAO {
   int B = 0, n = 0, deriv = 0;
   std::vector<double> val, gx, gy, gz;                // B x n column-major
-  std::vector<double> hxx, hxy, hxz, hyy, hyz, hzz;   // B x n (Cartesian shells only)
+  std::vector<double> hxx, hxy, hxz, hyy, hyz, hzz;   // B x n
   std::vector<int> neg;                               // n flags, 1 = negligible
 };
 
@@ -235,6 +274,16 @@ void ao_block(const Basis& bas, const double* pts, int B, double thr, int deriv,
             out.gx[o_] = R * H.x[l][m] + dR * X * H.v[l][m];
             out.gy[o_] = R * H.y[l][m] + dR * Y * H.v[l][m];
             out.gz[o_] = R * H.z[l][m] + dR * Z * H.v[l][m];
+          }
+          if (deriv >= 2) {
+            // d_i d_j (R S) = R S_ij + R' (x_i S_j + x_j S_i) + d_ij R' S + R'' x_i x_j S
+            const double S0 = H.v[l][m], Sx = H.x[l][m], Sy = H.y[l][m], Sz = H.z[l][m];
+            out.hxx[o_] = R * H.xx[l][m] + dR * (2.0 * X * Sx + S0) + ddR * X * X * S0;
+            out.hyy[o_] = R * H.yy[l][m] + dR * (2.0 * Y * Sy + S0) + ddR * Y * Y * S0;
+            out.hzz[o_] = R * H.zz[l][m] + dR * (2.0 * Z * Sz + S0) + ddR * Z * Z * S0;
+            out.hxy[o_] = R * H.xy[l][m] + dR * (X * Sy + Y * Sx) + ddR * X * Y * S0;
+            out.hxz[o_] = R * H.xz[l][m] + dR * (X * Sz + Z * Sx) + ddR * X * Z * S0;
+            out.hyz[o_] = R * H.yz[l][m] + dR * (Y * Sz + Z * Sy) + ddR * Y * Z * S0;
           }
         }
       }
@@ -622,6 +671,82 @@ int oracle_fock_build(int nshell, const int* l, const int* pure, const int* npri
                   gga ? g.data() : nullptr, F, 0, -1);
   if (rho_out) std::copy(rho.begin(), rho.end(), rho_out);
   if (grad_out && gga) std::copy(grad.begin(), grad.end(), grad_out);
+  return 0;
+}
+
+// FuncPotential::getGeomGradients (src/potentials/FuncPotential.cpp:113-239): the XC part of the
+// nuclear gradient at a fixed grid (no weight derivatives), natoms x 3 column-major as Eigen
+// stores it.  ao_atom[nu] is the atom of AO nu (BasisController::getAtomIndicesOfBasis).  For
+// every block and every pair of non-negligible AOs (mu, nu):
+//   grad[atom(nu)][c] -= 2 P_mu,nu sum_p w v phi_mu d_c phi_nu
+//   (GGA)             -= 2 P_mu,nu sum_p w sum_k g_k (phi_mu d_k d_c phi_nu + d_k phi_mu d_c phi_nu)
+int oracle_xc_gradient(int nshell, const int* l, const int* pure, const int* nprim, const double* center,
+                       const double* alpha, const double* coef, const double* pts, const double* w, int64_t N,
+                       int bs, double thr, int func, const double* params, const double* P, int natoms,
+                       const int* ao_atom, double* grad, double* energy) {
+  const bool gga = is_gga(func);
+  std::vector<double> rho(N), gr(gga ? 3 * N : 0), v(N), g(gga ? 3 * N : 0);
+  oracle_density(nshell, l, pure, nprim, center, alpha, coef, pts, N, bs, thr, P, rho.data(),
+                 gga ? gr.data() : nullptr, nullptr, nullptr);
+  oracle_functional(func, N, 128, rho.data(), gga ? gr.data() : nullptr, w, params, nullptr, v.data(),
+                    gga ? g.data() : nullptr, energy);
+  Basis bas(nshell, l, pure, nprim, center, alpha, coef);
+  const int n = bas.nao;
+  const int64_t nb = (N + bs - 1) / bs;
+  const int nthreads = omp_get_max_threads();
+  std::vector<std::vector<double>> Gt(nthreads, std::vector<double>((size_t)natoms * 3, 0.0));
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t b = 0; b < nb; ++b) {
+    auto& gp = Gt[omp_get_thread_num()];
+    const int64_t first = b * bs;
+    const int B = (int)std::min<int64_t>(bs, N - first);
+    BlockAO ao;
+    ao_block(bas, pts + 3 * first, B, thr, gga ? 2 : 1, ao);
+    std::vector<double> potMu(B), pre(B);
+    for (int mu = 0; mu < n; ++mu) {
+      if (ao.neg[mu]) continue;
+      const double* fm = &ao.val[(size_t)mu * B];
+      for (int p = 0; p < B; ++p) potMu[p] = w[first + p] * v[first + p] * fm[p];
+      for (int nu = 0; nu < n; ++nu) {
+        if (ao.neg[nu]) continue;
+        const int A = ao_atom[nu];
+        const double prefac = 2.0 * P[mu + (size_t)n * nu];
+        const size_t o = (size_t)nu * B;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int p = 0; p < B; ++p) {
+          sx += potMu[p] * ao.gx[o + p];
+          sy += potMu[p] * ao.gy[o + p];
+          sz += potMu[p] * ao.gz[o + p];
+        }
+        gp[A] -= prefac * sx;
+        gp[A + natoms] -= prefac * sy;
+        gp[A + 2 * natoms] -= prefac * sz;
+        if (gga) {
+          const size_t om = (size_t)mu * B;
+          double tx = 0.0, ty = 0.0, tz = 0.0;
+          for (int p = 0; p < B; ++p) {
+            const double c = prefac * w[first + p];
+            const double px = c * g[first + p], py = c * g[N + first + p], pz = c * g[2 * N + first + p];
+            const double f = fm[p];
+            const double mx = ao.gx[om + p], my = ao.gy[om + p], mz = ao.gz[om + p];
+            const double nx = ao.gx[o + p], ny = ao.gy[o + p], nz = ao.gz[o + p];
+            tx += px * (f * ao.hxx[o + p] + mx * nx) + py * (f * ao.hxy[o + p] + my * nx) +
+                  pz * (f * ao.hxz[o + p] + mz * nx);
+            ty += px * (f * ao.hxy[o + p] + mx * ny) + py * (f * ao.hyy[o + p] + my * ny) +
+                  pz * (f * ao.hyz[o + p] + mz * ny);
+            tz += px * (f * ao.hxz[o + p] + mx * nz) + py * (f * ao.hyz[o + p] + my * nz) +
+                  pz * (f * ao.hzz[o + p] + mz * nz);
+          }
+          gp[A] -= tx;
+          gp[A + natoms] -= ty;
+          gp[A + 2 * natoms] -= tz;
+        }
+      }
+    }
+  }
+  std::fill(grad, grad + (size_t)natoms * 3, 0.0);
+  for (int t = 0; t < nthreads; ++t)
+    for (size_t e = 0; e < (size_t)natoms * 3; ++e) grad[e] += Gt[t][e];
   return 0;
 }
 

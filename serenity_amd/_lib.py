@@ -81,6 +81,8 @@ def lib():
     _sig("xc_bfogc_info", C.c_int, vp, ip, C.POINTER(i64), ip, ip)
     _sig("xc_bfogc_stats", C.c_int, vp, C.POINTER(i64), dp, C.POINTER(i64))
     _sig("xc_bfogc_block_data", C.c_int, vp, C.c_int, dp, dp, dp, dp, ip, ip)
+    _sig("xc_bfogc_block_hessian", C.c_int, vp, C.c_int, dp)
+    _sig("xc_geom_gradient", C.c_int, vp, C.c_int, dp, C.c_int, ip, dp, dp)
     _sig("xc_density", C.c_int, vp, dp, dp, dp, ip)
     _sig("xc_functional", C.c_int, vp, C.c_int, i64, C.c_int, dp, dp, dp, dp, dp, dp, dp)
     _sig("xc_fock_add", C.c_int, vp, C.c_double, dp, dp, dp)

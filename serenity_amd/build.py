@@ -16,6 +16,7 @@ SOURCES = [
     "xc_api.hip",
     "kernels/xc_kernels.hip",
     "kernels/grid_kernels.hip",
+    "kernels/gradient_kernels.hip",
     "host/basis.cpp",
     "host/grid.cpp",
     "host/overlap.cpp",

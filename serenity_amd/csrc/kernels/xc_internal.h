@@ -5,7 +5,8 @@
 // blocks of at most 128 consecutive (Hilbert-ordered) points.  Block b keeps
 // the n_b AOs that survive Serenity's shell screening, padded to npad (a
 // multiple of 32) and stored as ncomp component planes
-//   tiles[tile_off + (c * npad + i) * 128 + p],  c = value, d/dx, d/dy, d/dz,
+//   tiles[tile_off + (c * npad + i) * 128 + p],  c = value, d/dx, d/dy, d/dz
+// (and, for highest derivative 2, d2/dxx, dxy, dxz, dyy, dyz, dzz),
 // i = compact AO index (ao_idx maps it to the global AO, -1 for padding),
 // p = point within the block (rows beyond the block's size are zero).  Point
 // is the fastest index so a wavefront reads 64 consecutive values.
@@ -96,6 +97,19 @@ template <typename T>
 void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,
                      const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
                      double block_ave_thr, double* U, int n, int64_t npts);
+
+// Work item of the gradient kernel: the block's AO columns [col0, col0 + 64).
+struct GradItem {
+  int block, col0;
+};
+constexpr int kGradCols = 64;
+// a[nu][c] += sum over the item's points of (G P)_p,nu d_c phi_nu + (phi P)_p,nu H^c_nu
+// (kernels/gradient_kernels.hip); tiles need 4 planes (LDA) or 10 (GGA).
+void launch_gradient(hipStream_t s, const GradItem* work, int nwork, const DevBlock* blocks, const int* ao_idx,
+                     const double* tiles, const double* P, int n, const double* vs, const double* gw, int gga,
+                     int64_t npts, double* acc);
+// grad[A + natoms c] = -2 sum_{nu on A} acc[nu][c].
+void launch_gradient_atoms(hipStream_t s, const double* acc, const int* ao_atom, int n, int natoms, double* grad);
 
 // Molecular-grid partition weights of ncand candidate points (kernels/grid_kernels.hip).
 void launch_partition(hipStream_t s, int type, unsigned smoothing, int natoms, const double* xyz, const double* dist,

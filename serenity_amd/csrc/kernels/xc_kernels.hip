@@ -87,7 +87,7 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
     const DevShell sh = shells[s];
     const double dx = px - sh.cx, dy = py - sh.cy, dz = pz - sh.cz;
     const double r2 = dx * dx + dy * dy + dz * dz;
-    double R = 0.0, dR = 0.0;
+    double R = 0.0, dR = 0.0, ddR = 0.0;
     for (int q = 0; q < sh.nprim; ++q) {
       const double a = alpha[sh.prim_off + q];
       const double t = a * r2;
@@ -96,6 +96,7 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
         const double ce = coef[sh.prim_off + q] * e;
         R += ce;
         dR -= 2.0 * a * ce;
+        ddR += 4.0 * a * a * ce;
       }
     }
     const int64_t base = bk.tile_off + (int64_t)pos * kTile + p;
@@ -114,8 +115,10 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
       yp[e] = yp[e - 1] * dy;
       zp[e] = zp[e - 1] * dz;
     }
+    const bool hess = ncomp >= 10;
     for (int f = 0; f < sh.nfunc; ++f) {
       double Y = 0.0, Yx = 0.0, Yy = 0.0, Yz = 0.0;
+      double Yxx = 0.0, Yxy = 0.0, Yxz = 0.0, Yyy = 0.0, Yyz = 0.0, Yzz = 0.0;
       int ci = 0;
       for (int a = sh.l; a >= 0; --a)
         for (int b = sh.l - a; b >= 0; --b, ++ci) {
@@ -126,6 +129,14 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
           if (a > 0) Yx += m * a * xp[a - 1] * yp[b] * zp[c];
           if (b > 0) Yy += m * b * xp[a] * yp[b - 1] * zp[c];
           if (c > 0) Yz += m * c * xp[a] * yp[b] * zp[c - 1];
+          if (hess) {
+            if (a > 1) Yxx += m * a * (a - 1) * xp[a - 2] * yp[b] * zp[c];
+            if (b > 1) Yyy += m * b * (b - 1) * xp[a] * yp[b - 2] * zp[c];
+            if (c > 1) Yzz += m * c * (c - 1) * xp[a] * yp[b] * zp[c - 2];
+            if (a > 0 && b > 0) Yxy += m * a * b * xp[a - 1] * yp[b - 1] * zp[c];
+            if (a > 0 && c > 0) Yxz += m * a * c * xp[a - 1] * yp[b] * zp[c - 1];
+            if (b > 0 && c > 0) Yyz += m * b * c * xp[a] * yp[b - 1] * zp[c - 1];
+          }
         }
       const int64_t o = base + (int64_t)f * kTile;
       const double val = R * Y;
@@ -144,6 +155,20 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
           t32[o + cstride] = (float)gx;
           t32[o + 2 * cstride] = (float)gy;
           t32[o + 3 * cstride] = (float)gz;
+        }
+      }
+      if (hess) {
+        // d_i d_j (R Y) = R Y_ij + R' (x_i Y_j + x_j Y_i) + d_ij R' Y + R'' x_i x_j Y,
+        // R' = sum -2 a c e^(-a r^2), R'' = sum 4 a^2 c e^(-a r^2); planes xx xy xz yy yz zz
+        const double h[6] = {R * Yxx + dR * (2.0 * dx * Yx + Y) + ddR * dx * dx * Y,
+                             R * Yxy + dR * (dx * Yy + dy * Yx) + ddR * dx * dy * Y,
+                             R * Yxz + dR * (dx * Yz + dz * Yx) + ddR * dx * dz * Y,
+                             R * Yyy + dR * (2.0 * dy * Yy + Y) + ddR * dy * dy * Y,
+                             R * Yyz + dR * (dy * Yz + dz * Yy) + ddR * dy * dz * Y,
+                             R * Yzz + dR * (2.0 * dz * Yz + Y) + ddR * dz * dz * Y};
+        for (int c = 0; c < 6; ++c) {
+          if (t64) t64[o + (4 + c) * cstride] = h[c];
+          if (t32) t32[o + (4 + c) * cstride] = (float)h[c];
         }
       }
     }

@@ -218,6 +218,11 @@ struct xc_bfogc {
   // cached grid density (xc_bfogc_update_density) and the energies of it per functional
   DBuf<double> d_rho_c, d_grad_c, d_ecache;
   bool rho_c_valid = false, rho_c_grad = false;
+  // nuclear gradient (allocated on first use): work items, per-AO partials, atom map, result
+  std::vector<GradItem> grad_work;
+  DBuf<GradItem> d_grad_work;
+  DBuf<double> d_gacc, d_gout;
+  DBuf<int> d_ao_atom;
   bool ecache_valid[kNumFunctionals] = {};
   // serving: a build of this controller is in flight on a server slot; captured builds
   bool in_flight = false;
@@ -664,8 +669,8 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
     require(basis->ctx == grid->ctx, XC_ERR_ARG, "xc_bfogc_create: basis and grid live on different contexts");
     require(max_block_size >= 1 && max_block_size <= kTile, XC_ERR_ARG,
             "xc_bfogc_create: max block size must be in [1, 128]");
-    require(highest_derivative == 0 || highest_derivative == 1, XC_ERR_ARG,
-            "xc_bfogc_create: highest derivative must be 0 or 1");
+    require(highest_derivative >= 0 && highest_derivative <= 2, XC_ERR_ARG,
+            "xc_bfogc_create: highest derivative must be 0, 1 or 2");
     require(radial_threshold >= 0.0, XC_ERR_ARG, "xc_bfogc_create: radial threshold must not be negative");
     require(keep_fp32 >= 0 && keep_fp32 <= 2, XC_ERR_ARG, "xc_bfogc_create: keep_fp32 must be 0, 1 or 2");
     xc_ctx* ctx = basis->ctx;
@@ -679,7 +684,7 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       bf->bs = max_block_size;
       bf->radial_thr = radial_threshold;
       bf->deriv = highest_derivative;
-      bf->ncomp = highest_derivative >= 1 ? 4 : 1;
+      bf->ncomp = highest_derivative >= 2 ? 10 : highest_derivative == 1 ? 4 : 1;
       const int64_t N = grid->npts;
       const int64_t nb64 = (N + max_block_size - 1) / max_block_size;
       require(nb64 < (1LL << 31), XC_ERR_ARG, "xc_bfogc_create: too many blocks");
@@ -877,11 +882,38 @@ int xc_bfogc_block_data(const xc_bfogc* bf, int block, double* values, double* g
     for (int i = 0; i < bk.nb; ++i) {
       const int mu = bf->ao_idx[bk.idx_off + i];
       if (negligible) negligible[mu] = 0;
-      for (int c = 0; c < bf->ncomp; ++c) {
+      for (int c = 0; c < std::min(bf->ncomp, 4); ++c) {
         if (!outs[c]) continue;
         for (int p = 0; p < B; ++p)
           outs[c][(size_t)mu * B + p] = t[((size_t)c * bk.npad + i) * kTile + p];
       }
+    }
+  });
+}
+
+int xc_bfogc_block_hessian(const xc_bfogc* bf, int block, double* hess) {
+  return guarded([&] {
+    require(bf && hess, XC_ERR_ARG, "xc_bfogc_block_hessian: null argument");
+    require(block >= 0 && block < bf->nblocks, XC_ERR_ARG, "block index out of range");
+    require(bf->deriv >= 2, XC_ERR_STATE, "second derivatives were not evaluated (highest derivative < 2)");
+    require(bf->d_t64.p, XC_ERR_STATE, "xc_bfogc_block_hessian: the controller keeps FP32 tiles only");
+    std::lock_guard<std::recursive_mutex> lk(bf->ctx->mu);
+    const DevBlock& bk = bf->blocks[block];
+    const int n = bf->basis->nao;
+    const int B = bk.npts;
+    const int64_t cnt = (int64_t)bk.npad * kTile * 6;
+    std::vector<double> t((size_t)std::max<int64_t>(cnt, 1));
+    if (cnt) {
+      HIPX(hipMemcpyAsync(t.data(), bf->d_t64.p + bk.tile_off + (int64_t)4 * bk.npad * kTile, sizeof(double) * cnt,
+                          hipMemcpyDeviceToHost, bf->ctx->stream));
+      HIPX(hipStreamSynchronize(bf->ctx->stream));
+    }
+    std::fill(hess, hess + (size_t)6 * B * n, 0.0);
+    for (int i = 0; i < bk.nb; ++i) {
+      const int mu = bf->ao_idx[bk.idx_off + i];
+      for (int c = 0; c < 6; ++c)
+        for (int p = 0; p < B; ++p)
+          hess[((size_t)c * n + mu) * B + p] = t[((size_t)c * bk.npad + i) * kTile + p];
     }
   });
 }
@@ -1271,6 +1303,58 @@ int xc_fock_build_device(xc_bfogc* bf, int functional, int precision, double thr
       HIPX(hipStreamSynchronize(ctx->stream));
       *energy = ctx->last_energy;
     }
+  });
+}
+
+int xc_geom_gradient(xc_bfogc* bf, int functional, const double* P, int natoms, const int* ao_atom, double* grad,
+                     double* energy) {
+  return guarded([&] {
+    require(bf && P && grad && ao_atom && natoms > 0, XC_ERR_ARG, "xc_geom_gradient: bad arguments");
+    require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
+    require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
+    const bool gga = functional_is_gga(functional);
+    require(bf->deriv >= (gga ? 2 : 1), XC_ERR_STATE,
+            gga ? "xc_geom_gradient: a GGA gradient needs AO second derivatives (highest derivative 2)"
+                : "xc_geom_gradient: needs AO gradients (highest derivative 1)");
+    require(bf->d_t64.p, XC_ERR_STATE, "xc_geom_gradient: needs FP64 tiles");
+    xc_ctx* ctx = bf->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    const int n = bf->basis->nao;
+    const int64_t N = bf->grid->npts;
+    const int nxc = (int)((N + kTile - 1) / kTile);
+    for (int i = 0; i < n; ++i)
+      require(ao_atom[i] >= 0 && ao_atom[i] < natoms, XC_ERR_INPUT, "xc_geom_gradient: AO atom index out of range");
+    if (bf->grad_work.empty()) {
+      for (int b = 0; b < bf->nblocks; ++b)
+        for (int c0 = 0; c0 < bf->blocks[b].npad; c0 += kGradCols) bf->grad_work.push_back(GradItem{b, c0});
+      if (!bf->grad_work.empty()) bf->d_grad_work.upload(ctx, bf->grad_work.data(), bf->grad_work.size());
+      bf->d_gacc.alloc(ctx, (size_t)3 * std::max(n, 1));
+      bf->d_ao_atom.alloc(ctx, (size_t)std::max(n, 1));
+    }
+    if ((int64_t)bf->d_gout.n < 3 * (int64_t)natoms) bf->d_gout.alloc(ctx, (size_t)3 * natoms);
+    HIPX(hipMemcpyAsync(bf->d_ao_atom.p, ao_atom, sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIPX(hipMemcpyAsync(bf->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+    // density and functional as in a Fock build (FP64), then the gradient contractions
+    run_density<double>(bf, bf->d_t64.p, bf->d_P.p, gga ? 1 : 0, ctx->lanes);
+    if (bf->bs == kTile) {
+      launch_functional(s, functional, ctx->prm, N, kTile, nullptr, nullptr, bf->d_blocks.p, bf->d_part.p,
+                        bf->grid->d_w.p, nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
+    } else {
+      launch_reduce_partials(s, bf->d_blocks.p, bf->bs, N, bf->d_part.p, gga ? 1 : 0, bf->d_rho.p, bf->d_grad.p);
+      launch_functional(s, functional, ctx->prm, N, kTile, bf->d_rho.p, gga ? bf->d_grad.p : nullptr, nullptr,
+                        nullptr, bf->grid->d_w.p, nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
+    }
+    launch_fill(s, bf->d_gacc.p, (int64_t)3 * n, 0.0);
+    launch_gradient(s, bf->d_grad_work.p, (int)bf->grad_work.size(), bf->d_blocks.p, bf->d_ao_idx.p, bf->d_t64.p,
+                    bf->d_P.p, n, bf->d_vs.p, bf->d_gw.p, gga ? 1 : 0, N, bf->d_gacc.p);
+    launch_gradient_atoms(s, bf->d_gacc.p, bf->d_ao_atom.p, n, natoms, bf->d_gout.p);
+    launch_sum(s, bf->d_eblk.p, nxc, ctx->d_energy);
+    HIPX(hipGetLastError());
+    HIPX(hipMemcpyAsync(grad, bf->d_gout.p, sizeof(double) * 3 * natoms, hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    if (energy) *energy = ctx->last_energy;
   });
 }
 

@@ -244,7 +244,8 @@ class ScfTask:
     as Serenity does (ScfTask.cpp:127-153, 358-359)."""
 
     def __init__(self, ctx, Z, coords_bohr, basis="DEF2-SVP", functional="LDA", ri=True, aux="RI_J_WEIGEND",
-                 accuracy=4, small_accuracy=2, spherical=True, schwarz=1e-14, fp32_tiles=True, log=None):
+                 accuracy=4, small_accuracy=2, spherical=True, schwarz=1e-14, fp32_tiles=True, log=None,
+                 aux_spherical=None, radial_threshold=1e-9, block_ave_threshold=1e-11, highest_derivative=1):
         self.ctx = ctx
         self.Z = np.ascontiguousarray(Z, dtype=np.int32)
         self.X = np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
@@ -252,7 +253,12 @@ class ScfTask:
         self.say = log or (lambda *a: None)
         t0 = time.perf_counter()
         self.basis = xc.BasisController.from_file(basis, self.Z, self.X, spherical=spherical)
-        self.auxbasis = xc.BasisController.from_file(aux, self.Z, self.X, spherical=spherical) if ri else None
+        self.auxbasis = (xc.BasisController.from_file(aux, self.Z, self.X,
+                                                      spherical=spherical if aux_spherical is None else aux_spherical)
+                         if ri else None)
+        self.radial_threshold = radial_threshold
+        self.block_ave_threshold = block_ave_threshold
+        self.highest_derivative = highest_derivative
         self.ints = ScfIntegrals(self.basis, self.Z, self.X, self.auxbasis, schwarz)
         self.nocc = int(self.Z.sum()) // 2
         self.grid_small = xc.molecular_grid(self.Z, self.X, accuracy=small_accuracy, ctx=ctx)
@@ -267,15 +273,29 @@ class ScfTask:
         (FP64 tiles plus, with fp32_tiles, their FP32 copy)."""
         grid = self.grid_small if which == "small" else self.grid_default
         if which not in self._bf:
-            self._bf[which] = xc.BasisFunctionOnGridController(self.ctx, self.basis, grid, 128, 1e-9, 1,
+            self._bf[which] = xc.BasisFunctionOnGridController(self.ctx, self.basis, grid, 128,
+                                                               self.radial_threshold, self.highest_derivative,
                                                                keep_fp32=self.fp32_tiles)
-        return xc.FuncPotential(self._bf[which], self.functional, precision)
+        return xc.FuncPotential(self._bf[which], self.functional, precision, self.block_ave_threshold)
 
     def final_energy(self, P, precision="FP64"):
         """finalDFTEnergyEvaluation: E_xc on the default grid from the converged P."""
+        return self.final_evaluation(P, precision)[0]
+
+    def final_evaluation(self, P, precision="FP64"):
+        """finalDFTEnergyEvaluation (ScfTask.cpp:127-153): the energy of P with V_xc on the default
+        grid, and the density of the orbitals updated with that Fock matrix (as the reference leaves
+        it in the electronic structure).  With equal grids the SCF density is kept."""
         pot = self._potential("default", precision)
-        pot.getMatrix(P)
-        return total_energy(self.ints, P, pot.getEnergy())
+        Vxc = pot.getMatrix(P)
+        J = self.ints.coulomb(P)
+        e = self.ints.e_nuc + float(np.vdot(P, self.ints.h)) + 0.5 * float(np.vdot(P, J)) + pot.getEnergy()
+        if self.grid_default is self.grid_small:
+            return e, P
+        X = canonical_orthogonalization(self.ints.S)
+        _, Cp = np.linalg.eigh(X.T @ (self.ints.h + J + Vxc) @ X)
+        Co = (X @ Cp)[:, :self.nocc]
+        return e, 2.0 * Co @ Co.T
 
     def run(self, mode="FP64", max_cycles=100, P0=None):
         """mode "FP64" or "MIXED"; returns energies, cycle counts and timings."""
@@ -286,7 +306,7 @@ class ScfTask:
             r = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "FP64")), P0=P0,
                         max_cycles=max_cycles, log=self.say)
             out.update(cycles=r["cycles"], converged=r["converged"], energy_small_grid=r["energy"])
-            out["energy"] = self.final_energy(r["P"], "FP64")
+            out["energy"], out["P_final"] = self.final_evaluation(r["P"], "FP64")
         elif mode == "MIXED":
             r32 = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "FP32")), P0=P0,
                           max_cycles=max_cycles, log=self.say, **MIXED_THRESHOLDS)
@@ -297,13 +317,18 @@ class ScfTask:
             out.update(cycles_fp64_finish=r["cycles"], converged=r32["converged"] and r["converged"],
                        energy_small_grid=r["energy"])
             out["cycles"] = r32["cycles"] + r["cycles"]
-            out["energy"] = self.final_energy(r["P"], "FP64")
+            out["energy"], out["P_final"] = self.final_evaluation(r["P"], "FP64")
         else:
             raise xc.XCError(1, "ScfTask: unknown mode " + str(mode))
         out["P"] = r["P"]
         out["t_xc_s"] = round(r["t_xc"], 3)
         out["wall_s"] = round(time.perf_counter() - t0, 3)
         return out
+
+    def geom_gradient(self, P):
+        """XC part of the nuclear gradient of P on the default grid (FuncPotential::getGeomGradients)."""
+        pot = self._potential("default", "FP64")
+        return pot.getGeomGradients(P, natoms=len(self.Z))
 
     def close(self):
         for bf in self._bf.values():

@@ -171,6 +171,12 @@ class BasisController:
     def overlap(self):
         return overlap_matrix(self)
 
+    def getAtomIndicesOfBasis(self):
+        """Atom of every AO (BasisController::getAtomIndicesOfBasis); needs shells loaded per atom."""
+        if self.atom is None:
+            raise XCError(4, "BasisController: shells carry no atom indices")
+        return np.repeat(self.atom, self.nfunc).astype(np.int32)
+
 
 def overlap_matrix(basis):
     S = np.zeros((basis.nao, basis.nao), order="F")
@@ -279,8 +285,8 @@ class BasisFunctionOnGridController(_Handle):
     """Screened AO values (and gradients) of every grid block, resident in HBM.
 
     ``BasisFunctionOnGridController(basis, grid, maxBlockSize, radialThreshold,
-    highestDerivative)`` as in the reference; ``highestDerivative`` must be 0 or
-    1.  ``keep_fp32`` keeps an FP32 copy of the tiles for the mixed-precision
+    highestDerivative)`` as in the reference; ``highestDerivative`` is 0, 1 or 2
+    (2 adds the second derivatives the GGA nuclear gradient needs).  ``keep_fp32`` keeps an FP32 copy of the tiles for the mixed-precision
     contractions; ``keep_fp32="only"`` keeps FP32 tiles alone (FP32 builds only,
     a third of the footprint)."""
 
@@ -346,6 +352,10 @@ class BasisFunctionOnGridController(_Handle):
                                              _i(neg), C.byref(bs)))
         out = {"functionValues": vals.reshape(n, B).T, "negligible": neg}
         out["derivativeValues"] = tuple(g.reshape(n, B).T for g in grads) if grads else None
+        if self.highest_derivative >= 2:
+            h = np.zeros(6 * n * B)
+            check(_lib.lib().xc_bfogc_block_hessian(self.handle, int(block), _d(h)))
+            out["secondDerivativeValues"] = tuple(h.reshape(6, n, B)[c].T for c in range(6))
         return out
 
     _destroy = "xc_bfogc_destroy"
@@ -470,6 +480,24 @@ class FuncPotential:
         if self._energy is None:
             raise XCError(4, "FuncPotential: getMatrix has not been called")
         return self._energy
+
+    def getGeomGradients(self, P, ao_atom=None, natoms=None):
+        """XC part of the nuclear gradient (natoms x 3) of the density P at the fixed grid
+        (FuncPotential::getGeomGradients).  The controller needs highestDerivative 1 (LDA)
+        or 2 (GGA).  ``ao_atom`` defaults to the basis' getAtomIndicesOfBasis()."""
+        n = self.bfogc.getNBasisFunctions()
+        Pf = np.asfortranarray(P, dtype=np.float64)
+        if Pf.shape != (n, n):
+            raise XCError(1, "FuncPotential: density matrix is %s, basis has %d functions" % (Pf.shape, n))
+        atom = np.ascontiguousarray(self.bfogc.basis.getAtomIndicesOfBasis() if ao_atom is None else ao_atom,
+                                    dtype=np.int32)
+        na = int(atom.max()) + 1 if natoms is None else int(natoms)
+        G = np.zeros((na, 3), order="F")
+        E = C.c_double()
+        check(_lib.lib().xc_geom_gradient(self.bfogc.handle, FUNCTIONALS[self.functional], Pf.ctypes.data_as(_lib.dp),
+                                          na, _i(atom), G.ctypes.data_as(_lib.dp), C.byref(E)))
+        self._energy = E.value
+        return np.array(G)
 
     def build_device(self, dP, dF, sync=True):
         """Resident form: dP, dF are device pointers (ctypes c_void_p)."""
