@@ -128,6 +128,20 @@ class ScfIntegrals:
             j = self.eri @ p
         return unpack_symmetric(j, self.n)
 
+    def exchange(self, P):
+        """K_mu,nu = sum_la,si (mu la|nu si) P_la,si from the exact four-centre integrals
+        (held as an n^4 array after the first call; small systems only)."""
+        if self.eri is None:
+            raise xc.XCError(4, "ScfIntegrals: exchange needs the exact four-centre integrals (aux=None)")
+        if getattr(self, "_eri4", None) is None:
+            n = self.n
+            r, c = np.tril_indices(n)
+            pair = np.zeros((n, n), dtype=np.int64)
+            pair[r, c] = np.arange(len(r))
+            pair[c, r] = pair[r, c]
+            self._eri4 = self.eri[pair[:, :, None, None], pair[None, None, :, :]]  # (mu nu|la si)
+        return np.einsum("mlns,ls->mn", self._eri4, P, optimize=True)
+
 
 def canonical_orthogonalization(S, thr=1e-7):
     """X with X^T S X = 1 (eigenvectors of S with eigenvalue above thr; canOrthThreshold)."""
@@ -170,9 +184,10 @@ class Diis:
 
 
 def perform(ints, nocc, xc_build, P0=None, max_cycles=100, energy_thr=1e-10, rmsd_thr=1e-9, diis_thr=1e-8,
-            diis_store=10, log=None):
-    """Scf<RESTRICTED>::perform.  xc_build(P) -> (E_xc, V_xc).  Returns a dict with the energy of the
-    last density, P, orbitals, orbital energies, cycles and the energy history."""
+            diis_store=10, log=None, exchange=0.0):
+    """Scf<RESTRICTED>::perform.  xc_build(P) -> (E_xc, V_xc), or None for Hartree-Fock;
+    ``exchange`` is the fraction of exact exchange (F -= x K / 2, E -= x tr(P K) / 4).  Returns a dict
+    with the energy of the last density, P, orbitals, orbital energies, cycles and the energy history."""
     say = log or (lambda *a: None)
     X = canonical_orthogonalization(ints.S)
     S, h = ints.S, ints.h
@@ -190,12 +205,16 @@ def perform(ints, nocc, xc_build, P0=None, max_cycles=100, energy_thr=1e-10, rms
         t0 = time.perf_counter()
         J = ints.coulomb(P)
         t1 = time.perf_counter()
-        e_xc, Vxc = xc_build(P)
+        e_xc, Vxc = xc_build(P) if xc_build is not None else (0.0, 0.0)
         t2 = time.perf_counter()
         t_j += t1 - t0
         t_xc += t2 - t1
         F = h + J + Vxc
         energy = ints.e_nuc + float(np.vdot(P, h)) + 0.5 * float(np.vdot(P, J)) + e_xc
+        if exchange:
+            K = ints.exchange(P)
+            F = F - 0.5 * exchange * K
+            energy -= 0.25 * exchange * float(np.vdot(P, K))
         FPS = F @ P @ S
         err = X.T @ (FPS - FPS.T) @ X
         err_norm = float(np.sqrt(np.mean(err * err)))

@@ -14,6 +14,9 @@ energy from the converged density, ScfTask.cpp:127-153, 358-359):
 * water/STO-3G PBE with RI-J, qcserenity/tests/test_tasks.py:50-76: -75.2218044178 with
   LibXC's PBE and -75.22180876683022 with XCFun's (the unrestricted run of a closed shell;
   places=7 there, i.e. 5e-8).
+* ethane/def2-SVP Hartree-Fock with exact four-centre J and K, Cartesian and spherical d
+  shells, src/tasks/ScfTask_test.cpp:43-88: -79.1786812939 and -79.1728833551 at 1e-8 --
+  no XC involved, so this pins the integrals (s, p, d; both AO conventions) and the SCF.
 """
 import numpy as np
 import pytest
@@ -90,6 +93,26 @@ def test_scf_converges_from_any_guess_to_the_same_energy():
     P, S = r0["P"], ints.S
     assert abs(np.trace(P @ S) - 10.0) < 1e-10
     assert np.abs(P @ S @ P - 2.0 * P).max() < 1e-10
+
+
+def ethane():
+    """EthaneA of SystemController__TEST_SUPPLY.cpp:2613-2634 (Angstrom)."""
+    A = xc.ANGSTROM_TO_BOHR
+    Z = np.array([6, 6, 1, 1, 1, 1, 1, 1])
+    X = np.array([[-3.86799, 1.86633, 0.0], [-2.662850, 2.791330, 0.0], [-4.69801, 2.32927, 0.57419],
+                  [-3.59828, 0.89649, 0.46861], [-4.20382, 1.68541, -1.04280], [-2.32703, 2.97225, 1.04280],
+                  [-2.93257, 3.76118, -0.46861], [-1.83284, 2.32840, -0.57419]]) * A
+    return Z, X
+
+
+@pytest.mark.parametrize("spherical,ref", [(False, -79.1786812939), (True, -79.1728833551)])
+def test_ethane_svp_hartree_fock(spherical, ref):
+    Z, X = ethane()
+    b = xc.BasisController.from_file("DEF2-SVP", Z, X, spherical=spherical)
+    ints = scf.ScfIntegrals(b, Z, X)
+    r = scf.perform(ints, 9, None, exchange=1.0)
+    assert r["converged"]
+    assert abs(r["energy"] - ref) < 1e-8
 
 
 def test_nuclear_repulsion():
