@@ -175,6 +175,16 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
   }
 }
 
+// The four FP32 tile values of a lane's MFMA accumulator rows: points 4 (lane / 16) + e,
+// e = 0..3, of a 16-point row segment -- one aligned 16-byte load.
+__device__ inline void quad_rows(const float* seg, int lane, double f[4]) {
+  const float4 v = *reinterpret_cast<const float4*>(seg + 4 * (lane >> 4));
+  f[0] = v.x;
+  f[1] = v.y;
+  f[2] = v.z;
+  f[3] = v.w;
+}
+
 // Density contraction on 16 x 16 sub-tiles.  One workgroup of 8 wavefronts
 // per (block, column pass): T[p][nu] = sum_mu phi[mu][p] P[mu][nu] for the
 // pass's 16 S columns, wavefront w owning points 16 w .. 16 w + 15 and all S
@@ -184,8 +194,10 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
 // k-step t of lane group g taking AO KV g + t (KV = KC / 4) every MFMA operand
 // of a chunk is one 32-byte LDS read.  The global loads of chunk k + 1 are
 // issued before the MFMAs of chunk k.  The pass's partial rho and
-// sum_nu T grad phi are then formed by dotting T (transposed through LDS)
-// with phi / grad phi read coalesced from the tiles.
+// sum_nu T grad phi are then formed from the accumulators in registers, each
+// lane dotting its T entries with the phi / grad phi values of the same
+// (point, column), and summed over the columns by lane shuffles.  (FP64
+// accumulates T transposed, so that its lanes read whole 128-byte point runs.)
 template <typename T, int S>
 __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __restrict__ work,
                                                           const DevBlock* __restrict__ blocks,
@@ -200,13 +212,9 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
   constexpr int NC = 16 * S;
   constexpr int NB = (4 * NC + 511) / 512;      // P staging tasks per thread
   constexpr int GS = 2;
-  constexpr int CG = 32;                        // columns per transposition round of the dot phase
-  constexpr int PS = kTile + 1;                 // odd row stride of the transposed T
   constexpr int STAGE_BYTES = (kTile + NC) * RS * (int)sizeof(T);
-  constexpr int DOT_BYTES = CG * PS * (int)sizeof(T);
   typedef T vec_t __attribute__((ext_vector_type(KV)));
-  __shared__ __attribute__((aligned(16))) unsigned char smem[STAGE_BYTES > DOT_BYTES ? STAGE_BYTES : DOT_BYTES];
-  __shared__ double redv[4][4][kTile];
+  __shared__ __attribute__((aligned(16))) unsigned char smem[STAGE_BYTES];
   __shared__ int sIdx[kDensIdxLds];
   T(*sA)[RS] = reinterpret_cast<T(*)[RS]>(smem);
   T(*sB)[RS] = reinterpret_cast<T(*)[RS]>(smem + kTile * RS * sizeof(T));
@@ -281,59 +289,116 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
       for (int t = 0; t < KV; ++t)
 #pragma unroll
         for (int u = 0; u < GS; ++u)
-          if (s0 + u < S) acc[s0 + u] = M::step(av[t], bv[u][t], acc[s0 + u]);
+          if (s0 + u < S) {
+            // FP64 accumulates T transposed (rows: AO columns, columns: points), so that a
+            // lane's entries are one point of four AO columns in the dot phase
+            if constexpr (sizeof(T) == 8)
+              acc[s0 + u] = M::step(bv[u][t], av[t], acc[s0 + u]);
+            else
+              acc[s0 + u] = M::step(av[t], bv[u][t], acc[s0 + u]);
+          }
       __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
   }
 
-  // dots: thread (point p, column group h of 8) over CG = 32 columns per round
-  T(*sT)[PS] = reinterpret_cast<T(*)[PS]>(smem);
-  const int p = tid & (kTile - 1), h = tid >> 7;
-  double racc = 0.0, gxa = 0.0, gya = 0.0, gza = 0.0;
+  if constexpr (sizeof(T) == 4) {
+    // Dots straight from the accumulators: lane (r, g) holds T[row(lane, e)][16 s + r] of its
+    // wavefront's 16 points for every column tile s, and reads the matching phi / grad phi of
+    // those 4 points from the tiles (FP32: one 16-byte load per component).  All loads of the
+    // phase are independent of each other and of LDS, so they are in flight together.
+    double rv[4] = {0.0, 0.0, 0.0, 0.0}, xv[4] = {0.0, 0.0, 0.0, 0.0};
+    double yv[4] = {0.0, 0.0, 0.0, 0.0}, zv[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int c0 = 0; c0 < NC; c0 += CG) {
+    for (int c = 0; c < S; ++c) {
+      if (16 * c < ncols) {
+        const T* row = tb + (int64_t)(col0 + 16 * c + r) * kTile + 16 * w;
+        double f[4];
+        quad_rows(row, lane, f);
 #pragma unroll
-    for (int u = 0; u < CG / 16; ++u) {
-      if (c0 / 16 + u < S) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) sT[16 * u + r][16 * w + M::row(lane, e)] = acc[c0 / 16 + u][e];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < CG / 4; ++j) {
-      const int nl = 8 * h + j;
-      const int col = c0 + nl;
-      if (col < ncols) {
-        const double tv = (double)sT[nl][p];
-        const int64_t row = (int64_t)(col0 + col) * kTile + p;
-        racc += tv * (double)tb[row];
+        for (int e = 0; e < 4; ++e) rv[e] += (double)acc[c][e] * f[e];
         if (want_grad) {
-          gxa += tv * (double)tb[row + cstride];
-          gya += tv * (double)tb[row + 2 * cstride];
-          gza += tv * (double)tb[row + 3 * cstride];
+          double fx[4], fy[4], fz[4];
+          quad_rows(row + cstride, lane, fx);
+          quad_rows(row + 2 * cstride, lane, fy);
+          quad_rows(row + 3 * cstride, lane, fz);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            xv[e] += (double)acc[c][e] * fx[e];
+            yv[e] += (double)acc[c][e] * fy[e];
+            zv[e] += (double)acc[c][e] * fz[e];
+          }
         }
       }
     }
-    __syncthreads();
-  }
-  // sum the four column groups: h = 1..3 through LDS, in a fixed order
-  if (h > 0) {
-    redv[0][h][p] = racc;
-    redv[1][h][p] = gxa;
-    redv[2][h][p] = gya;
-    redv[3][h][p] = gza;
-  }
-  __syncthreads();
-  if (h == 0 && p < bk.npts) {
-    const int64_t q = bk.first + p;
-    double* dst = part + (int64_t)wk.pass * 4 * npts + q;
-    dst[0] = racc + redv[0][1][p] + redv[0][2][p] + redv[0][3][p];
+    // sum over the 16 lanes of a lane group (the columns), a fixed butterfly order
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        rv[e] += __shfl_xor(rv[e], m);
+        if (want_grad) {
+          xv[e] += __shfl_xor(xv[e], m);
+          yv[e] += __shfl_xor(yv[e], m);
+          zv[e] += __shfl_xor(zv[e], m);
+        }
+      }
+    }
+    if (r == 0) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int p = 16 * w + M::row(lane, e);
+        if (p < bk.npts) {
+          double* dst = part + (int64_t)wk.pass * 4 * npts + bk.first + p;
+          dst[0] = rv[e];
+          if (want_grad) {
+            dst[npts] = xv[e];
+            dst[2 * npts] = yv[e];
+            dst[3 * npts] = zv[e];
+          }
+        }
+      }
+    }
+  } else {
+    // lane (r, g) holds T[point 16 w + r][AO column 16 c + g + 4 e]: per e the dot reads 16
+    // consecutive points (128 bytes) of each of four tile rows
+    double rv = 0.0, xv = 0.0, yv = 0.0, zv = 0.0;
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      if (16 * c < ncols) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const T* row = tb + (int64_t)(col0 + 16 * c + g + 4 * e) * kTile + 16 * w + r;
+          const double tv = acc[c][e];
+          rv += tv * row[0];
+          if (want_grad) {
+            xv += tv * row[cstride];
+            yv += tv * row[2 * cstride];
+            zv += tv * row[3 * cstride];
+          }
+        }
+      }
+    }
+    // sum over the four lane groups (the AO columns g + 4 e of each tile)
+    rv += __shfl_xor(rv, 16);
+    rv += __shfl_xor(rv, 32);
     if (want_grad) {
-      dst[npts] = gxa + redv[1][1][p] + redv[1][2][p] + redv[1][3][p];
-      dst[2 * npts] = gya + redv[2][1][p] + redv[2][2][p] + redv[2][3][p];
-      dst[3 * npts] = gza + redv[3][1][p] + redv[3][2][p] + redv[3][3][p];
+      xv += __shfl_xor(xv, 16);
+      xv += __shfl_xor(xv, 32);
+      yv += __shfl_xor(yv, 16);
+      yv += __shfl_xor(yv, 32);
+      zv += __shfl_xor(zv, 16);
+      zv += __shfl_xor(zv, 32);
+    }
+    const int p = 16 * w + r;
+    if (g == 0 && p < bk.npts) {
+      double* dst = part + (int64_t)wk.pass * 4 * npts + bk.first + p;
+      dst[0] = rv;
+      if (want_grad) {
+        dst[npts] = xv;
+        dst[2 * npts] = yv;
+        dst[3 * npts] = zv;
+      }
     }
   }
 }
