@@ -75,7 +75,7 @@ void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const in
 // Sub-tile density kernel: 8 wavefronts, S = pass columns / 16 (1..kDensMaxS).
 // Partial densities: part[(pass * 4 + c) * npts + q], c = rho, d/dx, d/dy, d/dz
 // (gradient slots hold sum T grad phi without the factor 2).
-constexpr int kDensMaxS = 12;
+constexpr int kDensMaxS = 6;
 constexpr int kDensIdxLds = 2048;  // AO index lists up to this length are staged in LDS
 template <typename T>
 void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
