@@ -221,6 +221,20 @@ int xc_fock_build(xc_bfogc* bfogc, int functional, int precision, double block_a
 int xc_fock_build_device(xc_bfogc* bfogc, int functional, int precision, double block_ave_threshold,
                          const double* dP, double* dF, double* energy);
 
+/* FuncPotential<UNRESTRICTED>::getMatrix / getEnergy (FuncPotential.cpp:73-111 with
+ * SCFMode = UNRESTRICTED): spin density matrices P_a, P_b in, the alpha and beta
+ * XC matrices F_a, F_b (overwritten) and E_xc out.  The functional takes the spin
+ * densities (XCFun's XC_A_B_AX_AY_AZ_BX_BY_BZ variables); a block is skipped only
+ * when both spin densities are negligible, and each spin's Fock pass has its own
+ * block-average skip (ScalarOperatorToMatrixAdder.cpp:200-206, 251-268). */
+int xc_fock_build_unrestricted(xc_bfogc* bfogc, int functional, int precision, double block_ave_threshold,
+                               const double* P_alpha, const double* P_beta, double* F_alpha, double* F_beta,
+                               double* energy);
+/* Same on device buffers (xc_device_alloc). */
+int xc_fock_build_unrestricted_device(xc_bfogc* bfogc, int functional, int precision, double block_ave_threshold,
+                                      const double* dP_alpha, const double* dP_beta, double* dF_alpha,
+                                      double* dF_beta, double* energy);
+
 /* FuncPotential::getGeomGradients (FuncPotential.cpp:113-239): the XC part of
  * the nuclear gradient of the restricted density P (host, n x n column-major)
  * at a fixed grid (no weight derivatives, as the reference), written to grad

@@ -24,6 +24,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "serenity_xc.h"
@@ -306,6 +307,17 @@ class FuncPotential {
     check(xc_fock_build(_bf->get(), (int)_f, (int)_p, _thr, P.data.data(), F.data.data(), &_energy));
     _built = true;
     return F;
+  }
+  /* FuncPotential<UNRESTRICTED>::getMatrix: the alpha and beta XC matrices. */
+  std::pair<Matrix, Matrix> getMatrix(const Matrix& Palpha, const Matrix& Pbeta) {
+    const int n = (int)_bf->getNBasisFunctions();
+    if (Palpha.rows != n || Palpha.cols != n || Pbeta.rows != n || Pbeta.cols != n)
+      throw SerenityError(XC_ERR_ARG, "FuncPotential: spin density matrices and basis differ in size");
+    Matrix Fa(n, n), Fb(n, n);
+    check(xc_fock_build_unrestricted(_bf->get(), (int)_f, (int)_p, _thr, Palpha.data.data(), Pbeta.data.data(),
+                                     Fa.data.data(), Fb.data.data(), &_energy));
+    _built = true;
+    return {std::move(Fa), std::move(Fb)};
   }
   double getEnergy() const {
     if (!_built) throw SerenityError(XC_ERR_STATE, "FuncPotential: getMatrix has not been called");

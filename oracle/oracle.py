@@ -163,6 +163,40 @@ def nadd_fock_build(active_basis, env_bases, points, weights, P_active, P_envs, 
     return F, E
 
 
+def functional_pol(name, rho_a, rho_b, grad_a, grad_b, weights, block_size=128, params=None):
+    """Unrestricted calcData: (eps, v_a, v_b, g_a, g_b, E) with g_s = d eps / d grad rho_s (3 x N)."""
+    N = len(rho_a)
+    gga = name in GGA
+    eps, va, vb = np.zeros(N), np.zeros(N), np.zeros(N)
+    ga, gb = np.zeros(3 * N), np.zeros(3 * N)
+    E = C.c_double()
+    prm = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    lib().oracle_functional_pol(C.c_int(FUNCTIONALS[name]), C.c_int64(N), C.c_int(block_size), _d(f(rho_a)),
+                                _d(f(rho_b)), _d(f(grad_a)) if gga else None, _d(f(grad_b)) if gga else None,
+                                _d(f(weights)), _d(prm), _d(eps), _d(va), _d(vb), _d(ga), _d(gb), C.byref(E))
+    return eps, va, vb, ga.reshape(3, N), gb.reshape(3, N), E.value
+
+
+def fock_build_pol(basis, points, weights, Pa, Pb, name, block_size=128, thr=1e-9, block_ave_thr=1e-11,
+                   params=None):
+    """FuncPotential<UNRESTRICTED>::getMatrix/getEnergy: (F_a, F_b, E_xc)."""
+    pts = _pts(points)
+    N = len(pts)
+    n = basis.nao
+    Fa = np.zeros((n, n), order="F")
+    Fb = np.zeros((n, n), order="F")
+    E = C.c_double()
+    prm = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    lib().oracle_fock_build_pol(*_basis_args(basis), _d(pts), _d(np.ascontiguousarray(weights, dtype=np.float64)),
+                                C.c_int64(N), C.c_int(block_size), C.c_double(thr), C.c_double(block_ave_thr),
+                                C.c_int(FUNCTIONALS[name]), _d(prm),
+                                np.asfortranarray(Pa, dtype=np.float64).ctypes.data_as(dp),
+                                np.asfortranarray(Pb, dtype=np.float64).ctypes.data_as(dp),
+                                Fa.ctypes.data_as(dp), Fb.ctypes.data_as(dp), C.byref(E))
+    return Fa, Fb, E.value
+
+
 def xc_gradient(basis, points, weights, P, name, ao_atom, natoms, block_size=128, thr=1e-9, params=None):
     """FuncPotential::getGeomGradients: (natoms x 3 gradient, E_xc)."""
     pts = _pts(points)

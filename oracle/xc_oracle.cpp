@@ -17,6 +17,9 @@
 //   oracle_fock_build    FuncPotential::getMatrix / getEnergy (src/potentials/FuncPotential.cpp:66-111),
 //                        with the reference's three AO evaluations per build
 //   oracle_xc_gradient   FuncPotential::getGeomGradients (src/potentials/FuncPotential.cpp:113-239)
+//   oracle_functional_pol, oracle_fock_build_pol
+//                        the unrestricted forms (FuncPotential<UNRESTRICTED>, XCFun.cpp:38-158 with
+//                        the spin-density inputs; ScalarOperatorToMatrixAdder for_spin)
 //
 // Parity pins (tests/test_oracle.py): the reference's Mathematica known-answer
 // values for AO values/gradients/Hessians, density/gradient/Hessian and the
@@ -403,6 +406,139 @@ EV eval(int f, double r, double sig, const FParams& P) {
   return {a.e + b.e, a.dr + b.dr, a.ds + b.ds};
 }
 
+// ---- spin-polarised functionals (unrestricted) ------------------------------
+// A small forward-mode derivative type over (rho_a, rho_b, sigma_aa, sigma_ab,
+// sigma_bb), written for this restatement.  Formulas: Slater exchange in its
+// spin form -(3/2)(3/(4 pi))^(1/3) sum_s rho_s^(4/3); PBE exchange by spin
+// scaling E_x[2 rho_s, 4 sigma_ss] / 2; VWN5 with its spin interpolation
+// (paramagnetic, ferromagnetic and spin-stiffness fits); PBE correlation with
+// the spin-interpolated PW92 energy, phi(zeta) and t^2 = sigma / (4 phi^2 k_s^2 rho^2).
+struct A5 {
+  double v, d[5];
+};
+A5 cst(double v) { return {v, {0, 0, 0, 0, 0}}; }
+A5 un(const A5& a, double f, double df) {
+  A5 r{f, {}};
+  for (int i = 0; i < 5; ++i) r.d[i] = df * a.d[i];
+  return r;
+}
+A5 operator+(const A5& a, const A5& b) {
+  A5 r{a.v + b.v, {}};
+  for (int i = 0; i < 5; ++i) r.d[i] = a.d[i] + b.d[i];
+  return r;
+}
+A5 operator-(const A5& a, const A5& b) {
+  A5 r{a.v - b.v, {}};
+  for (int i = 0; i < 5; ++i) r.d[i] = a.d[i] - b.d[i];
+  return r;
+}
+A5 operator*(const A5& a, const A5& b) {
+  A5 r{a.v * b.v, {}};
+  for (int i = 0; i < 5; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+  return r;
+}
+A5 operator/(const A5& a, const A5& b) {
+  A5 r{a.v / b.v, {}};
+  for (int i = 0; i < 5; ++i) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v);
+  return r;
+}
+A5 operator+(const A5& a, double b) { return a + cst(b); }
+A5 operator+(double b, const A5& a) { return cst(b) + a; }
+A5 operator-(const A5& a, double b) { return a - cst(b); }
+A5 operator-(double b, const A5& a) { return cst(b) - a; }
+A5 operator*(double b, const A5& a) { return un(a, a.v * b, b); }
+A5 operator/(const A5& a, double b) { return un(a, a.v / b, 1.0 / b); }
+A5 operator/(double b, const A5& a) { return cst(b) / a; }
+A5 aln(const A5& a) { return un(a, std::log(a.v), 1.0 / a.v); }
+A5 aexp(const A5& a) { return un(a, std::exp(a.v), std::exp(a.v)); }
+A5 asqrt(const A5& a) { return un(a, std::sqrt(a.v), 0.5 / std::sqrt(a.v)); }
+A5 aatan(const A5& a) { return un(a, std::atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+A5 apow(const A5& a, double p) { return un(a, std::pow(a.v, p), a.v > 0 ? p * std::pow(a.v, p - 1.0) : 0.0); }
+
+A5 vwn_g(const A5& x, double A, double b, double c, double x0) {
+  // VWN fit in x = sqrt(r_s)
+  const double Q = std::sqrt(4.0 * c - b * b), X0 = x0 * x0 + b * x0 + c;
+  const A5 X = x * x + b * x + c;
+  const A5 at = aatan(Q / (2.0 * x + b));
+  return A * (aln(x * x / X) + (2.0 * b / Q) * at -
+              (b * x0 / X0) * (aln((x - x0) * (x - x0) / X) + (2.0 * (b + 2.0 * x0) / Q) * at));
+}
+A5 pw_g(const A5& rs, double A, double a1, double b1, double b2, double b3, double b4) {
+  const A5 s = asqrt(rs);
+  return -2.0 * A * (1.0 + a1 * rs) * aln(1.0 + 1.0 / (2.0 * A * (b1 * s + b2 * rs + b3 * rs * s + b4 * rs * rs)));
+}
+const double kF2 = 4.0 / (9.0 * (std::cbrt(2.0) - 1.0));  // f''(0)
+A5 fzeta(const A5& z) {
+  return (apow(1.0 + z, 4.0 / 3.0) + apow(1.0 - z, 4.0 / 3.0) - 2.0) / (2.0 * std::cbrt(2.0) - 2.0);
+}
+// kind: 0 Slater, 1 TF, 2 PBE exchange with mu_xcfun, 3 PBE exchange with mu_libxc
+A5 x_pol(int kind, const A5& ra, const A5& rb, const A5& saa, const A5& sbb, bool aon, bool bon, const FParams& P) {
+  A5 e = cst(0.0);
+  for (int s = 0; s < 2; ++s) {
+    if (!(s == 0 ? aon : bon)) continue;
+    const A5& r = s == 0 ? ra : rb;
+    const A5& sg = s == 0 ? saa : sbb;
+    if (kind == 0) {
+      e = e + (-1.5 * std::cbrt(3.0 / (4.0 * M_PI))) * apow(r, 4.0 / 3.0);
+    } else if (kind == 1) {
+      e = e + 0.5 * (0.3 * std::pow(3.0 * M_PI * M_PI, 2.0 / 3.0)) * apow(2.0 * r, 5.0 / 3.0);
+    } else {
+      const double mu = kind == 2 ? P.mu : P.mu_libxc;
+      const A5 r2 = 2.0 * r, s4 = 4.0 * sg;
+      const A5 s2 = s4 / (4.0 * std::pow(3.0 * M_PI * M_PI, 2.0 / 3.0) * apow(r2, 8.0 / 3.0));
+      const A5 F = (1.0 + P.kappa) - P.kappa / (1.0 + mu * s2 / P.kappa);
+      e = e + 0.5 * (-0.75 * std::cbrt(3.0 / M_PI)) * apow(r2, 4.0 / 3.0) * F;
+    }
+  }
+  return e;
+}
+A5 vwn_pol(const A5& ra, const A5& rb, const FParams& P) {
+  const A5 n = ra + rb, z = (ra - rb) / n;
+  const A5 x = asqrt(apow(3.0 / (4.0 * M_PI * n), 1.0 / 3.0));
+  const A5 eP = vwn_g(x, P.vwn_A, P.vwn_b, P.vwn_c, P.vwn_x0);
+  const A5 eF = vwn_g(x, 0.01554535, 7.06042, 18.0578, -0.32500);
+  const A5 aS = vwn_g(x, -1.0 / (6.0 * M_PI * M_PI), 1.13107, 13.0045, -0.0047584);
+  const A5 f = fzeta(z), z4 = z * z * z * z;
+  return n * (eP + aS * f / kF2 * (1.0 - z4) + (eF - eP) * f * z4);
+}
+A5 pbec_pol(const A5& ra, const A5& rb, const A5& sig, const FParams& P) {
+  const A5 n = ra + rb, z = (ra - rb) / n;
+  const A5 rs = apow(3.0 / (4.0 * M_PI * n), 1.0 / 3.0);
+  const A5 f = fzeta(z), z4 = z * z * z * z;
+  const A5 e0 = pw_g(rs, P.pw_A, P.pw_a1, P.pw_b1, P.pw_b2, P.pw_b3, P.pw_b4);
+  const A5 e1 = pw_g(rs, 0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  const A5 ac = -1.0 * pw_g(rs, 0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
+  const A5 ec = e0 + ac * f / kF2 * (1.0 - z4) + (e1 - e0) * f * z4;
+  A5 opz = 1.0 + z, omz = 1.0 - z;
+  if (opz.v < 1e-12) opz.v = 1e-12;
+  if (omz.v < 1e-12) omz.v = 1e-12;
+  const A5 phi = 0.5 * (apow(opz, 2.0 / 3.0) + apow(omz, 2.0 / 3.0));
+  const A5 kF = apow(3.0 * M_PI * M_PI * n, 1.0 / 3.0);
+  const A5 ks2 = 4.0 * kF / M_PI;
+  const A5 t2 = sig / (4.0 * phi * phi * ks2 * n * n);
+  const double gam = (1.0 - std::log(2.0)) / (M_PI * M_PI);
+  const A5 phi3 = phi * phi * phi;
+  const A5 A = (P.beta / gam) / (aexp(-1.0 * ec / (gam * phi3)) - 1.0);
+  const A5 H = gam * phi3 * aln(1.0 + (P.beta / gam) * t2 * (1.0 + A * t2) / (1.0 + A * t2 + A * A * t2 * t2));
+  return n * (ec + H);
+}
+A5 eval_pol(int f, const A5& ra, const A5& rb, const A5& saa, const A5& sab, const A5& sbb, bool aon, bool bon,
+            const FParams& P) {
+  const A5 sig = saa + 2.0 * sab + sbb;
+  const A5 rac = aon ? ra : cst(0.0), rbc = bon ? rb : cst(0.0);
+  switch (f) {
+    case 0: return x_pol(0, ra, rb, saa, sbb, aon, bon, P) + vwn_pol(rac, rbc, P);
+    case 1: return x_pol(2, ra, rb, saa, sbb, aon, bon, P) + pbec_pol(rac, rbc, sig, P);
+    case 2: return x_pol(1, ra, rb, saa, sbb, aon, bon, P);
+    case 3: return x_pol(0, ra, rb, saa, sbb, aon, bon, P);
+    case 4: return vwn_pol(rac, rbc, P);
+    case 5: return x_pol(2, ra, rb, saa, sbb, aon, bon, P);
+    case 6: return pbec_pol(rac, rbc, sig, P);
+    case 7: return x_pol(3, ra, rb, saa, sbb, aon, bon, P) + pbec_pol(rac, rbc, sig, P);
+    default: return cst(0.0);
+  }
+}
+
 FParams params_from(const double* p) {
   FParams P;
   if (p) {
@@ -747,6 +883,98 @@ int oracle_xc_gradient(int nshell, const int* l, const int* pure, const int* npr
   std::fill(grad, grad + (size_t)natoms * 3, 0.0);
   for (int t = 0; t < nthreads; ++t)
     for (size_t e = 0; e < (size_t)natoms * 3; ++e) grad[e] += Gt[t][e];
+  return 0;
+}
+
+// Unrestricted XCFun/LibXC calcData (XC_A_B_AX_AY_AZ_BX_BY_BZ): the block is skipped when
+// both spin densities are negligible (XCFun.cpp:136-140 with for_spin); a point counts when
+// rho_a + rho_b >= 1e-14, a spin channel below 1e-14 is dropped.  g_s = d eps / d grad rho_s.
+int oracle_functional_pol(int func, int64_t N, int bs, const double* ra, const double* rb, const double* gra,
+                          const double* grb, const double* w, const double* params, double* eps, double* va,
+                          double* vb, double* ga, double* gb, double* energy) {
+  const FParams P = params_from(params);
+  const bool gga = is_gga(func);
+  const int64_t nb = (N + bs - 1) / bs;
+  std::vector<double> eb(nb, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t first = b * bs;
+    const int B = (int)std::min<int64_t>(bs, N - first);
+    double sa = 0.0, sb = 0.0;
+    for (int p = 0; p < B; ++p) {
+      sa += std::fabs(ra[first + p]);
+      sb += std::fabs(rb[first + p]);
+    }
+    const bool skip = sa < B * 1e-12 && sb < B * 1e-12;
+    double e_acc = 0.0;
+    for (int p = 0; p < B; ++p) {
+      const int64_t q = first + p;
+      double e = 0.0, v2[2] = {0, 0}, g6[6] = {0, 0, 0, 0, 0, 0};
+      const double a = ra[q], bb = rb[q];
+      const bool aon = a >= 1e-14, bon = bb >= 1e-14;
+      if (!skip && a + bb >= 1e-14 && (aon || bon)) {
+        double x[3] = {0, 0, 0}, y[3] = {0, 0, 0};
+        if (gga)
+          for (int c = 0; c < 3; ++c) {
+            x[c] = gra[c * N + q];
+            y[c] = grb[c * N + q];
+          }
+        A5 in[5];
+        const double vals[5] = {aon ? a : 0.0, bon ? bb : 0.0, x[0] * x[0] + x[1] * x[1] + x[2] * x[2],
+                                x[0] * y[0] + x[1] * y[1] + x[2] * y[2], y[0] * y[0] + y[1] * y[1] + y[2] * y[2]};
+        for (int k = 0; k < 5; ++k) {
+          in[k] = cst(vals[k]);
+          in[k].d[k] = 1.0;
+        }
+        const A5 r = eval_pol(func, in[0], in[1], in[2], in[3], in[4], aon, bon, P);
+        e = r.v;
+        v2[0] = r.d[0];
+        v2[1] = r.d[1];
+        if (gga)
+          for (int c = 0; c < 3; ++c) {
+            g6[c] = 2.0 * r.d[2] * x[c] + r.d[3] * y[c];
+            g6[3 + c] = 2.0 * r.d[4] * y[c] + r.d[3] * x[c];
+          }
+      }
+      if (eps) eps[q] = e;
+      if (va) va[q] = v2[0];
+      if (vb) vb[q] = v2[1];
+      for (int c = 0; c < 3; ++c) {
+        if (ga) ga[c * N + q] = g6[c];
+        if (gb) gb[c * N + q] = g6[3 + c];
+      }
+      e_acc += w[q] * e;
+    }
+    eb[b] = e_acc;
+  }
+  double E = 0.0;
+  for (int64_t b = 0; b < nb; ++b) E += eb[b];
+  if (energy) *energy = E;
+  return 0;
+}
+
+// FuncPotential<UNRESTRICTED>::getMatrix / getEnergy: F_a, F_b overwritten.
+int oracle_fock_build_pol(int nshell, const int* l, const int* pure, const int* nprim, const double* center,
+                          const double* alpha, const double* coef, const double* pts, const double* w, int64_t N,
+                          int bs, double thr, double block_ave_thr, int func, const double* params, const double* Pa,
+                          const double* Pb, double* Fa, double* Fb, double* energy) {
+  const bool gga = is_gga(func);
+  std::vector<double> ra(N), rb(N), gra(gga ? 3 * N : 0), grb(gga ? 3 * N : 0), va(N), vb(N), ga(gga ? 3 * N : 0),
+      gb(gga ? 3 * N : 0);
+  oracle_density(nshell, l, pure, nprim, center, alpha, coef, pts, N, bs, thr, Pa, ra.data(),
+                 gga ? gra.data() : nullptr, nullptr, nullptr);
+  oracle_density(nshell, l, pure, nprim, center, alpha, coef, pts, N, bs, thr, Pb, rb.data(),
+                 gga ? grb.data() : nullptr, nullptr, nullptr);
+  oracle_functional_pol(func, N, 128, ra.data(), rb.data(), gga ? gra.data() : nullptr, gga ? grb.data() : nullptr,
+                        w, params, nullptr, va.data(), vb.data(), gga ? ga.data() : nullptr,
+                        gga ? gb.data() : nullptr, energy);
+  Basis bas(nshell, l, pure, nprim, center, alpha, coef);
+  std::fill(Fa, Fa + (size_t)bas.nao * bas.nao, 0.0);
+  std::fill(Fb, Fb + (size_t)bas.nao * bas.nao, 0.0);
+  oracle_fock_add(nshell, l, pure, nprim, center, alpha, coef, pts, w, N, bs, thr, block_ave_thr, va.data(),
+                  gga ? ga.data() : nullptr, Fa, 0, -1);
+  oracle_fock_add(nshell, l, pure, nprim, center, alpha, coef, pts, w, N, bs, thr, block_ave_thr, vb.data(),
+                  gga ? gb.data() : nullptr, Fb, 0, -1);
   return 0;
 }
 

@@ -1,15 +1,21 @@
 // Exchange-correlation functionals of the XC Fock build, FP64, evaluated per
-// grid point with forward-mode derivatives (value, d/d rho, d/d sigma) so the
-// potential terms come from the same code as the energy.
+// grid point with forward-mode derivatives so the potential terms come from
+// the same code as the energy.
 //
-// The formulas restate the basic functionals Serenity takes from XCFun for
-// the LDA and PBE composites (functional_definitions.dat:44,47;
+// The formulas restate the basic functionals Serenity takes from XCFun (or
+// LibXC) for the LDA and PBE composites (functional_definitions.dat:44,47;
 // basic_functionals.dat:35,78,99,168,230): Slater exchange, VWN5 correlation
-// (Vosko-Wilk-Nusair 1980, paramagnetic fit), PBE exchange and correlation
-// (Perdew-Burke-Ernzerhof 1996 with the Perdew-Wang 1992 LDA correlation) and
-// Thomas-Fermi kinetic energy.  Inputs are restricted (closed-shell) total
-// density rho and sigma = |grad rho|^2; XCFun's XC_N_NX_NY_NZ outputs are then
+// (Vosko-Wilk-Nusair 1980 with their spin interpolation), PBE exchange and
+// correlation (Perdew-Burke-Ernzerhof 1996 with the Perdew-Wang 1992 LDA
+// correlation) and Thomas-Fermi kinetic energy.
+//
+// Restricted inputs: total density rho and sigma = |grad rho|^2 (partials in
+// that order); XCFun's XC_N_NX_NY_NZ outputs are then
 //   eps (energy per volume), v = d eps/d rho, g = 2 d eps/d sigma * grad rho.
+// Unrestricted inputs (XC_A_B_AX_AY_AZ_BX_BY_BZ): rho_a, rho_b and
+// sigma_aa, sigma_ab, sigma_bb (partials in that order); outputs
+//   v_a = d eps/d rho_a, g_a = 2 d eps/d sigma_aa grad rho_a + d eps/d sigma_ab grad rho_b
+// and likewise for b.
 #pragma once
 
 #include <cmath>
@@ -22,42 +28,142 @@
 
 namespace sxc {
 
-// Value and first partials with respect to (rho, sigma).
-struct D2 {
-  double v, dr, ds;
+// Value and N first partials.
+template <int N>
+struct Dual {
+  double v;
+  double d[N];
 };
+typedef Dual<2> D2;  // (rho, sigma)
+typedef Dual<5> D5;  // (rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb)
 
-SXC_HD D2 mk(double v) { return {v, 0.0, 0.0}; }
-SXC_HD D2 operator+(D2 a, D2 b) { return {a.v + b.v, a.dr + b.dr, a.ds + b.ds}; }
-SXC_HD D2 operator-(D2 a, D2 b) { return {a.v - b.v, a.dr - b.dr, a.ds - b.ds}; }
-SXC_HD D2 operator-(D2 a) { return {-a.v, -a.dr, -a.ds}; }
-SXC_HD D2 operator*(D2 a, D2 b) { return {a.v * b.v, a.dr * b.v + a.v * b.dr, a.ds * b.v + a.v * b.ds}; }
-SXC_HD D2 operator/(D2 a, D2 b) {
+template <int N>
+SXC_HD Dual<N> mkc(double v) {
+  Dual<N> r;
+  r.v = v;
+  for (int i = 0; i < N; ++i) r.d[i] = 0.0;
+  return r;
+}
+template <int N>
+SXC_HD Dual<N> var(double v, int k) {
+  Dual<N> r = mkc<N>(v);
+  r.d[k] = 1.0;
+  return r;
+}
+SXC_HD D2 mk(double v) { return mkc<2>(v); }
+
+template <int N>
+SXC_HD Dual<N> operator+(Dual<N> a, Dual<N> b) {
+  a.v += b.v;
+  for (int i = 0; i < N; ++i) a.d[i] += b.d[i];
+  return a;
+}
+template <int N>
+SXC_HD Dual<N> operator-(Dual<N> a, Dual<N> b) {
+  a.v -= b.v;
+  for (int i = 0; i < N; ++i) a.d[i] -= b.d[i];
+  return a;
+}
+template <int N>
+SXC_HD Dual<N> operator-(Dual<N> a) {
+  a.v = -a.v;
+  for (int i = 0; i < N; ++i) a.d[i] = -a.d[i];
+  return a;
+}
+template <int N>
+SXC_HD Dual<N> operator*(Dual<N> a, Dual<N> b) {
+  Dual<N> r;
+  r.v = a.v * b.v;
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+  return r;
+}
+template <int N>
+SXC_HD Dual<N> operator/(Dual<N> a, Dual<N> b) {
   const double ib = 1.0 / b.v, q = a.v * ib;
-  return {q, (a.dr - q * b.dr) * ib, (a.ds - q * b.ds) * ib};
+  Dual<N> r;
+  r.v = q;
+  for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - q * b.d[i]) * ib;
+  return r;
 }
-SXC_HD D2 operator+(D2 a, double b) { return {a.v + b, a.dr, a.ds}; }
-SXC_HD D2 operator+(double b, D2 a) { return {a.v + b, a.dr, a.ds}; }
-SXC_HD D2 operator-(D2 a, double b) { return {a.v - b, a.dr, a.ds}; }
-SXC_HD D2 operator-(double b, D2 a) { return {b - a.v, -a.dr, -a.ds}; }
-SXC_HD D2 operator*(D2 a, double b) { return {a.v * b, a.dr * b, a.ds * b}; }
-SXC_HD D2 operator*(double b, D2 a) { return {a.v * b, a.dr * b, a.ds * b}; }
-SXC_HD D2 operator/(D2 a, double b) { return {a.v / b, a.dr / b, a.ds / b}; }
-SXC_HD D2 operator/(double b, D2 a) {
-  const double q = b / a.v, d = -q / a.v;
-  return {q, d * a.dr, d * a.ds};
+template <int N>
+SXC_HD Dual<N> operator+(Dual<N> a, double b) {
+  a.v += b;
+  return a;
 }
-SXC_HD D2 chain(D2 a, double f, double df) { return {f, df * a.dr, df * a.ds}; }
-SXC_HD D2 dlog(D2 a) { return chain(a, log(a.v), 1.0 / a.v); }
-SXC_HD D2 dexpm1(D2 a) { return chain(a, expm1(a.v), exp(a.v)); }
-SXC_HD D2 dsqrt(D2 a) {
+template <int N>
+SXC_HD Dual<N> operator+(double b, Dual<N> a) {
+  a.v += b;
+  return a;
+}
+template <int N>
+SXC_HD Dual<N> operator-(Dual<N> a, double b) {
+  a.v -= b;
+  return a;
+}
+template <int N>
+SXC_HD Dual<N> operator-(double b, Dual<N> a) {
+  a.v = b - a.v;
+  for (int i = 0; i < N; ++i) a.d[i] = -a.d[i];
+  return a;
+}
+template <int N>
+SXC_HD Dual<N> operator*(Dual<N> a, double b) {
+  a.v *= b;
+  for (int i = 0; i < N; ++i) a.d[i] *= b;
+  return a;
+}
+template <int N>
+SXC_HD Dual<N> operator*(double b, Dual<N> a) {
+  return a * b;
+}
+template <int N>
+SXC_HD Dual<N> operator/(Dual<N> a, double b) {
+  a.v /= b;
+  for (int i = 0; i < N; ++i) a.d[i] /= b;
+  return a;
+}
+template <int N>
+SXC_HD Dual<N> operator/(double b, Dual<N> a) {
+  const double q = b / a.v, dq = -q / a.v;
+  Dual<N> r;
+  r.v = q;
+  for (int i = 0; i < N; ++i) r.d[i] = dq * a.d[i];
+  return r;
+}
+template <int N>
+SXC_HD Dual<N> chain(Dual<N> a, double f, double df) {
+  Dual<N> r;
+  r.v = f;
+  for (int i = 0; i < N; ++i) r.d[i] = df * a.d[i];
+  return r;
+}
+template <int N>
+SXC_HD Dual<N> dlog(Dual<N> a) {
+  return chain(a, log(a.v), 1.0 / a.v);
+}
+template <int N>
+SXC_HD Dual<N> dexpm1(Dual<N> a) {
+  return chain(a, expm1(a.v), exp(a.v));
+}
+template <int N>
+SXC_HD Dual<N> dsqrt(Dual<N> a) {
   const double s = sqrt(a.v);
   return chain(a, s, 0.5 / s);
 }
-SXC_HD D2 datan(D2 a) { return chain(a, atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
-SXC_HD D2 dpow(D2 a, double p) {
+template <int N>
+SXC_HD Dual<N> datan(Dual<N> a) {
+  return chain(a, atan(a.v), 1.0 / (1.0 + a.v * a.v));
+}
+template <int N>
+SXC_HD Dual<N> dpow(Dual<N> a, double p) {
   const double f = pow(a.v, p);
   return chain(a, f, p * f / a.v);
+}
+// a^(4/3) with the derivative 4/3 a^(1/3), finite at a = 0 (spin-interpolation of a full polarisation)
+template <int N>
+SXC_HD Dual<N> dpow43(Dual<N> a) {
+  const double c = cbrt(a.v);
+  return chain(a, a.v * c, (4.0 / 3.0) * c);
 }
 
 // Constants of the functionals (the published VWN5, PW92 and PBE values).
@@ -67,68 +173,162 @@ SXC_HD D2 dpow(D2 a, double p) {
 // use A = 0.0310907 in PW92 and the accurate beta in the correlation.  The
 // full-SCF energies of Serenity's water/STO-3G PBE tests decide these values
 // (qcserenity/tests/test_tasks.py:50-76: 2.8e-8 for XCFun, 2.2e-10 for LibXC).
+// Spin-polarised parts: VWN5 ferromagnetic and spin-stiffness fits, PW92
+// ferromagnetic and spin-stiffness fits (the A of the "modified" PW92 that
+// goes with 0.0310907), f''(0) = 4 / (9 (2^(1/3) - 1)).
 struct XcParams {
   double vwn_A = 0.0310907, vwn_b = 3.72744, vwn_c = 12.9352, vwn_x0 = -0.10498;
   double pw_A = 0.0310907, pw_a1 = 0.21370, pw_b1 = 7.5957, pw_b2 = 3.5876, pw_b3 = 1.6382, pw_b4 = 0.49294;
   double pbe_kappa = 0.804, pbe_mu = 0.066725 * M_PI * M_PI / 3.0, pbe_beta = 0.06672455060314922;
   double pbe_mu_libxc = 0.2195149727645171;
+  // VWN5: ferromagnetic (A, b, c, x0) and spin stiffness
+  double vwnF_A = 0.01554535, vwnF_b = 7.06042, vwnF_c = 18.0578, vwnF_x0 = -0.32500;
+  double vwnS_A = -1.0 / (6.0 * M_PI * M_PI), vwnS_b = 1.13107, vwnS_c = 13.0045, vwnS_x0 = -0.0047584;
+  // PW92: ferromagnetic and (minus) spin stiffness
+  double pwF_A = 0.01554535, pwF_a1 = 0.20548, pwF_b1 = 14.1189, pwF_b2 = 6.1977, pwF_b3 = 3.3662,
+         pwF_b4 = 0.62517;
+  double pwS_A = 0.0168869, pwS_a1 = 0.11125, pwS_b1 = 10.357, pwS_b2 = 3.6231, pwS_b3 = 0.88026,
+         pwS_b4 = 0.49671;
 };
 
-SXC_HD D2 slater_x(D2 rho) {
+// f''(0) of the spin interpolation f(zeta) = ((1+zeta)^(4/3) + (1-zeta)^(4/3) - 2) / (2^(4/3) - 2)
+constexpr double kFpp0 = 1.709920934161365617563962776245;
+
+template <class D>
+SXC_HD D slater_x(D rho) {
   // -(3/4) (3/pi)^(1/3) rho^(4/3)
   const double c = -0.75 * cbrt(3.0 / M_PI);
   return c * dpow(rho, 4.0 / 3.0);
 }
 
-SXC_HD D2 vwn5_eps(D2 rs, const XcParams& p) {
-  // per-particle VWN5 paramagnetic correlation as a function of x = sqrt(r_s)
-  const double b = p.vwn_b, c = p.vwn_c, x0 = p.vwn_x0;
+// VWN per-particle correlation fit as a function of x = sqrt(r_s)
+template <class D>
+SXC_HD D vwn_fit(D rs, double A, double b, double c, double x0) {
   const double Q = sqrt(4.0 * c - b * b);
   const double X0 = x0 * x0 + b * x0 + c;
-  D2 x = dsqrt(rs);
-  D2 X = x * x + b * x + c;
-  D2 at = datan(Q / (2.0 * x + b));
-  D2 t1 = dlog(x * x / X) + (2.0 * b / Q) * at;
-  D2 t2 = dlog((x - x0) * (x - x0) / X) + (2.0 * (b + 2.0 * x0) / Q) * at;
-  return p.vwn_A * (t1 - (b * x0 / X0) * t2);
+  D x = dsqrt(rs);
+  D X = x * x + b * x + c;
+  D at = datan(Q / (2.0 * x + b));
+  D t1 = dlog(x * x / X) + (2.0 * b / Q) * at;
+  D t2 = dlog((x - x0) * (x - x0) / X) + (2.0 * (b + 2.0 * x0) / Q) * at;
+  return A * (t1 - (b * x0 / X0) * t2);
 }
 
-SXC_HD D2 rs_of(D2 rho) { return dpow((3.0 / (4.0 * M_PI)) / rho, 1.0 / 3.0); }
-
-SXC_HD D2 vwn5_c(D2 rho, const XcParams& p) { return rho * vwn5_eps(rs_of(rho), p); }
-
-SXC_HD D2 pw92_eps(D2 rs, const XcParams& p) {
-  // Perdew-Wang 1992 unpolarised correlation energy per particle (p = 1)
-  D2 srs = dsqrt(rs);
-  D2 den = 2.0 * p.pw_A * (p.pw_b1 * srs + p.pw_b2 * rs + p.pw_b3 * rs * srs + p.pw_b4 * rs * rs);
-  return -2.0 * p.pw_A * (1.0 + p.pw_a1 * rs) * dlog(1.0 + 1.0 / den);
+template <class D>
+SXC_HD D vwn5_eps(D rs, const XcParams& p) {
+  return vwn_fit(rs, p.vwn_A, p.vwn_b, p.vwn_c, p.vwn_x0);
 }
 
-SXC_HD D2 pbe_x(D2 rho, D2 sigma, const XcParams& p, double mu) {
+template <class D>
+SXC_HD D rs_of(D rho) {
+  return dpow((3.0 / (4.0 * M_PI)) / rho, 1.0 / 3.0);
+}
+
+template <class D>
+SXC_HD D vwn5_c(D rho, const XcParams& p) {
+  return rho * vwn5_eps(rs_of(rho), p);
+}
+
+// Perdew-Wang 1992 fit G(r_s) (p = 1)
+template <class D>
+SXC_HD D pw_fit(D rs, double A, double a1, double b1, double b2, double b3, double b4) {
+  D srs = dsqrt(rs);
+  D den = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+  return -2.0 * A * (1.0 + a1 * rs) * dlog(1.0 + 1.0 / den);
+}
+
+template <class D>
+SXC_HD D pw92_eps(D rs, const XcParams& p) {
+  // Perdew-Wang 1992 unpolarised correlation energy per particle
+  return pw_fit(rs, p.pw_A, p.pw_a1, p.pw_b1, p.pw_b2, p.pw_b3, p.pw_b4);
+}
+
+template <class D>
+SXC_HD D pbe_x(D rho, D sigma, const XcParams& p, double mu) {
   // s^2 = sigma / (4 (3 pi^2)^(2/3) rho^(8/3))
   const double k = 4.0 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
-  D2 s2 = sigma / (k * dpow(rho, 8.0 / 3.0));
-  D2 Fx = (1.0 + p.pbe_kappa) - p.pbe_kappa / (1.0 + (mu / p.pbe_kappa) * s2);
+  D s2 = sigma / (k * dpow(rho, 8.0 / 3.0));
+  D Fx = (1.0 + p.pbe_kappa) - p.pbe_kappa / (1.0 + (mu / p.pbe_kappa) * s2);
   return slater_x(rho) * Fx;
 }
 
-SXC_HD D2 pbe_c(D2 rho, D2 sigma, const XcParams& p) {
+// PBE gradient correction H(eps, t^2, phi) per particle (phi = 1 unpolarised)
+template <class D>
+SXC_HD D pbe_H(D eps, D t2, D phi3, const XcParams& p) {
   const double gamma = (1.0 - log(2.0)) / (M_PI * M_PI);
   const double bg = p.pbe_beta / gamma;
-  D2 eps = pw92_eps(rs_of(rho), p);
+  D A = bg / dexpm1(-eps / (gamma * phi3));
+  D At2 = A * t2;
+  return gamma * phi3 * dlog(1.0 + bg * t2 * (1.0 + At2) / (1.0 + At2 * (1.0 + At2)));
+}
+
+template <class D>
+SXC_HD D pbe_c(D rho, D sigma, const XcParams& p) {
+  const double gamma = (1.0 - log(2.0)) / (M_PI * M_PI);
+  const double bg = p.pbe_beta / gamma;
+  D eps = pw92_eps(rs_of(rho), p);
   // t^2 = sigma pi / (16 (3 pi^2)^(1/3) rho^(7/3))  (zeta = 0, phi = 1)
   const double kt = M_PI / (16.0 * cbrt(3.0 * M_PI * M_PI));
-  D2 t2 = kt * sigma / dpow(rho, 7.0 / 3.0);
-  D2 A = bg / dexpm1(-eps / gamma);
-  D2 At2 = A * t2;
-  D2 H = gamma * dlog(1.0 + bg * t2 * (1.0 + At2) / (1.0 + At2 * (1.0 + At2)));
+  D t2 = kt * sigma / dpow(rho, 7.0 / 3.0);
+  D A = bg / dexpm1(-eps / gamma);
+  D At2 = A * t2;
+  D H = gamma * dlog(1.0 + bg * t2 * (1.0 + At2) / (1.0 + At2 * (1.0 + At2)));
   return rho * (eps + H);
 }
 
-SXC_HD D2 tf_k(D2 rho) {
+template <class D>
+SXC_HD D tf_k(D rho) {
   // C_F rho^(5/3), C_F = (3/10)(3 pi^2)^(2/3)
   const double cf = 0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
   return cf * dpow(rho, 5.0 / 3.0);
+}
+
+// ---- spin-polarised forms ---------------------------------------------------
+// Spin densities below kTinyDensity count as zero: their terms are dropped.
+
+// spin interpolation f(zeta) from 1 + zeta = 2 rho_a / rho and 1 - zeta = 2 rho_b / rho
+template <class D>
+SXC_HD D spin_f(D opz, D omz) {
+  return (dpow43(opz) + dpow43(omz) - 2.0) / (2.0 * cbrt(2.0) - 2.0);
+}
+
+template <class D>
+SXC_HD D vwn5_c_pol(D ra, D rb, const XcParams& p) {
+  D rho = ra + rb;
+  D rs = rs_of(rho);
+  D zeta = (ra - rb) / rho;
+  D f = spin_f(1.0 + zeta, 1.0 - zeta);
+  D z4 = zeta * zeta * zeta * zeta;
+  D eP = vwn_fit(rs, p.vwn_A, p.vwn_b, p.vwn_c, p.vwn_x0);
+  D eF = vwn_fit(rs, p.vwnF_A, p.vwnF_b, p.vwnF_c, p.vwnF_x0);
+  D aS = vwn_fit(rs, p.vwnS_A, p.vwnS_b, p.vwnS_c, p.vwnS_x0);
+  return rho * (eP + aS * f * (1.0 - z4) / kFpp0 + (eF - eP) * f * z4);
+}
+
+template <class D>
+SXC_HD D pw92_eps_pol(D rs, D zeta, D f, const XcParams& p) {
+  D z4 = zeta * zeta * zeta * zeta;
+  D e0 = pw_fit(rs, p.pw_A, p.pw_a1, p.pw_b1, p.pw_b2, p.pw_b3, p.pw_b4);
+  D e1 = pw_fit(rs, p.pwF_A, p.pwF_a1, p.pwF_b1, p.pwF_b2, p.pwF_b3, p.pwF_b4);
+  D mac = pw_fit(rs, p.pwS_A, p.pwS_a1, p.pwS_b1, p.pwS_b2, p.pwS_b3, p.pwS_b4);  // -alpha_c
+  return e0 - mac * f * (1.0 - z4) / kFpp0 + (e1 - e0) * f * z4;
+}
+
+template <class D>
+SXC_HD D pbe_c_pol(D ra, D rb, D sigma, const XcParams& p) {
+  D rho = ra + rb;
+  D rs = rs_of(rho);
+  D zeta = (ra - rb) / rho;
+  D opz = 1.0 + zeta, omz = 1.0 - zeta;
+  D eps = pw92_eps_pol(rs, zeta, spin_f(opz, omz), p);
+  // phi = ((1+zeta)^(2/3) + (1-zeta)^(2/3)) / 2, arguments kept away from 0 (its slope diverges there)
+  if (opz.v < 1e-12) opz.v = 1e-12;
+  if (omz.v < 1e-12) omz.v = 1e-12;
+  D phi = 0.5 * (dpow(opz, 2.0 / 3.0) + dpow(omz, 2.0 / 3.0));
+  D phi3 = phi * phi * phi;
+  const double kt = M_PI / (16.0 * cbrt(3.0 * M_PI * M_PI));
+  D t2 = kt * sigma / (phi * phi * dpow(rho, 7.0 / 3.0));
+  return rho * (eps + pbe_H(eps, t2, phi3, p));
 }
 
 // Functional ids as in serenity_xc.h.
@@ -158,6 +358,59 @@ SXC_HD D2 eval_functional(int f, D2 rho, D2 sigma, const XcParams& p) {
   }
 }
 
+// Spin-polarised evaluation: v = (rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb) as D5
+// variables; a_on / b_on say whether each spin density is above kTinyDensity.
+SXC_HD D5 eval_functional_pol(int f, D5 ra, D5 rb, D5 saa, D5 sab, D5 sbb, bool a_on, bool b_on,
+                              const XcParams& p) {
+  D5 sigma = saa + 2.0 * sab + sbb;
+  D5 e = mkc<5>(0.0);
+  // exchange-type parts by spin scaling, E[rho_a, rho_b] = (E[2 rho_a] + E[2 rho_b]) / 2
+  // with 4 sigma_ss as the gradient invariant
+  auto xpart = [&](int kind) {
+    D5 out = mkc<5>(0.0);
+    if (a_on) {
+      D5 r2 = 2.0 * ra, s4 = 4.0 * saa;
+      out = out + 0.5 * (kind == 0 ? slater_x(r2) : kind == 1 ? tf_k(r2) : pbe_x(r2, s4, p, kind == 2 ? p.pbe_mu : p.pbe_mu_libxc));
+    }
+    if (b_on) {
+      D5 r2 = 2.0 * rb, s4 = 4.0 * sbb;
+      out = out + 0.5 * (kind == 0 ? slater_x(r2) : kind == 1 ? tf_k(r2) : pbe_x(r2, s4, p, kind == 2 ? p.pbe_mu : p.pbe_mu_libxc));
+    }
+    return out;
+  };
+  // a fully polarised point (one spin density zero) keeps the exact zeta = +-1 limits
+  D5 ra_c = a_on ? ra : mkc<5>(0.0), rb_c = b_on ? rb : mkc<5>(0.0);
+  switch (f) {
+    case 0:
+      e = xpart(0) + vwn5_c_pol(ra_c, rb_c, p);
+      break;
+    case 1:
+      e = xpart(2) + pbe_c_pol(ra_c, rb_c, sigma, p);
+      break;
+    case 2:
+      e = xpart(1);
+      break;
+    case 3:
+      e = xpart(0);
+      break;
+    case 4:
+      e = vwn5_c_pol(ra_c, rb_c, p);
+      break;
+    case 5:
+      e = xpart(2);
+      break;
+    case 6:
+      e = pbe_c_pol(ra_c, rb_c, sigma, p);
+      break;
+    case 7:
+      e = xpart(3) + pbe_c_pol(ra_c, rb_c, sigma, p);
+      break;
+    default:
+      break;
+  }
+  return e;
+}
+
 // Densities below this value contribute nothing (energy and potential zero).
 constexpr double kTinyDensity = 1e-14;
 
@@ -169,14 +422,39 @@ SXC_HD double xc_point(int f, double rho, double gx, double gy, double gz, const
     return 0.0;
   }
   const double sigma = gx * gx + gy * gy + gz * gz;
-  D2 r = {rho, 1.0, 0.0};
-  D2 s = {sigma, 0.0, 1.0};
+  D2 r = var<2>(rho, 0);
+  D2 s = var<2>(sigma, 1);
   D2 e = eval_functional(f, r, s, p);
-  *v = e.dr;
+  *v = e.d[0];
   if (functional_is_gga(f)) {
-    g[0] = 2.0 * e.ds * gx;
-    g[1] = 2.0 * e.ds * gy;
-    g[2] = 2.0 * e.ds * gz;
+    g[0] = 2.0 * e.d[1] * gx;
+    g[1] = 2.0 * e.d[1] * gy;
+    g[2] = 2.0 * e.d[1] * gz;
+  }
+  return e.v;
+}
+
+// One grid point, spin-polarised: ga / gb are the spin density gradients (3 each);
+// writes v[0..1] = d eps/d rho_a, d eps/d rho_b and (GGA) g[0..2] = d eps/d grad rho_a,
+// g[3..5] = d eps/d grad rho_b.  Zero when the total density is below kTinyDensity.
+SXC_HD double xc_point_pol(int f, double ra, double rb, const double* ga, const double* gb, const XcParams& p,
+                           double* v, double* g) {
+  for (int i = 0; i < 6; ++i) g[i] = 0.0;
+  v[0] = v[1] = 0.0;
+  const bool a_on = ra >= kTinyDensity, b_on = rb >= kTinyDensity;
+  if (!(ra + rb >= kTinyDensity) || !(a_on || b_on)) return 0.0;
+  const double saa = ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2];
+  const double sbb = gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2];
+  const double sab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+  D5 e = eval_functional_pol(f, var<5>(a_on ? ra : 0.0, 0), var<5>(b_on ? rb : 0.0, 1), var<5>(saa, 2),
+                             var<5>(sab, 3), var<5>(sbb, 4), a_on, b_on, p);
+  v[0] = e.d[0];
+  v[1] = e.d[1];
+  if (functional_is_gga(f)) {
+    for (int c = 0; c < 3; ++c) {
+      g[c] = 2.0 * e.d[2] * ga[c] + e.d[3] * gb[c];
+      g[3 + c] = 2.0 * e.d[4] * gb[c] + e.d[3] * ga[c];
+    }
   }
   return e.v;
 }

@@ -92,6 +92,12 @@ void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npt
                        double* v, double* g, double* vs, double* gw, double* e_blk,
                        const double* rho_sub = nullptr, const double* grad_sub = nullptr);
 
+// Unrestricted functional: spin densities/gradients (3 x npts each) in; w v_s, w g_s and
+// block energies out.
+void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* ra,
+                           const double* ga, const double* rb, const double* gb, const double* w, double* vsa,
+                           double* gwa, double* vsb, double* gwb, double* e_blk);
+
 // Sub-tile Fock kernel: 8 wavefronts, slot class sc (kFockS[sc] sub-tiles per wavefront).
 template <typename T>
 void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,

@@ -535,6 +535,50 @@ __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm,
   if (t == 0) e_blk[b] = eb;
 }
 
+// Unrestricted functional on 128-point blocks: spin densities and gradients in,
+// w v_s and w g_s (the Fock kernel's inputs) per spin and the block energies out.
+// The block is skipped when both spin densities are negligible (sum |rho_s| <
+// B 1e-12 for each spin), as the reference's for_spin test does.
+__global__ __launch_bounds__(128) void functional_pol_kernel(int func, XcParams prm, int64_t npts, int bs,
+                                                             const double* __restrict__ ra,
+                                                             const double* __restrict__ ga,
+                                                             const double* __restrict__ rb,
+                                                             const double* __restrict__ gb,
+                                                             const double* __restrict__ w, double* __restrict__ vsa,
+                                                             double* __restrict__ gwa, double* __restrict__ vsb,
+                                                             double* __restrict__ gwb, double* __restrict__ e_blk) {
+  __shared__ double red[128];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int64_t first = (int64_t)b * bs;
+  const int B = (int)min((int64_t)bs, npts - first);
+  const int64_t q = first + t;
+  const bool in = t < B;
+  const bool gga = functional_is_gga(func);
+  const double a = in ? ra[q] : 0.0, bb = in ? rb[q] : 0.0;
+  double xa[3] = {0.0, 0.0, 0.0}, xb[3] = {0.0, 0.0, 0.0};
+  if (in && gga)
+    for (int c = 0; c < 3; ++c) {
+      xa[c] = ga[c * npts + q];
+      xb[c] = gb[c * npts + q];
+    }
+  const bool skip_a = block_sum_128(fabs(a), red) < B * 1e-12;
+  const bool skip_b = block_sum_128(fabs(bb), red) < B * 1e-12;
+  double e = 0.0, v[2] = {0.0, 0.0}, g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (in && !(skip_a && skip_b)) e = xc_point_pol(func, a, bb, xa, xb, prm, v, g);
+  const double wq = in ? w[q] : 0.0;
+  if (in) {
+    vsa[q] = wq * v[0];
+    vsb[q] = wq * v[1];
+    if (gga)
+      for (int c = 0; c < 3; ++c) {
+        gwa[c * npts + q] = wq * g[c];
+        gwb[c * npts + q] = wq * g[3 + c];
+      }
+  }
+  const double eb = block_sum_128(wq * e, red);
+  if (t == 0) e_blk[b] = eb;
+}
+
 // Sub-tile Fock contraction.  One workgroup of 8 wavefronts per work item; the
 // item's 16 x 16 AO sub-tiles (i, j) are dealt to the wavefronts round-robin,
 // S per wavefront, each accumulating S_ij = phi_i^T G_j + G_i^T phi_j over the
@@ -874,6 +918,15 @@ void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npt
   if (nb <= 0) return;
   hipLaunchKernelGGL(functional_kernel, dim3(nb), dim3(128), 0, s, func, prm, npts, bs, rho, grad, blocks, part, w,
                      eps, v, g, vs, gw, e_blk, rho_sub, grad_sub);
+}
+
+void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* ra,
+                           const double* ga, const double* rb, const double* gb, const double* w, double* vsa,
+                           double* gwa, double* vsb, double* gwb, double* e_blk) {
+  const int nb = (int)((npts + bs - 1) / bs);
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(functional_pol_kernel, dim3(nb), dim3(128), 0, s, func, prm, npts, bs, ra, ga, rb, gb, w, vsa,
+                     gwa, vsb, gwb, e_blk);
 }
 
 template <typename T, int S>

@@ -223,6 +223,9 @@ struct xc_bfogc {
   DBuf<GradItem> d_grad_work;
   DBuf<double> d_gacc, d_gout;
   DBuf<int> d_ao_atom;
+  // unrestricted builds (allocated on first use): beta density, potential, P and F
+  DBuf<double> d_rho_b, d_grad_b, d_vs_b, d_gw_b, d_Pb, d_Fb;
+  DBuf<float> d_P32b;
   bool ecache_valid[kNumFunctionals] = {};
   // serving: a build of this controller is in flight on a server slot; captured builds
   bool in_flight = false;
@@ -259,11 +262,14 @@ void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad, const 
 }
 
 template <typename T>
-void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr, const Lanes& L) {
+void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr, const Lanes& L, const double* vs = nullptr,
+              const double* gw = nullptr) {
   const int f = sizeof(T) == 8 ? 1 : 0;
+  if (!vs) vs = bf->d_vs.p;
+  if (!gw) gw = bf->d_gw.p;
   over_lanes(L, kFockNumS, [&](int c) { return (int)bf->fock_items[f][c].size(); }, [&](int c, hipStream_t s) {
     launch_fock_sub<T>(s, c, bf->d_fock_items[f][c].p, (int)bf->fock_items[f][c].size(),
-                       bf->d_blocks.p, bf->d_ao_idx.p, tiles, bf->d_vs.p, bf->d_gw.p, gga, thr, bf->d_U.p,
+                       bf->d_blocks.p, bf->d_ao_idx.p, tiles, vs, gw, gga, thr, bf->d_U.p,
                        bf->basis->nao, bf->grid->npts);
   });
 }
@@ -1049,6 +1055,71 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   HIPX(hipGetLastError());
 }
 
+/* Buffers of unrestricted builds, each allocated once (never reallocated: pointers to
+ * them may already be in enqueued work). */
+static void ensure_pol_buffers(xc_bfogc* bf, bool fp32) {
+  xc_ctx* ctx = bf->ctx;
+  const int n = bf->basis->nao;
+  const int64_t N = bf->grid->npts;
+  if (!bf->d_rho_b.p) bf->d_rho_b.alloc(ctx, N);
+  if (!bf->d_grad_b.p) bf->d_grad_b.alloc(ctx, 3 * N);
+  if (!bf->d_vs_b.p) bf->d_vs_b.alloc(ctx, N);
+  if (!bf->d_gw_b.p) bf->d_gw_b.alloc(ctx, 3 * N);
+  if (!bf->d_Pb.p) bf->d_Pb.alloc(ctx, (size_t)n * n);
+  if (!bf->d_Fb.p) bf->d_Fb.alloc(ctx, (size_t)n * n);
+  if (fp32 && !bf->d_P32b.p) bf->d_P32b.alloc(ctx, (size_t)n * n);
+}
+
+/* Unrestricted build (FuncPotential<UNRESTRICTED>): spin densities from P_a and P_b,
+ * the spin-polarised functional, and F_a, F_b from w v_s, w g_s (one Fock pass per
+ * spin, each with its own block skip); E_xc left in d_energy. */
+static void enqueue_build_pol(xc_bfogc* bf, int functional, int precision, double thr, const double* dPa,
+                              const double* dPb, double* dFa, double* dFb, const Lanes& L, double* d_energy) {
+  xc_ctx* ctx = bf->ctx;
+  hipStream_t s = L.main;
+  require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
+  require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
+  require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+  const bool gga = functional_is_gga(functional);
+  require(!gga || bf->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
+  require(precision == XC_PREC_FP64 || bf->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+  require(precision == XC_PREC_FP32 || bf->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
+  const int n = bf->basis->nao;
+  const int64_t N = bf->grid->npts;
+  const int nxc = (int)((N + kTile - 1) / kTile);
+  ensure_pol_buffers(bf, precision == XC_PREC_FP32);
+  // spin densities: one density pass per spin through the shared partial buffer
+  const double* Ps[2] = {dPa, dPb};
+  double* rho[2] = {bf->d_rho.p, bf->d_rho_b.p};
+  double* grad[2] = {bf->d_grad.p, bf->d_grad_b.p};
+  float* P32[2] = {bf->d_P32.p, bf->d_P32b.p};
+  for (int sp = 0; sp < 2; ++sp) {
+    if (precision == XC_PREC_FP64) {
+      run_density<double>(bf, bf->d_t64.p, Ps[sp], gga ? 1 : 0, L);
+    } else {
+      launch_to_float(s, Ps[sp], P32[sp], (int64_t)n * n);
+      run_density<float>(bf, bf->d_t32.p, P32[sp], gga ? 1 : 0, L);
+    }
+    launch_reduce_partials(s, bf->d_blocks.p, bf->bs, N, bf->d_part.p, gga ? 1 : 0, rho[sp], grad[sp]);
+  }
+  launch_functional_pol(s, functional, ctx->prm, N, kTile, bf->d_rho.p, bf->d_grad.p, bf->d_rho_b.p,
+                        bf->d_grad_b.p, bf->grid->d_w.p, bf->d_vs.p, bf->d_gw.p, bf->d_vs_b.p, bf->d_gw_b.p,
+                        bf->d_eblk.p);
+  const double* vs[2] = {bf->d_vs.p, bf->d_vs_b.p};
+  const double* gw[2] = {bf->d_gw.p, bf->d_gw_b.p};
+  double* Fs[2] = {dFa, dFb};
+  for (int sp = 0; sp < 2; ++sp) {
+    launch_fill(s, bf->d_U.p, (int64_t)n * n, 0.0);
+    if (precision == XC_PREC_FP64)
+      run_fock<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr, L, vs[sp], gw[sp]);
+    else
+      run_fock<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr, L, vs[sp], gw[sp]);
+    launch_finalize(s, bf->d_U.p, Fs[sp], n, 0);
+  }
+  launch_sum(s, bf->d_eblk.p, nxc, d_energy);
+  HIPX(hipGetLastError());
+}
+
 /* Non-additive build: rho_sup = rho_act + sum_env rho_env on the shared grid,
  * v = v[rho_sup] - v[rho_act] integrated with the active basis,
  * E = E[rho_sup] - E[rho_act] - sum_env E[rho_env] (left in ctx->d_energy). */
@@ -1355,6 +1426,41 @@ int xc_geom_gradient(xc_bfogc* bf, int functional, const double* P, int natoms, 
     HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     if (energy) *energy = ctx->last_energy;
+  });
+}
+
+int xc_fock_build_unrestricted(xc_bfogc* bf, int functional, int precision, double thr, const double* Pa,
+                               const double* Pb, double* Fa, double* Fb, double* energy) {
+  return guarded([&] {
+    require(bf && Pa && Pb && Fa && Fb, XC_ERR_ARG, "xc_fock_build_unrestricted: null argument");
+    xc_ctx* ctx = bf->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const int n = bf->basis->nao;
+    ensure_pol_buffers(bf, precision == XC_PREC_FP32);
+    HIPX(hipMemcpyAsync(bf->d_P.p, Pa, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPX(hipMemcpyAsync(bf->d_Pb.p, Pb, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
+    enqueue_build_pol(bf, functional, precision, thr, bf->d_P.p, bf->d_Pb.p, bf->d_F.p, bf->d_Fb.p, ctx->lanes,
+                      ctx->d_energy);
+    HIPX(hipMemcpyAsync(Fa, bf->d_F.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipMemcpyAsync(Fb, bf->d_Fb.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+    if (energy) *energy = ctx->last_energy;
+  });
+}
+
+int xc_fock_build_unrestricted_device(xc_bfogc* bf, int functional, int precision, double thr, const double* dPa,
+                                      const double* dPb, double* dFa, double* dFb, double* energy) {
+  return guarded([&] {
+    require(bf && dPa && dPb && dFa && dFb, XC_ERR_ARG, "xc_fock_build_unrestricted_device: null argument");
+    xc_ctx* ctx = bf->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    enqueue_build_pol(bf, functional, precision, thr, dPa, dPb, dFa, dFb, ctx->lanes, ctx->d_energy);
+    if (energy) {
+      HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIPX(hipStreamSynchronize(ctx->stream));
+      *energy = ctx->last_energy;
+    }
   });
 }
 

@@ -481,6 +481,25 @@ class FuncPotential:
             raise XCError(4, "FuncPotential: getMatrix has not been called")
         return self._energy
 
+    def getMatrixUnrestricted(self, Pa, Pb):
+        """FuncPotential<UNRESTRICTED>::getMatrix: (F_alpha, F_beta) of the spin density matrices."""
+        n = self.bfogc.getNBasisFunctions()
+        Pa = np.asfortranarray(Pa, dtype=np.float64)
+        Pb = np.asfortranarray(Pb, dtype=np.float64)
+        if Pa.shape != (n, n) or Pb.shape != (n, n):
+            raise XCError(1, "FuncPotential: spin density matrices are %s and %s, basis has %d functions"
+                          % (Pa.shape, Pb.shape, n))
+        Fa = np.zeros((n, n), order="F")
+        Fb = np.zeros((n, n), order="F")
+        E = C.c_double()
+        check(_lib.lib().xc_fock_build_unrestricted(self.bfogc.handle, FUNCTIONALS[self.functional],
+                                                    PRECISIONS[self.precision], float(self.blockAveThreshold),
+                                                    Pa.ctypes.data_as(_lib.dp), Pb.ctypes.data_as(_lib.dp),
+                                                    Fa.ctypes.data_as(_lib.dp), Fb.ctypes.data_as(_lib.dp),
+                                                    C.byref(E)))
+        self._energy = E.value
+        return Fa, Fb
+
     def getGeomGradients(self, P, ao_atom=None, natoms=None):
         """XC part of the nuclear gradient (natoms x 3) of the density P at the fixed grid
         (FuncPotential::getGeomGradients).  The controller needs highestDerivative 1 (LDA)

@@ -176,6 +176,17 @@ def fixtures():
         out[tag + "_xyz_angstrom"] = pos
         out[tag + "_energy_ids"] = np.array(sorted(e), dtype=np.int32)
         out[tag + "_energy_values"] = np.array([e[k] for k in sorted(e)])
+    # open shell: triplet O2 / 6-31G / PBE (LibXC), unrestricted density and Fock matrices
+    d = "TestSystem_O2_TRIP_6_31G_PBE_TDA"
+    for spin in ("alpha", "beta"):
+        out["o2_P" + spin[0]] = h5_dataset(os.path.join(base, d, d + ".dmat.unres.h5"), "densityMatrix_" + spin)
+        out["o2_F" + spin[0]] = h5_dataset(os.path.join(base, d, d + ".FockMatrix.unres.h5"), "FockMatrix_" + spin)
+    sym, pos = xyz(os.path.join(base, d, d + ".xyz"))
+    e = energies(os.path.join(base, d, d + ".energies.unres"))
+    out["o2_Z"] = np.array([{"O": 8}[s] for s in sym], dtype=np.int32)
+    out["o2_xyz_angstrom"] = pos
+    out["o2_energy_ids"] = np.array(sorted(e), dtype=np.int32)
+    out["o2_energy_values"] = np.array([e[k] for k in sorted(e)])
     np.savez(os.path.join(HERE, "fixtures.npz"), **out)
 
 
@@ -227,6 +238,7 @@ def basis_entry(src, element, out):
 def sc_basis():
     basis_entry("CC-PV5Z", "sc", "SC-CC-PV5Z")
     basis_entry("6-31GS", "h", "H-6-31GS")
+    basis_entry("6-31G", "o", "O-6-31G")
 
 
 if __name__ == "__main__":
@@ -239,4 +251,4 @@ if __name__ == "__main__":
         print(name, len(data.get("entries", data.get("F", data.get("cases", [])))))
     fixtures()
     sc_basis()
-    print("fixtures.npz, SC-CC-PV5Z, H-6-31GS written")
+    print("fixtures.npz, SC-CC-PV5Z, H-6-31GS, O-6-31G written")
