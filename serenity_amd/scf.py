@@ -6,7 +6,7 @@ Mirrors the callers of FuncPotential in Serenity:
   and the Coulomb matrix J (exact four-centre integrals or density fitting,
   Serenity's default ``densFitJ = RI`` with the RI_J_WEIGEND auxiliary basis,
   Settings.h:128-138; RI_J_IntegralController.cpp:41-121).
-* ``perform`` -- ``Scf<RESTRICTED>::perform`` (src/scf/Scf.cpp:38-148): build
+* ``perform`` / ``perform_unrestricted`` -- ``Scf<SCFMode>::perform`` (src/scf/Scf.cpp:38-148): build
   F = h + J[P] + V_xc[P] from the current density, extrapolate it by DIIS,
   diagonalise in the canonically orthogonalised basis, occupy the lowest
   orbitals, repeat until the energy, the density and the DIIS error are
@@ -237,6 +237,59 @@ def perform(ints, nocc, xc_build, P0=None, max_cycles=100, energy_thr=1e-10, rms
                 t_xc=t_xc, t_j=t_j)
 
 
+def perform_unrestricted(ints, nalpha, nbeta, xc_build, P0=None, max_cycles=150, energy_thr=1e-10, rmsd_thr=1e-9,
+                         diis_thr=1e-8, diis_store=10, log=None):
+    """Scf<UNRESTRICTED>::perform.  xc_build(Pa, Pb) -> (E_xc, Va, Vb).  F_s = h + J[Pa + Pb] + V_s;
+    one DIIS over both spins (stacked commutator errors, common coefficients)."""
+    say = log or (lambda *a: None)
+    X = canonical_orthogonalization(ints.S)
+    S, h = ints.S, ints.h
+    if P0 is None:
+        _, Cp = np.linalg.eigh(X.T @ h @ X)
+        C0 = X @ Cp
+        Pa, Pb = C0[:, :nalpha] @ C0[:, :nalpha].T, C0[:, :nbeta] @ C0[:, :nbeta].T
+    else:
+        Pa, Pb = (np.array(p, dtype=np.float64) for p in P0)
+    diis = Diis(diis_store)
+    e_old = None
+    hist = []
+    for cycle in range(1, max_cycles + 1):
+        P = Pa + Pb
+        J = ints.coulomb(P)
+        e_xc, Va, Vb = xc_build(Pa, Pb)
+        Fa, Fb = h + J + Va, h + J + Vb
+        energy = ints.e_nuc + float(np.vdot(P, h)) + 0.5 * float(np.vdot(P, J)) + e_xc
+        errs = []
+        for F, Ps in ((Fa, Pa), (Fb, Pb)):
+            FPS = F @ Ps @ S
+            errs.append(X.T @ (FPS - FPS.T) @ X)
+        err = np.concatenate([e.ravel() for e in errs])
+        err_norm = float(np.sqrt(np.mean(err * err)))
+        Fx = diis.extrapolate(np.stack([Fa, Fb]), err)
+        new = []
+        for F, no in ((Fx[0], nalpha), (Fx[1], nbeta)):
+            _, Cp = np.linalg.eigh(X.T @ F @ X)
+            Co = (X @ Cp)[:, :no]
+            new.append(Co @ Co.T)
+        rmsd = float(np.sqrt(0.5 * (np.mean((new[0] - Pa) ** 2) + np.mean((new[1] - Pb) ** 2))))
+        de = None if e_old is None else energy - e_old
+        hist.append(dict(cycle=cycle, energy=energy, dE=de, rmsd=rmsd, diis_error=err_norm, e_xc=e_xc))
+        say("cycle %3d  E = %.12f  dE = %s  rmsd = %.2e  diis = %.2e" %
+            (cycle, energy, "      -     " if de is None else "%.2e" % de, rmsd, err_norm))
+        if de is not None and abs(de) < energy_thr and rmsd < rmsd_thr and err_norm < diis_thr:
+            return dict(energy=energy, Pa=Pa, Pb=Pb, cycles=cycle, converged=True, history=hist, e_xc=e_xc)
+        e_old = energy
+        Pa, Pb = new
+    return dict(energy=energy, Pa=Pa, Pb=Pb, cycles=max_cycles, converged=False, history=hist, e_xc=e_xc)
+
+
+def gpu_xc_build_unrestricted(pot):
+    def build(Pa, Pb):
+        Fa, Fb = pot.getMatrixUnrestricted(Pa, Pb)
+        return pot.getEnergy(), Fa, Fb
+    return build
+
+
 def total_energy(ints, P, e_xc):
     """E = E_nuc + tr(P h) + 1/2 tr(P J[P]) + E_xc[P]."""
     J = ints.coulomb(P)
@@ -264,7 +317,8 @@ class ScfTask:
 
     def __init__(self, ctx, Z, coords_bohr, basis="DEF2-SVP", functional="LDA", ri=True, aux="RI_J_WEIGEND",
                  accuracy=4, small_accuracy=2, spherical=True, schwarz=1e-14, fp32_tiles=True, log=None,
-                 aux_spherical=None, radial_threshold=1e-9, block_ave_threshold=1e-11, highest_derivative=1):
+                 aux_spherical=None, radial_threshold=1e-9, block_ave_threshold=1e-11, highest_derivative=1,
+                 spin=0, unrestricted=False):
         self.ctx = ctx
         self.Z = np.ascontiguousarray(Z, dtype=np.int32)
         self.X = np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
@@ -279,7 +333,12 @@ class ScfTask:
         self.block_ave_threshold = block_ave_threshold
         self.highest_derivative = highest_derivative
         self.ints = ScfIntegrals(self.basis, self.Z, self.X, self.auxbasis, schwarz)
-        self.nocc = int(self.Z.sum()) // 2
+        nel = int(self.Z.sum())
+        if (nel - spin) % 2 or spin < 0 or spin > nel:
+            raise xc.XCError(1, "ScfTask: %d electrons cannot have %d unpaired" % (nel, spin))
+        self.nocc = nel // 2
+        self.nalpha, self.nbeta = (nel + spin) // 2, (nel - spin) // 2
+        self.unrestricted = bool(unrestricted or spin)
         self.grid_small = xc.molecular_grid(self.Z, self.X, accuracy=small_accuracy, ctx=ctx)
         self.grid_default = (self.grid_small if accuracy == small_accuracy
                              else xc.molecular_grid(self.Z, self.X, accuracy=accuracy, ctx=ctx))
@@ -316,8 +375,27 @@ class ScfTask:
         Co = (X @ Cp)[:, :self.nocc]
         return e, 2.0 * Co @ Co.T
 
+    def run_unrestricted(self, precision="FP64", max_cycles=150):
+        """Scf<UNRESTRICTED> on the small grid, final energy of the converged spin densities on the
+        default grid (no orbital update is reported)."""
+        t0 = time.perf_counter()
+        r = perform_unrestricted(self.ints, self.nalpha, self.nbeta,
+                                 gpu_xc_build_unrestricted(self._potential("small", precision)),
+                                 max_cycles=max_cycles, log=self.say,
+                                 **(MIXED_THRESHOLDS if precision == "FP32" else {}))
+        pot = self._potential("default", precision)
+        pot.getMatrixUnrestricted(r["Pa"], r["Pb"])
+        energy = total_energy(self.ints, r["Pa"] + r["Pb"], pot.getEnergy())
+        return dict(mode="UNRESTRICTED " + precision, energy=energy, energy_small_grid=r["energy"],
+                    cycles=r["cycles"], converged=r["converged"], Pa=r["Pa"], Pb=r["Pb"],
+                    wall_s=round(time.perf_counter() - t0, 3))
+
     def run(self, mode="FP64", max_cycles=100, P0=None):
         """mode "FP64" or "MIXED"; returns energies, cycle counts and timings."""
+        if self.unrestricted:
+            if mode not in ("FP64", "FP32"):
+                raise xc.XCError(1, "ScfTask: unrestricted runs take the precision FP64 or FP32")
+            return self.run_unrestricted(mode, max_cycles)
         t0 = time.perf_counter()
         out = dict(mode=mode, nao=self.basis.nao, naux=None if self.auxbasis is None else self.auxbasis.nao,
                    npts_small=self.grid_small.getNGridPoints(), npts_default=self.grid_default.getNGridPoints())

@@ -59,6 +59,23 @@ def test_water_sto3g_pbe_fp64_and_mixed(ctx, functional, ref):
     assert abs(mixed["energy"] - fp64["energy"]) < 1e-8
 
 
+def test_o2_triplet_unrestricted_pbe(ctx):
+    """Unrestricted SCF with the GPU unrestricted build: the reference's stored energy total."""
+    import common
+    from test_unrestricted import o2_system
+    Z, X, b, _, _, _, _, E = o2_system()
+    t = scf.ScfTask(ctx, Z, X, basis=common.GOLDEN + "/O-6-31G", functional="PBE", ri=False, accuracy=7,
+                    small_accuracy=7, spin=2)
+    fp64 = t.run("FP64")
+    fp32 = t.run("FP32")
+    t.close()
+    ref = E[3] + E[5] + E[6] + E[202]
+    print("O2 triplet UKS: FP64 %.10f (%d cycles) FP32 %.10f ref %.10f" % (fp64["energy"], fp64["cycles"],
+                                                                          fp32["energy"], ref))
+    assert fp64["converged"] and abs(fp64["energy"] - ref) < 1e-8
+    assert abs(fp32["energy"] - fp64["energy"]) < 1e-5
+
+
 def test_water8_svp_lda_full_scf(ctx):
     """(H2O)8/def2-SVP/LDA with RI-J: the full SCF of the first benchmark configuration."""
     Z, X = W.geometry(W.CONFIGS[1]["system"])

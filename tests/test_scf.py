@@ -14,6 +14,9 @@ energy from the converged density, ScfTask.cpp:127-153, 358-359):
 * water/STO-3G PBE with RI-J, qcserenity/tests/test_tasks.py:50-76: -75.2218044178 with
   LibXC's PBE and -75.22180876683022 with XCFun's (the unrestricted run of a closed shell;
   places=7 there, i.e. 5e-8).
+* triplet O2 / 6-31G / PBE, unrestricted, exact J, accuracy-7 grids
+  (data/testresources/TestSystem_O2_TRIP_6_31G_PBE_TDA): the total of the stored energy
+  components (ids 3 + 5 + 6 + 202) -150.1286472735, to 1e-8.
 * ethane/def2-SVP Hartree-Fock with exact four-centre J and K, Cartesian and spherical d
   shells, src/tasks/ScfTask_test.cpp:43-88: -79.1786812939 and -79.1728833551 at 1e-8 --
   no XC involved, so this pins the integrals (s, p, d; both AO conventions) and the SCF.
@@ -113,6 +116,23 @@ def test_ethane_svp_hartree_fock(spherical, ref):
     r = scf.perform(ints, 9, None, exchange=1.0)
     assert r["converged"]
     assert abs(r["energy"] - ref) < 1e-8
+
+
+def test_o2_triplet_unrestricted_pbe():
+    from test_unrestricted import o2_system
+    Z, X, b, _, _, _, _, E = o2_system()
+    ints = scf.ScfIntegrals(b, Z, X)
+    g = xc.molecular_grid(Z, X, accuracy=7)
+
+    def build(Pa, Pb):
+        Fa, Fb, e = O.fock_build_pol(b, g.points, g.weights, Pa, Pb, "PBE")
+        return e, Fa, Fb
+    r = scf.perform_unrestricted(ints, 9, 7, build)
+    assert r["converged"]
+    ref = E[3] + E[5] + E[6] + E[202]
+    assert abs(r["energy"] - ref) < 1e-8
+    S = ints.S
+    assert abs(np.trace(r["Pa"] @ S) - 9) < 1e-10 and abs(np.trace(r["Pb"] @ S) - 7) < 1e-10
 
 
 def test_nuclear_repulsion():
