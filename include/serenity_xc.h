@@ -244,6 +244,10 @@ int xc_fock_build_unrestricted_device(xc_bfogc* bfogc, int functional, int preci
  * of P (may be NULL). */
 int xc_geom_gradient(xc_bfogc* bfogc, int functional, const double* P, int natoms, const int* ao_atom,
                      double* grad, double* energy);
+/* The unrestricted form (FuncPotential<UNRESTRICTED>::getGeomGradients): the same sum
+ * per spin, 2 P^s_mu,nu with the spin's potential w v_s, w g_s. */
+int xc_geom_gradient_unrestricted(xc_bfogc* bfogc, int functional, const double* P_alpha, const double* P_beta,
+                                  int natoms, const int* ao_atom, double* grad, double* energy);
 int xc_last_energy(xc_ctx* ctx, double* energy);
 
 /* NAddFuncPotential::getMatrix + getEnergy, restricted (src/potentials/

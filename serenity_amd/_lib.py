@@ -83,6 +83,7 @@ def lib():
     _sig("xc_bfogc_block_data", C.c_int, vp, C.c_int, dp, dp, dp, dp, ip, ip)
     _sig("xc_bfogc_block_hessian", C.c_int, vp, C.c_int, dp)
     _sig("xc_geom_gradient", C.c_int, vp, C.c_int, dp, C.c_int, ip, dp, dp)
+    _sig("xc_geom_gradient_unrestricted", C.c_int, vp, C.c_int, dp, dp, C.c_int, ip, dp, dp)
     _sig("xc_fock_build_unrestricted", C.c_int, vp, C.c_int, C.c_int, C.c_double, dp, dp, dp, dp, dp)
     _sig("xc_fock_build_unrestricted_device", C.c_int, vp, C.c_int, C.c_int, C.c_double, vp, vp, vp, vp, dp)
     _sig("xc_density", C.c_int, vp, dp, dp, dp, ip)

@@ -1377,36 +1377,40 @@ int xc_fock_build_device(xc_bfogc* bf, int functional, int precision, double thr
   });
 }
 
-int xc_geom_gradient(xc_bfogc* bf, int functional, const double* P, int natoms, const int* ao_atom, double* grad,
-                     double* energy) {
-  return guarded([&] {
-    require(bf && P && grad && ao_atom && natoms > 0, XC_ERR_ARG, "xc_geom_gradient: bad arguments");
-    require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
-    require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
-    const bool gga = functional_is_gga(functional);
-    require(bf->deriv >= (gga ? 2 : 1), XC_ERR_STATE,
-            gga ? "xc_geom_gradient: a GGA gradient needs AO second derivatives (highest derivative 2)"
-                : "xc_geom_gradient: needs AO gradients (highest derivative 1)");
-    require(bf->d_t64.p, XC_ERR_STATE, "xc_geom_gradient: needs FP64 tiles");
-    xc_ctx* ctx = bf->ctx;
-    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    hipStream_t s = ctx->stream;
-    const int n = bf->basis->nao;
-    const int64_t N = bf->grid->npts;
-    const int nxc = (int)((N + kTile - 1) / kTile);
-    for (int i = 0; i < n; ++i)
-      require(ao_atom[i] >= 0 && ao_atom[i] < natoms, XC_ERR_INPUT, "xc_geom_gradient: AO atom index out of range");
-    if (bf->grad_work.empty()) {
-      for (int b = 0; b < bf->nblocks; ++b)
-        for (int c0 = 0; c0 < bf->blocks[b].npad; c0 += kGradCols) bf->grad_work.push_back(GradItem{b, c0});
-      if (!bf->grad_work.empty()) bf->d_grad_work.upload(ctx, bf->grad_work.data(), bf->grad_work.size());
-      bf->d_gacc.alloc(ctx, (size_t)3 * std::max(n, 1));
-      bf->d_ao_atom.alloc(ctx, (size_t)std::max(n, 1));
-    }
-    if ((int64_t)bf->d_gout.n < 3 * (int64_t)natoms) bf->d_gout.alloc(ctx, (size_t)3 * natoms);
-    HIPX(hipMemcpyAsync(bf->d_ao_atom.p, ao_atom, sizeof(int) * n, hipMemcpyHostToDevice, s));
-    HIPX(hipMemcpyAsync(bf->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, s));
-    // density and functional as in a Fock build (FP64), then the gradient contractions
+namespace {
+// XC gradient of one (restricted: Pb == nullptr) or two spin density matrices (host).
+void geom_gradient(xc_bfogc* bf, int functional, const double* Pa, const double* Pb, int natoms, const int* ao_atom,
+                   double* grad, double* energy) {
+  require(bf && Pa && grad && ao_atom && natoms > 0, XC_ERR_ARG, "xc_geom_gradient: bad arguments");
+  require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
+  require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
+  const bool gga = functional_is_gga(functional);
+  require(bf->deriv >= (gga ? 2 : 1), XC_ERR_STATE,
+          gga ? "xc_geom_gradient: a GGA gradient needs AO second derivatives (highest derivative 2)"
+              : "xc_geom_gradient: needs AO gradients (highest derivative 1)");
+  require(bf->d_t64.p, XC_ERR_STATE, "xc_geom_gradient: needs FP64 tiles");
+  xc_ctx* ctx = bf->ctx;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  hipStream_t s = ctx->stream;
+  const int n = bf->basis->nao;
+  const int64_t N = bf->grid->npts;
+  const int nxc = (int)((N + kTile - 1) / kTile);
+  for (int i = 0; i < n; ++i)
+    require(ao_atom[i] >= 0 && ao_atom[i] < natoms, XC_ERR_INPUT, "xc_geom_gradient: AO atom index out of range");
+  if (bf->grad_work.empty()) {
+    for (int b = 0; b < bf->nblocks; ++b)
+      for (int c0 = 0; c0 < bf->blocks[b].npad; c0 += kGradCols) bf->grad_work.push_back(GradItem{b, c0});
+    if (!bf->grad_work.empty()) bf->d_grad_work.upload(ctx, bf->grad_work.data(), bf->grad_work.size());
+    bf->d_gacc.alloc(ctx, (size_t)3 * std::max(n, 1));
+    bf->d_ao_atom.alloc(ctx, (size_t)std::max(n, 1));
+  }
+  if ((int64_t)bf->d_gout.n < 3 * (int64_t)natoms) bf->d_gout.alloc(ctx, (size_t)3 * natoms);
+  if (Pb) ensure_pol_buffers(bf, false);
+  HIPX(hipMemcpyAsync(bf->d_ao_atom.p, ao_atom, sizeof(int) * n, hipMemcpyHostToDevice, s));
+  HIPX(hipMemcpyAsync(bf->d_P.p, Pa, sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+  if (Pb) HIPX(hipMemcpyAsync(bf->d_Pb.p, Pb, sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+  // density and functional as in a Fock build (FP64), then the gradient contractions
+  if (!Pb) {
     run_density<double>(bf, bf->d_t64.p, bf->d_P.p, gga ? 1 : 0, ctx->lanes);
     if (bf->bs == kTile) {
       launch_functional(s, functional, ctx->prm, N, kTile, nullptr, nullptr, bf->d_blocks.p, bf->d_part.p,
@@ -1416,16 +1420,45 @@ int xc_geom_gradient(xc_bfogc* bf, int functional, const double* P, int natoms, 
       launch_functional(s, functional, ctx->prm, N, kTile, bf->d_rho.p, gga ? bf->d_grad.p : nullptr, nullptr,
                         nullptr, bf->grid->d_w.p, nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
     }
-    launch_fill(s, bf->d_gacc.p, (int64_t)3 * n, 0.0);
+  } else {
+    const double* Ps[2] = {bf->d_P.p, bf->d_Pb.p};
+    double* rho[2] = {bf->d_rho.p, bf->d_rho_b.p};
+    double* grd[2] = {bf->d_grad.p, bf->d_grad_b.p};
+    for (int sp = 0; sp < 2; ++sp) {
+      run_density<double>(bf, bf->d_t64.p, Ps[sp], gga ? 1 : 0, ctx->lanes);
+      launch_reduce_partials(s, bf->d_blocks.p, bf->bs, N, bf->d_part.p, gga ? 1 : 0, rho[sp], grd[sp]);
+    }
+    launch_functional_pol(s, functional, ctx->prm, N, kTile, bf->d_rho.p, bf->d_grad.p, bf->d_rho_b.p,
+                          bf->d_grad_b.p, bf->grid->d_w.p, bf->d_vs.p, bf->d_gw.p, bf->d_vs_b.p, bf->d_gw_b.p,
+                          bf->d_eblk.p);
+  }
+  launch_fill(s, bf->d_gacc.p, (int64_t)3 * n, 0.0);
+  // restricted: -2 P_mu,nu with the total-density potential; unrestricted: the same per spin
+  launch_gradient(s, bf->d_grad_work.p, (int)bf->grad_work.size(), bf->d_blocks.p, bf->d_ao_idx.p, bf->d_t64.p,
+                  bf->d_P.p, n, bf->d_vs.p, bf->d_gw.p, gga ? 1 : 0, N, bf->d_gacc.p);
+  if (Pb)
     launch_gradient(s, bf->d_grad_work.p, (int)bf->grad_work.size(), bf->d_blocks.p, bf->d_ao_idx.p, bf->d_t64.p,
-                    bf->d_P.p, n, bf->d_vs.p, bf->d_gw.p, gga ? 1 : 0, N, bf->d_gacc.p);
-    launch_gradient_atoms(s, bf->d_gacc.p, bf->d_ao_atom.p, n, natoms, bf->d_gout.p);
-    launch_sum(s, bf->d_eblk.p, nxc, ctx->d_energy);
-    HIPX(hipGetLastError());
-    HIPX(hipMemcpyAsync(grad, bf->d_gout.p, sizeof(double) * 3 * natoms, hipMemcpyDeviceToHost, s));
-    HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPX(hipStreamSynchronize(s));
-    if (energy) *energy = ctx->last_energy;
+                    bf->d_Pb.p, n, bf->d_vs_b.p, bf->d_gw_b.p, gga ? 1 : 0, N, bf->d_gacc.p);
+  launch_gradient_atoms(s, bf->d_gacc.p, bf->d_ao_atom.p, n, natoms, bf->d_gout.p);
+  launch_sum(s, bf->d_eblk.p, nxc, ctx->d_energy);
+  HIPX(hipGetLastError());
+  HIPX(hipMemcpyAsync(grad, bf->d_gout.p, sizeof(double) * 3 * natoms, hipMemcpyDeviceToHost, s));
+  HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPX(hipStreamSynchronize(s));
+  if (energy) *energy = ctx->last_energy;
+}
+}  // namespace
+
+int xc_geom_gradient(xc_bfogc* bf, int functional, const double* P, int natoms, const int* ao_atom, double* grad,
+                     double* energy) {
+  return guarded([&] { geom_gradient(bf, functional, P, nullptr, natoms, ao_atom, grad, energy); });
+}
+
+int xc_geom_gradient_unrestricted(xc_bfogc* bf, int functional, const double* Pa, const double* Pb, int natoms,
+                                  const int* ao_atom, double* grad, double* energy) {
+  return guarded([&] {
+    require(Pb, XC_ERR_ARG, "xc_geom_gradient_unrestricted: null beta density");
+    geom_gradient(bf, functional, Pa, Pb, natoms, ao_atom, grad, energy);
   });
 }
 

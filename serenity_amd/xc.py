@@ -502,19 +502,28 @@ class FuncPotential:
 
     def getGeomGradients(self, P, ao_atom=None, natoms=None):
         """XC part of the nuclear gradient (natoms x 3) of the density P at the fixed grid
-        (FuncPotential::getGeomGradients).  The controller needs highestDerivative 1 (LDA)
+        (FuncPotential::getGeomGradients).  ``P`` is a matrix (restricted) or a pair
+        (P_alpha, P_beta) (unrestricted).  The controller needs highestDerivative 1 (LDA)
         or 2 (GGA).  ``ao_atom`` defaults to the basis' getAtomIndicesOfBasis()."""
         n = self.bfogc.getNBasisFunctions()
-        Pf = np.asfortranarray(P, dtype=np.float64)
-        if Pf.shape != (n, n):
-            raise XCError(1, "FuncPotential: density matrix is %s, basis has %d functions" % (Pf.shape, n))
+        pair = isinstance(P, (tuple, list))
+        Ps = [np.asfortranarray(p, dtype=np.float64) for p in (P if pair else [P])]
+        for Pf in Ps:
+            if Pf.shape != (n, n):
+                raise XCError(1, "FuncPotential: density matrix is %s, basis has %d functions" % (Pf.shape, n))
         atom = np.ascontiguousarray(self.bfogc.basis.getAtomIndicesOfBasis() if ao_atom is None else ao_atom,
                                     dtype=np.int32)
         na = int(atom.max()) + 1 if natoms is None else int(natoms)
         G = np.zeros((na, 3), order="F")
         E = C.c_double()
-        check(_lib.lib().xc_geom_gradient(self.bfogc.handle, FUNCTIONALS[self.functional], Pf.ctypes.data_as(_lib.dp),
-                                          na, _i(atom), G.ctypes.data_as(_lib.dp), C.byref(E)))
+        f = FUNCTIONALS[self.functional]
+        if pair:
+            check(_lib.lib().xc_geom_gradient_unrestricted(self.bfogc.handle, f, Ps[0].ctypes.data_as(_lib.dp),
+                                                           Ps[1].ctypes.data_as(_lib.dp), na, _i(atom),
+                                                           G.ctypes.data_as(_lib.dp), C.byref(E)))
+        else:
+            check(_lib.lib().xc_geom_gradient(self.bfogc.handle, f, Ps[0].ctypes.data_as(_lib.dp), na, _i(atom),
+                                              G.ctypes.data_as(_lib.dp), C.byref(E)))
         self._energy = E.value
         return np.array(G)
 

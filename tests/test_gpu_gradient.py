@@ -93,6 +93,40 @@ def test_h2_631gs_lda_gradient_pin(ctx):
     assert np.abs(G - ref).max() < 1e-6
 
 
+@pytest.mark.parametrize("functional", ["LDA", "PBE"])
+def test_unrestricted_gradient(ctx, functional):
+    """Open-shell spin densities: the GPU gradient against finite differences of the oracle's
+    unrestricted E_xc with the basis moved at a fixed grid and densities; a closed shell equals
+    the restricted gradient."""
+    Z, X = shifted_water()
+    b = xc.BasisController.from_file("DEF2-SVP", Z, X)
+    g = xc.molecular_grid(Z, X, accuracy=2)
+    S = b.overlap()
+    Pa = W.synthetic_density(S, 5, seed=3) / 2
+    Pb = W.synthetic_density(S, 4, seed=4) / 2
+    bf = xc.BasisFunctionOnGridController(ctx, b, g, 128, 0.0, 2)
+    pot = xc.FuncPotential(bf, functional, "FP64", 0.0)
+    G = pot.getGeomGradients((Pa, Pb))
+    h = 1e-4
+    fd = np.zeros((3, 3))
+    for a in range(3):
+        for c in range(3):
+            E = []
+            for sgn in (1, -1):
+                Xs = X.copy()
+                Xs[a, c] += sgn * h
+                bs = xc.BasisController.from_file("DEF2-SVP", Z, Xs)
+                E.append(O.fock_build_pol(bs, g.points, g.weights, Pa, Pb, functional, thr=0.0, block_ave_thr=0.0)[2])
+            fd[a, c] = (E[0] - E[1]) / (2 * h)
+    print("%s unrestricted gradient: max|G - FD| %.2e" % (functional, np.abs(G - fd).max()))
+    assert np.abs(G - fd).max() < 5e-8
+    P = W.synthetic_density(S, 5, seed=3)
+    Gr = pot.getGeomGradients(P)
+    Gu = pot.getGeomGradients((P / 2, P / 2))
+    assert np.abs(Gr - Gu).max() < 1e-12
+    bf._release()
+
+
 def test_gradient_errors(ctx):
     Z, X = shifted_water()
     b = xc.BasisController.from_file("DEF2-SVP", Z, X)
