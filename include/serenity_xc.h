@@ -134,6 +134,27 @@ int xc_one_electron(int nshell, const int* shell_l, const int* shell_pure, const
 int xc_eri_packed(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,
                   const double* shell_center, const double* alpha, const double* coef, double schwarz_threshold,
                   double* eri);
+/* The metric (P|Q) of an auxiliary shell list alone (naux x naux). */
+int xc_ri_metric(int naux_shell, const int* aux_l, const int* aux_pure, const int* aux_nprim,
+                 const double* aux_center, const double* aux_alpha, const double* aux_coef, double* M);
+
+/* ---- density-fitted Coulomb matrix on the GPU (J engine) -----------------
+ * The Coulomb part of Serenity's DFT Fock matrix with densFitJ = RI
+ * (RI_J_IntegralController.cpp:41-121 over libint2 3-centre integrals) without
+ * storing 3-centre integrals: every significant primitive pair of the orbital
+ * basis (|K| (pi/p)^1.5 >= threshold) is contracted with the density into
+ * Hermite form and meets the auxiliary functions through Hermite Coulomb
+ * integrals.  Orbital shells up to f, auxiliary up to g.
+ *   xc_coulomb_rhs:    b_P = sum_mu,nu P_mu,nu (mu nu|P)     (P host n x n, b host naux)
+ *   xc_coulomb_matrix: J_mu,nu = sum_P c_P (mu nu|P)         (c host naux, J host n x n)
+ * with c = M^-1 b between the two (M from xc_ri_metric). */
+typedef struct xc_coulomb xc_coulomb;
+int xc_coulomb_create(xc_basis* basis, xc_basis* aux, double threshold, xc_coulomb** out);
+int xc_coulomb_info(const xc_coulomb* jc, int64_t* npairs, int64_t* hermite_terms);
+int xc_coulomb_rhs(xc_coulomb* jc, const double* P, double* b);
+int xc_coulomb_matrix(xc_coulomb* jc, const double* c, double* J);
+int xc_coulomb_destroy(xc_coulomb* jc);
+
 /* Density-fitted Coulomb integrals over an auxiliary shell list: B3 is
  * naux x npair (row P holds (P|mu nu)), M the naux x naux metric (P|Q). */
 int xc_ri_j_integrals(int nshell, const int* shell_l, const int* shell_pure, const int* shell_nprim,

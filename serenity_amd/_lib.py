@@ -71,6 +71,12 @@ def lib():
     _sig("xc_eri_packed", C.c_int, C.c_int, ip, ip, ip, dp, dp, dp, C.c_double, dp)
     _sig("xc_ri_j_integrals", C.c_int, C.c_int, ip, ip, ip, dp, dp, dp, C.c_int, ip, ip, ip, dp, dp, dp,
          C.c_double, dp, dp)
+    _sig("xc_ri_metric", C.c_int, C.c_int, ip, ip, ip, dp, dp, dp, dp)
+    _sig("xc_coulomb_create", C.c_int, vp, vp, C.c_double, C.POINTER(vp))
+    _sig("xc_coulomb_info", C.c_int, vp, C.POINTER(i64), C.POINTER(i64))
+    _sig("xc_coulomb_rhs", C.c_int, vp, dp, dp)
+    _sig("xc_coulomb_matrix", C.c_int, vp, dp, dp)
+    _sig("xc_coulomb_destroy", C.c_int, vp)
     _sig("xc_basis_create", C.c_int, vp, C.c_int, ip, ip, ip, dp, dp, dp, C.POINTER(vp))
     _sig("xc_basis_destroy", C.c_int, vp)
     _sig("xc_basis_nao", C.c_int, vp, ip)

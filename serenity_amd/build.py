@@ -17,6 +17,7 @@ SOURCES = [
     "kernels/xc_kernels.hip",
     "kernels/grid_kernels.hip",
     "kernels/gradient_kernels.hip",
+    "kernels/coulomb_kernels.hip",
     "host/basis.cpp",
     "host/grid.cpp",
     "host/overlap.cpp",

@@ -117,5 +117,7 @@ void eri_packed(const std::vector<Shell>& shells, double schwarz_threshold, std:
 // an naux x npair matrix B3 (row P) and the metric (P|Q), naux x naux.
 void ri_j_integrals(const std::vector<Shell>& shells, const std::vector<Shell>& aux, double schwarz_threshold,
                     std::vector<double>& B3, std::vector<double>& M);
+// The metric (P|Q) alone, naux x naux.
+void ri_metric(const std::vector<Shell>& aux, std::vector<double>& M);
 
 }  // namespace sxc

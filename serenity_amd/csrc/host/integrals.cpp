@@ -427,6 +427,36 @@ void eri_packed(const std::vector<Shell>& shells, double schwarz_threshold, std:
   }
 }
 
+void ri_metric(const std::vector<Shell>& aux, std::vector<double>& M) {
+  int naux = 0;
+  const auto ca = cart_shells(aux, naux);
+  const int na = (int)aux.size();
+  M.assign((size_t)naux * naux, 0.0);
+  std::vector<Shell> units(na);
+  std::vector<CartShell> cu(na);
+  std::vector<std::vector<PrimPair>> ap(na);
+  for (int P = 0; P < na; ++P) {
+    units[P] = unit_shell(aux[P]);
+    int dummy = 0;
+    cu[P] = cart_shells({units[P]}, dummy)[0];
+    ap[P] = prim_pairs(aux[P], units[P]);
+  }
+#pragma omp parallel for schedule(dynamic)
+  for (int P = 0; P < na; ++P) {
+    HermR R;
+    std::vector<double> qc, half, ao;
+    for (int Q = 0; Q <= P; ++Q) {
+      eri_quartet(aux[P], units[P], aux[Q], units[Q], ca[P], cu[P], ca[Q], cu[Q], ap[P], ap[Q], R, qc);
+      quartet_to_ao(ca[P], cu[P], ca[Q], cu[Q], qc, half, ao);
+      for (int f = 0; f < ca[P].nfunc; ++f)
+        for (int h = 0; h < ca[Q].nfunc; ++h) {
+          const size_t r = ca[P].off + f, c = ca[Q].off + h;
+          M[r * naux + c] = M[c * naux + r] = ao[(size_t)f * ca[Q].nfunc + h];
+        }
+    }
+  }
+}
+
 void ri_j_integrals(const std::vector<Shell>& shells, const std::vector<Shell>& aux, double schwarz_threshold,
                     std::vector<double>& B3, std::vector<double>& M) {
   int n = 0, naux = 0;

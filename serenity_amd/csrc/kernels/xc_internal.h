@@ -117,6 +117,35 @@ void launch_gradient(hipStream_t s, const GradItem* work, int nwork, const DevBl
 // grad[A + natoms c] = -2 sum_{nu on A} acc[nu][c].
 void launch_gradient_atoms(hipStream_t s, const double* acc, const int* ao_atom, int n, int natoms, double* grad);
 
+// ---- density-fitted Coulomb matrix (kernels/coulomb_kernels.hip) ----------
+// One significant primitive pair (a of shell A, b of shell B, A >= B) of the orbital basis:
+// p = a + b, K = c_a c_b exp(-a b / p |AB|^2), centre P, P - A, P - B; sym = 2 for A > B
+// (the pair stands for both orders), 1 for A == B; herm_off: offset of its (L + 1)(L + 2)(L + 3) / 6
+// Hermite coefficients (L = l_A + l_B) in the per-pair arrays.
+struct JPair {
+  int A, B, L, herm_off;
+  double sym, p, K;
+  double P[3], PA[3], PB[3];
+};
+constexpr int kJMaxLPrim = 3;                      // orbital shells up to f
+constexpr int kJMaxLAux = 4;                       // auxiliary shells up to g
+constexpr int kJMaxL = 2 * kJMaxLPrim + kJMaxLAux;
+constexpr int kJMaxH = (kJMaxL + 1) * (kJMaxL + 2) * (kJMaxL + 3) / 6;
+constexpr int kJMaxHab = (2 * kJMaxLPrim + 1) * (2 * kJMaxLPrim + 2) * (2 * kJMaxLPrim + 3) / 6;
+constexpr int kJMaxHs = (kJMaxLAux + 1) * (kJMaxLAux + 2) * (kJMaxLAux + 3) / 6;
+constexpr int kJMaxNcart = (kJMaxLPrim + 1) * (kJMaxLPrim + 2) / 2;
+constexpr int kJMaxNcartAux = (kJMaxLAux + 1) * (kJMaxLAux + 2) / 2;
+constexpr int kJMaxE = (kJMaxLPrim + 1) * (kJMaxLPrim + 1) * (2 * kJMaxLPrim + 1);
+void launch_jdens(hipStream_t s, const JPair* pairs, int npair, const DevShell* shells, const double* ang,
+                  const double* P, int n, double* D);
+void launch_jrhs(hipStream_t s, const JPair* pairs, int npair, const double* D, const DevShell* aux, int naux_shell,
+                 const double* aux_alpha, const double* aux_coef, const int* aux_cart_off, double* bc);
+void launch_jpot(hipStream_t s, const JPair* pairs, int npair, const DevShell* aux, int naux_shell,
+                 const double* aux_alpha, const double* aux_coef, const int* aux_cart_off, const double* cc,
+                 double* V);
+void launch_jassemble(hipStream_t s, const JPair* pairs, int npair, const DevShell* shells, const double* ang,
+                      const double* V, int n, double* J);
+
 // Molecular-grid partition weights of ncand candidate points (kernels/grid_kernels.hip).
 void launch_partition(hipStream_t s, int type, unsigned smoothing, int natoms, const double* xyz, const double* dist,
                       const double* apair, const double* min_dist, int64_t ncand, const double* cand_xyz,

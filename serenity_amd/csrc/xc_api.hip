@@ -182,6 +182,20 @@ struct xc_basis {
   DBuf<double> d_alpha, d_coef, d_ang;
 };
 
+// Density-fitted Coulomb matrix of an orbital basis over an auxiliary basis (the J engine).
+struct xc_coulomb {
+  xc_ctx* ctx = nullptr;
+  xc_basis* basis = nullptr;
+  xc_basis* aux = nullptr;
+  std::vector<JPair> pairs;
+  int64_t herm_total = 0;
+  int ncart_aux = 0;
+  std::vector<int> cart_off;  // first Cartesian function of every auxiliary shell
+  DBuf<JPair> d_pairs;
+  DBuf<int> d_cart_off;
+  DBuf<double> d_D, d_V, d_bc, d_cc, d_P, d_J;
+};
+
 struct xc_grid {
   xc_ctx* ctx = nullptr;
   int64_t npts = 0;
@@ -1494,6 +1508,169 @@ int xc_fock_build_unrestricted_device(xc_bfogc* bf, int functional, int precisio
       HIPX(hipStreamSynchronize(ctx->stream));
       *energy = ctx->last_energy;
     }
+  });
+}
+
+int xc_ri_metric(int naux_shell, const int* aux_l, const int* aux_pure, const int* aux_nprim,
+                 const double* aux_center, const double* aux_alpha, const double* aux_coef, double* M) {
+  return guarded([&] {
+    require(M, XC_ERR_ARG, "xc_ri_metric: null output");
+    auto ax = host_shells("xc_ri_metric", naux_shell, aux_l, aux_pure, aux_nprim, aux_center, aux_alpha, aux_coef, 4);
+    std::vector<double> m;
+    ri_metric(ax, m);
+    std::copy(m.begin(), m.end(), M);
+  });
+}
+
+int xc_coulomb_create(xc_basis* basis, xc_basis* aux, double threshold, xc_coulomb** out) {
+  return guarded([&] {
+    require(basis && aux && out && threshold >= 0.0, XC_ERR_ARG, "xc_coulomb_create: bad arguments");
+    require(basis->ctx == aux->ctx, XC_ERR_ARG, "xc_coulomb_create: bases live on different contexts");
+    require(basis->maxl <= kJMaxLPrim, XC_ERR_INPUT,
+            "xc_coulomb_create: orbital shells above l = " + std::to_string(kJMaxLPrim));
+    require(aux->maxl <= kJMaxLAux, XC_ERR_INPUT,
+            "xc_coulomb_create: auxiliary shells above l = " + std::to_string(kJMaxLAux));
+    xc_ctx* ctx = basis->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipSetDevice(ctx->device));
+    auto* jc = new xc_coulomb();
+    try {
+      jc->ctx = ctx;
+      jc->basis = basis;
+      jc->aux = aux;
+      // significant primitive pairs, grouped by angular momentum
+      const auto& sh = basis->shells;
+      const int ns = (int)sh.size();
+      for (int A = 0; A < ns; ++A)
+        for (int B = 0; B <= A; ++B) {
+          const DevShell& a = sh[A];
+          const DevShell& b = sh[B];
+          const double dx = a.cx - b.cx, dy = a.cy - b.cy, dz = a.cz - b.cz, ab2 = dx * dx + dy * dy + dz * dz;
+          for (int i = 0; i < a.nprim; ++i)
+            for (int j = 0; j < b.nprim; ++j) {
+              const double ea = basis->alpha[a.prim_off + i], eb = basis->alpha[b.prim_off + j];
+              const double p = ea + eb;
+              const double K = basis->coef[a.prim_off + i] * basis->coef[b.prim_off + j] * std::exp(-ea * eb / p * ab2);
+              if (std::fabs(K) * std::pow(M_PI / p, 1.5) < threshold) continue;
+              JPair q{};
+              q.A = A;
+              q.B = B;
+              q.L = a.l + b.l;
+              q.sym = A == B ? 1.0 : 2.0;
+              q.p = p;
+              q.K = K;
+              const double ac[3] = {a.cx, a.cy, a.cz}, bc[3] = {b.cx, b.cy, b.cz};
+              for (int d = 0; d < 3; ++d) {
+                q.P[d] = (ea * ac[d] + eb * bc[d]) / p;
+                q.PA[d] = q.P[d] - ac[d];
+                q.PB[d] = q.P[d] - bc[d];
+              }
+              jc->pairs.push_back(q);
+            }
+        }
+      std::stable_sort(jc->pairs.begin(), jc->pairs.end(), [&](const JPair& x, const JPair& y) {
+        return x.L != y.L ? x.L < y.L : sh[x.A].l < sh[y.A].l;
+      });
+      int64_t off = 0;
+      for (auto& q : jc->pairs) {
+        q.herm_off = (int)off;
+        off += (q.L + 1) * (q.L + 2) * (q.L + 3) / 6;
+      }
+      require(off < (1LL << 31), XC_ERR_INPUT, "xc_coulomb_create: too many primitive pairs");
+      jc->herm_total = off;
+      for (const auto& s : aux->shells) {
+        jc->cart_off.push_back(jc->ncart_aux);
+        jc->ncart_aux += s.ncart;
+      }
+      const int n = basis->nao;
+      jc->d_pairs.upload(ctx, jc->pairs.data(), jc->pairs.size());
+      jc->d_cart_off.upload(ctx, jc->cart_off.data(), jc->cart_off.size());
+      jc->d_D.alloc(ctx, std::max<int64_t>(off, 1));
+      jc->d_V.alloc(ctx, std::max<int64_t>(off, 1));
+      jc->d_bc.alloc(ctx, std::max(jc->ncart_aux, 1));
+      jc->d_cc.alloc(ctx, std::max(jc->ncart_aux, 1));
+      jc->d_P.alloc(ctx, (size_t)n * n);
+      jc->d_J.alloc(ctx, (size_t)n * n);
+      HIPX(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      delete jc;
+      throw;
+    }
+    ++ctx->live_handles;
+    *out = jc;
+  });
+}
+
+int xc_coulomb_info(const xc_coulomb* jc, int64_t* npairs, int64_t* hermite_terms) {
+  return guarded([&] {
+    require(jc, XC_ERR_ARG, "xc_coulomb_info: null handle");
+    if (npairs) *npairs = (int64_t)jc->pairs.size();
+    if (hermite_terms) *hermite_terms = jc->herm_total;
+  });
+}
+
+int xc_coulomb_destroy(xc_coulomb* jc) {
+  return guarded([&] {
+    if (!jc) return;
+    (void)hipStreamSynchronize(jc->ctx->stream);
+    --jc->ctx->live_handles;
+    delete jc;
+  });
+}
+
+int xc_coulomb_rhs(xc_coulomb* jc, const double* P, double* b) {
+  return guarded([&] {
+    require(jc && P && b, XC_ERR_ARG, "xc_coulomb_rhs: null argument");
+    xc_ctx* ctx = jc->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    const int n = jc->basis->nao;
+    const int npair = (int)jc->pairs.size();
+    HIPX(hipMemcpyAsync(jc->d_P.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+    launch_jdens(s, jc->d_pairs.p, npair, jc->basis->d_shells.p, jc->basis->d_ang.p, jc->d_P.p, n, jc->d_D.p);
+    launch_fill(s, jc->d_bc.p, jc->ncart_aux, 0.0);
+    launch_jrhs(s, jc->d_pairs.p, npair, jc->d_D.p, jc->aux->d_shells.p, jc->aux->nshell, jc->aux->d_alpha.p,
+                jc->aux->d_coef.p, jc->d_cart_off.p, jc->d_bc.p);
+    HIPX(hipGetLastError());
+    std::vector<double> bc(jc->ncart_aux);
+    HIPX(hipMemcpyAsync(bc.data(), jc->d_bc.p, sizeof(double) * bc.size(), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    // Cartesian -> auxiliary AO: b_f = sum_c M[f][c] b_c
+    for (int k = 0; k < jc->aux->nshell; ++k) {
+      const DevShell& S = jc->aux->shells[k];
+      for (int f = 0; f < S.nfunc; ++f) {
+        double v = 0.0;
+        for (int c = 0; c < S.ncart; ++c) v += jc->aux->ang[S.ang_off + f * S.ncart + c] * bc[jc->cart_off[k] + c];
+        b[S.ao_off + f] = v;
+      }
+    }
+  });
+}
+
+int xc_coulomb_matrix(xc_coulomb* jc, const double* c, double* J) {
+  return guarded([&] {
+    require(jc && c && J, XC_ERR_ARG, "xc_coulomb_matrix: null argument");
+    xc_ctx* ctx = jc->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    const int n = jc->basis->nao;
+    const int npair = (int)jc->pairs.size();
+    // auxiliary AO -> Cartesian: cc_c = sum_f c_f M[f][c]
+    std::vector<double> cc(std::max(jc->ncart_aux, 1), 0.0);
+    for (int k = 0; k < jc->aux->nshell; ++k) {
+      const DevShell& S = jc->aux->shells[k];
+      for (int f = 0; f < S.nfunc; ++f)
+        for (int q = 0; q < S.ncart; ++q)
+          cc[jc->cart_off[k] + q] += c[S.ao_off + f] * jc->aux->ang[S.ang_off + f * S.ncart + q];
+    }
+    HIPX(hipMemcpyAsync(jc->d_cc.p, cc.data(), sizeof(double) * jc->ncart_aux, hipMemcpyHostToDevice, s));
+    launch_jpot(s, jc->d_pairs.p, npair, jc->aux->d_shells.p, jc->aux->nshell, jc->aux->d_alpha.p, jc->aux->d_coef.p,
+                jc->d_cart_off.p, jc->d_cc.p, jc->d_V.p);
+    launch_fill(s, jc->d_J.p, (int64_t)n * n, 0.0);
+    launch_jassemble(s, jc->d_pairs.p, npair, jc->basis->d_shells.p, jc->basis->d_ang.p, jc->d_V.p, n, jc->d_J.p);
+    HIPX(hipGetLastError());
+    HIPX(hipMemcpyAsync(J, jc->d_J.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
   });
 }
 

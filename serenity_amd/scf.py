@@ -101,7 +101,7 @@ class ScfIntegrals:
     ``aux`` = None: exact four-centre J (Serenity's densFitJ NONE);
     otherwise density-fitted J over the auxiliary basis (densFitJ RI)."""
 
-    def __init__(self, basis, Z, coords_bohr, aux=None, schwarz=1e-14):
+    def __init__(self, basis, Z, coords_bohr, aux=None, schwarz=1e-14, coulomb=None):
         t0 = time.perf_counter()
         self.basis, self.aux = basis, aux
         self.n = basis.nao
@@ -109,7 +109,10 @@ class ScfIntegrals:
         self.T, self.V = one_electron(basis, Z, coords_bohr)
         self.h = self.T + self.V
         self.e_nuc = nuclear_repulsion(Z, coords_bohr)
-        if aux is None:
+        self.gpu_j = coulomb  # a DensityFittedCoulomb: J on the GPU instead of the host integrals
+        if coulomb is not None:
+            self.eri = self.D = None
+        elif aux is None:
             self.eri = eri_packed(basis, schwarz)
             self.D = None
         else:
@@ -121,6 +124,8 @@ class ScfIntegrals:
         self.setup_s = time.perf_counter() - t0
 
     def coulomb(self, P):
+        if self.gpu_j is not None:
+            return self.gpu_j.coulomb(P)
         p = pack_density(P)
         if self.D is not None:
             j = self.D.T @ (self.D @ p)
@@ -318,7 +323,7 @@ class ScfTask:
     def __init__(self, ctx, Z, coords_bohr, basis="DEF2-SVP", functional="LDA", ri=True, aux="RI_J_WEIGEND",
                  accuracy=4, small_accuracy=2, spherical=True, schwarz=1e-14, fp32_tiles=True, log=None,
                  aux_spherical=None, radial_threshold=1e-9, block_ave_threshold=1e-11, highest_derivative=1,
-                 spin=0, unrestricted=False):
+                 spin=0, unrestricted=False, gpu_coulomb=False):
         self.ctx = ctx
         self.Z = np.ascontiguousarray(Z, dtype=np.int32)
         self.X = np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
@@ -332,7 +337,8 @@ class ScfTask:
         self.radial_threshold = radial_threshold
         self.block_ave_threshold = block_ave_threshold
         self.highest_derivative = highest_derivative
-        self.ints = ScfIntegrals(self.basis, self.Z, self.X, self.auxbasis, schwarz)
+        jgpu = xc.DensityFittedCoulomb(ctx, self.basis, self.auxbasis) if (gpu_coulomb and ri) else None
+        self.ints = ScfIntegrals(self.basis, self.Z, self.X, self.auxbasis, schwarz, coulomb=jgpu)
         nel = int(self.Z.sum())
         if (nel - spin) % 2 or spin < 0 or spin > nel:
             raise xc.XCError(1, "ScfTask: %d electrons cannot have %d unpaired" % (nel, spin))
@@ -431,3 +437,5 @@ class ScfTask:
         for bf in self._bf.values():
             bf._release()
         self._bf = {}
+        if self.ints.gpu_j is not None:
+            self.ints.gpu_j._release()

@@ -34,6 +34,7 @@ __all__ = [
     "Context", "BasisController", "GridController", "BasisFunctionOnGridController",
     "DensityOnGridCalculator", "FunctionalLibrary", "FunctionalData", "ScalarOperatorToMatrixAdder",
     "FuncPotential", "molecular_grid", "lebedev", "overlap_matrix", "XCError", "ANGSTROM_TO_BOHR",
+    "DensityFittedCoulomb",
 ]
 
 
@@ -112,7 +113,8 @@ class Context:
         if not self.handle:
             return
         kids = list(self._children)
-        order = lambda o: 0 if isinstance(o, Server) else 1 if isinstance(o, BasisFunctionOnGridController) else 2
+        order = lambda o: (0 if isinstance(o, Server) else
+                           1 if isinstance(o, (BasisFunctionOnGridController, DensityFittedCoulomb)) else 2)
         for k in sorted(kids, key=order):
             k._release()
         check(_lib.lib().xc_ctx_destroy(self.handle))
@@ -267,6 +269,49 @@ class _DeviceBasis(_Handle):
                                          _d(basis.center), _d(basis.alpha), _d(basis.coef), C.byref(h)))
         self.handle = h
         ctx._register(self)
+
+
+class DensityFittedCoulomb(_Handle):
+    """Density-fitted Coulomb matrix on the GPU (the J engine behind Serenity's densFitJ = RI):
+    ``coulomb(P)`` returns J[P] = B^T M^-1 B p, with the 3-centre contractions on the device
+    (no 3-centre integral stored) and the metric M = (P|Q) factorised once on the host."""
+    _destroy = "xc_coulomb_destroy"
+
+    def __init__(self, ctx, basis, aux, threshold=1e-14):
+        import scipy.linalg
+        self._db = _DeviceBasis(ctx, basis)
+        self._da = _DeviceBasis(ctx, aux)
+        h = C.c_void_p()
+        check(_lib.lib().xc_coulomb_create(self._db.handle, self._da.handle, C.c_double(threshold), C.byref(h)))
+        self.handle = h
+        ctx._register(self)
+        self.n, self.naux = basis.nao, aux.nao
+        M = np.zeros((aux.nao, aux.nao))
+        check(_lib.lib().xc_ri_metric(*aux.as_args(), _d(M)))
+        self.metric = M
+        self._cho = scipy.linalg.cho_factor(M)
+        np_, nh = C.c_int64(), C.c_int64()
+        check(_lib.lib().xc_coulomb_info(h, C.byref(np_), C.byref(nh)))
+        self.npairs, self.hermite_terms = np_.value, nh.value
+
+    def rhs(self, P):
+        """b_Q = sum P_mu,nu (mu nu|Q)."""
+        b = np.zeros(self.naux)
+        Pf = np.asfortranarray(P, dtype=np.float64)
+        check(_lib.lib().xc_coulomb_rhs(self.handle, Pf.ctypes.data_as(_lib.dp), _d(b)))
+        return b
+
+    def matrix(self, c):
+        """J_mu,nu = sum_Q c_Q (mu nu|Q)."""
+        J = np.zeros((self.n, self.n), order="F")
+        check(_lib.lib().xc_coulomb_matrix(self.handle, _d(np.ascontiguousarray(c, dtype=np.float64)),
+                                           J.ctypes.data_as(_lib.dp)))
+        return np.array(J)
+
+    def coulomb(self, P):
+        import scipy.linalg
+        c = scipy.linalg.cho_solve(self._cho, self.rhs(P))
+        return self.matrix(c)
 
 
 class _DeviceGrid(_Handle):
