@@ -149,17 +149,23 @@ def fde_sweep(ctx, steps, warmup, check_oracle):
     return out
 
 
-def scf_leg(ctx, check_oracle):
-    """Full SCF of (H2O)8/def2-SVP/LDA (RI-J): FP64, mixed with FP64 finish, and the oracle-driven SCF."""
+def scf_leg(ctx, check_oracle, config=1, cycle_log=None):
+    """Full SCF of a benchmark configuration (RI-J): FP64, mixed with FP64 finish, and (optionally) the
+    oracle-driven SCF.  Config 1 computes J on the host, config 3 with the GPU J engine."""
     from serenity_amd import scf as SCF, workloads as W
-    Z, X = W.geometry(W.CONFIGS[1]["system"])
+    cfg = W.CONFIGS[config]
+    Z, X = W.geometry(cfg["system"])
     t0 = time.perf_counter()
-    task = SCF.ScfTask(ctx, Z, X, basis="DEF2-SVP", functional="LDA")
-    out = dict(workload="(H2O)8/def2-SVP/LDA, RI-J (RI_J_WEIGEND), SCF on the accuracy-2 grid, final energy on "
-                        "the accuracy-4 grid", nao=task.basis.nao, naux=task.auxbasis.nao,
+    gpu_j = config != 1
+    task = SCF.ScfTask(ctx, Z, X, basis=cfg["basis"], functional=cfg["functional"], gpu_coulomb=gpu_j,
+                       log=cycle_log)
+    out = dict(workload="%s, RI-J (RI_J_WEIGEND, %s), SCF on the accuracy-2 grid, final energy on the accuracy-4 "
+                        "grid" % (cfg["name"], "GPU J engine" if gpu_j else "host 3-centre integrals"),
+               nao=task.basis.nao, naux=task.auxbasis.nao,
                npts_small=task.grid_small.getNGridPoints(), npts_default=task.grid_default.getNGridPoints(),
                setup_s=round(time.perf_counter() - t0, 3))
-    task.run("FP64")  # first run builds the resident tiles of both grids
+    if config == 1:
+        task.run("FP64")  # first run builds the resident tiles of both grids
     r64 = task.run("FP64")
     rmx = task.run("MIXED")
     out["fp64"] = dict(energy=r64["energy"], cycles=r64["cycles"], wall_s=r64["wall_s"], xc_s=r64["t_xc_s"])
@@ -173,11 +179,13 @@ def scf_leg(ctx, check_oracle):
         t0 = time.perf_counter()
         b, gs, gd = task.basis, task.grid_small, task.grid_default
 
+        f = cfg["functional"]
+
         def build(P):
-            F, E, _, _ = O.fock_build(b, gs.points, gs.weights, P, "LDA")
+            F, E, _, _ = O.fock_build(b, gs.points, gs.weights, P, f)
             return E, F
-        r = SCF.perform(task.ints, task.nocc, build)
-        _, e_xc, _, _ = O.fock_build(b, gd.points, gd.weights, r["P"], "LDA")
+        r = SCF.perform(task.ints, task.nocc, build, log=cycle_log)
+        _, e_xc, _, _ = O.fock_build(b, gd.points, gd.weights, r["P"], f)
         e = SCF.total_energy(task.ints, r["P"], e_xc)
         out["cpu_oracle"] = dict(energy=e, cycles=r["cycles"], wall_s=round(time.perf_counter() - t0, 3),
                                  threads=O.max_threads())
@@ -197,7 +205,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fde", action="store_true", help="skip the embedding (non-additive potential) sweep")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
-    ap.add_argument("--no-scf", action="store_true", help="skip the full SCF of (H2O)8/def2-SVP/LDA")
+    ap.add_argument("--no-scf", action="store_true", help="skip the full SCFs of configs 1 and 3")
+    ap.add_argument("--scf3", action="store_true",
+                    help="also run the full SCF of (H2O)64/def2-TZVP/PBE (FP64 and mixed; a few minutes)")
+    ap.add_argument("--scf3-oracle", action="store_true",
+                    help="with --scf3: also the SCF driven by the CPU oracle's XC build (a few more minutes)")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--serving-qps", type=float, default=400.0)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -310,6 +322,14 @@ def main():
              "" if "cpu_oracle" not in sc else "; CPU oracle SCF %.2f s, |dE| %.2e" %
              (sc["cpu_oracle"]["wall_s"], sc["fp64"]["dE_vs_cpu_oracle"])))
         out_extra["scf"] = sc
+    if args.scf3 and not args.no_scf:
+        sc3 = scf_leg(ctx, args.scf3_oracle, config=3, cycle_log=lambda *a: log("  scf3", *a))
+        log("SCF (H2O)64/TZVP/PBE: FP64 E %.10f (%d cycles, %.1f s); mixed before finish dE %.2e, after %.2e "
+            "(%d+%d cycles, %.1f s)" %
+            (sc3["fp64"]["energy"], sc3["fp64"]["cycles"], sc3["fp64"]["wall_s"],
+             sc3["mixed"]["dE_before_finish_vs_fp64"], sc3["mixed"]["dE_after_finish_vs_fp64"],
+             sc3["mixed"]["cycles_fp32"], sc3["mixed"]["cycles_fp64_finish"], sc3["mixed"]["wall_s"]))
+        out_extra["scf_config3"] = sc3
 
     if not args.no_serving:
         from serenity_amd import serving as SV
