@@ -173,6 +173,8 @@ def scf_leg(ctx, check_oracle, config=1, cycle_log=None):
                         cycles_fp32=rmx["cycles_fp32"], cycles_fp64_finish=rmx["cycles_fp64_finish"],
                         wall_s=rmx["wall_s"])
     out["mixed"]["dE_before_finish_vs_fp64"] = abs(rmx["energy_before_fp64_finish"] - r64["energy"])
+    # the FP32-converged density's energy in FP64: its convergence error alone
+    out["mixed"]["dE_fp32_density_in_fp64_vs_fp64"] = abs(task.final_energy(rmx["P_fp32"], "FP64") - r64["energy"])
     out["mixed"]["dE_after_finish_vs_fp64"] = abs(rmx["energy"] - r64["energy"])
     if check_oracle:
         from oracle import oracle as O

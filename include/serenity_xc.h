@@ -43,7 +43,9 @@ enum { XC_FUNC_LDA = 0, XC_FUNC_PBE = 1, XC_FUNC_TF = 2, XC_FUNC_SLATER = 3, XC_
        XC_FUNC_PBEX = 5, XC_FUNC_PBEC = 6, XC_FUNC_PBE_LIBXC = 7 };
 
 /* Precision of the two dense contractions (phi.P and phi^T.(w v phi)).  The
- * functional, the E_xc reduction and the F accumulation are FP64 in every mode. */
+ * functional, the E_xc reduction and the F accumulation are FP64 in every mode.
+ * XC_PREC_FP32 contracts with fl32(P) and adds the FP64 term tr(F (P - fl32(P)))
+ * to E_xc (dE_xc/dP = F), which removes the first-order error of rounding P. */
 enum { XC_PREC_FP64 = 0, XC_PREC_FP32 = 1 };
 
 /* Grid partition and radial quadrature (Serenity Options::GRID_TYPES / RADIAL_GRID_TYPES). */

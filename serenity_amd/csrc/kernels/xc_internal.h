@@ -171,6 +171,11 @@ void launch_subtract_scalars(hipStream_t s, const DensityList& list, double* out
 void launch_to_float(hipStream_t s, const double* src, float* dst, int64_t n);
 // F[i + n j] (+)= U[max(i,j) + n min(i,j)] (U holds the lower triangle)
 void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accumulate);
-void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int subtract = 0);
+// mode 0: *out = sum x; 1: *out -= sum x; 2: *out += sum x
+void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int mode = 0);
+// *out += sum_e F[e] (P[e] - P32[e]) over nn elements, through kRoundBlocks partials in part
+constexpr int kRoundBlocks = 256;
+void launch_rounding_term(hipStream_t s, const double* F, const double* P, const float* P32, int64_t nn, double* part,
+                          double* out);
 
 }  // namespace sxc

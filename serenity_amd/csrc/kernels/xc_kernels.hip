@@ -837,8 +837,9 @@ __global__ void finalize_kernel(const double* __restrict__ U, double* __restrict
   F[e] = accumulate ? F[e] + u : u;
 }
 
+// mode 0: out = sum; 1: out -= sum; 2: out += sum
 __global__ __launch_bounds__(256) void sum_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ out,
-                                                  int subtract) {
+                                                  int mode) {
   __shared__ double red[256];
   const int t = threadIdx.x;
   double s = 0.0;
@@ -849,7 +850,26 @@ __global__ __launch_bounds__(256) void sum_kernel(const double* __restrict__ x, 
     if (t < k) red[t] += red[t + k];
     __syncthreads();
   }
-  if (t == 0) out[0] = subtract ? out[0] - red[0] : red[0];
+  if (t == 0) out[0] = mode == 0 ? red[0] : mode == 1 ? out[0] - red[0] : out[0] + red[0];
+}
+
+// One partial per block of sum_e F[e] (P[e] - P32[e]): the first-order change of E_xc
+// from the FP32-rounded density matrix P32 back to P, since dE_xc/dP = F.
+__global__ __launch_bounds__(256) void rounding_term_kernel(const double* __restrict__ F, const double* __restrict__ P,
+                                                            const float* __restrict__ P32, int64_t nn,
+                                                            double* __restrict__ part) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + t; e < nn; e += (int64_t)gridDim.x * 256)
+    s += F[e] * (P[e] - (double)P32[e]);
+  red[t] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) red[t] += red[t + k];
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = red[0];
 }
 
 inline unsigned grid1d(int64_t n, int bsz) { return (unsigned)((n + bsz - 1) / bsz); }
@@ -1003,8 +1023,15 @@ void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accum
   hipLaunchKernelGGL(finalize_kernel, dim3(grid1d(nn, 256)), dim3(256), 0, s, U, F, n, accumulate);
 }
 
-void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int subtract) {
-  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, x, n, out, subtract);
+void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int mode) {
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, x, n, out, mode);
+}
+
+void launch_rounding_term(hipStream_t s, const double* F, const double* P, const float* P32, int64_t nn, double* part,
+                          double* out) {
+  if (nn <= 0) return;
+  hipLaunchKernelGGL(rounding_term_kernel, dim3(kRoundBlocks), dim3(256), 0, s, F, P, P32, nn, part);
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, part, (int64_t)kRoundBlocks, out, 2);
 }
 
 }  // namespace sxc

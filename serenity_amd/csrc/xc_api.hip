@@ -227,6 +227,7 @@ struct xc_bfogc {
   // per-build work buffers
   DBuf<double> d_part, d_rho, d_grad, d_vs, d_gw, d_eblk, d_U, d_P, d_F, d_v, d_g, d_E;
   DBuf<float> d_P32;
+  DBuf<double> d_round;  // kRoundBlocks partials of the FP32 rounding term
   // supersystem density of a non-additive build with this controller active (allocated on first use)
   DBuf<double> d_rho_sup, d_grad_sup;
   // cached grid density (xc_bfogc_update_density) and the energies of it per functional
@@ -835,7 +836,10 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       bf->d_P.alloc(ctx, (size_t)n * n);
       bf->d_F.alloc(ctx, (size_t)n * n);
       bf->d_E.alloc(ctx, 1);
-      if (keep_fp32) bf->d_P32.alloc(ctx, (size_t)n * n);
+      if (keep_fp32) {
+        bf->d_P32.alloc(ctx, (size_t)n * n);
+        bf->d_round.alloc(ctx, kRoundBlocks);
+      }
       HIPX(hipStreamSynchronize(ctx->stream));
     } catch (...) {
       delete bf;
@@ -1065,6 +1069,9 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   if (s == ctx->stream) ctx->mark();
   launch_finalize(s, bf->d_U.p, dF, n, 0);
   launch_sum(s, bf->d_eblk.p, nxc, d_energy);
+  // FP32 contractions see fl32(P): E_xc[P] = E_xc[fl32(P)] + tr(F (P - fl32(P))) to first order
+  if (precision == XC_PREC_FP32)
+    launch_rounding_term(s, dF, dP, bf->d_P32.p, (int64_t)n * n, bf->d_round.p, d_energy);
   if (s == ctx->stream) ctx->mark();
   HIPX(hipGetLastError());
 }
@@ -1131,6 +1138,10 @@ static void enqueue_build_pol(xc_bfogc* bf, int functional, int precision, doubl
     launch_finalize(s, bf->d_U.p, Fs[sp], n, 0);
   }
   launch_sum(s, bf->d_eblk.p, nxc, d_energy);
+  // first-order FP32 rounding term of each spin density matrix (dE_xc/dP_s = F_s)
+  if (precision == XC_PREC_FP32)
+    for (int sp = 0; sp < 2; ++sp)
+      launch_rounding_term(s, Fs[sp], Ps[sp], P32[sp], (int64_t)n * n, bf->d_round.p, d_energy);
   HIPX(hipGetLastError());
 }
 

@@ -364,9 +364,9 @@ class ScfTask:
 
     def final_energy(self, P, precision="FP64"):
         """finalDFTEnergyEvaluation: E_xc on the default grid from the converged P."""
-        return self.final_evaluation(P, precision)[0]
+        return self.final_evaluation(P, precision, update=False)[0]
 
-    def final_evaluation(self, P, precision="FP64"):
+    def final_evaluation(self, P, precision="FP64", update=True):
         """finalDFTEnergyEvaluation (ScfTask.cpp:127-153): the energy of P with V_xc on the default
         grid, and the density of the orbitals updated with that Fock matrix (as the reference leaves
         it in the electronic structure).  With equal grids the SCF density is kept."""
@@ -374,7 +374,7 @@ class ScfTask:
         Vxc = pot.getMatrix(P)
         J = self.ints.coulomb(P)
         e = self.ints.e_nuc + float(np.vdot(P, self.ints.h)) + 0.5 * float(np.vdot(P, J)) + pot.getEnergy()
-        if self.grid_default is self.grid_small:
+        if self.grid_default is self.grid_small or not update:
             return e, P
         X = canonical_orthogonalization(self.ints.S)
         _, Cp = np.linalg.eigh(X.T @ (self.ints.h + J + Vxc) @ X)
@@ -415,6 +415,7 @@ class ScfTask:
                           max_cycles=max_cycles, log=self.say, **MIXED_THRESHOLDS)
             out["cycles_fp32"] = r32["cycles"]
             out["energy_before_fp64_finish"] = self.final_energy(r32["P"], "FP32")
+            out["P_fp32"] = r32["P"]
             r = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "FP64")), P0=r32["P"],
                         max_cycles=max_cycles, log=self.say)
             out.update(cycles_fp64_finish=r["cycles"], converged=r32["converged"] and r["converged"],

@@ -178,6 +178,29 @@ def test_fock_build_fp64_matches_oracle(ctx, kind):
     assert np.abs(Fm - Fo).max() < 1e-4
 
 
+def test_fp32_density_matrix_rounding_term(ctx):
+    """FP32 builds contract fl32(P) and add tr(F (P - fl32(P))) to E_xc in FP64.  Builds of P and of
+    fl32(P) contract identical FP32 operands, so their energies differ by exactly that term; with it
+    the FP32 energy of C20H42/def2-SVP/PBE lands within 1e-6 of FP64 (7.3e-6 without it)."""
+    d = W.build_system(2)
+    bf = xc.BasisFunctionOnGridController(ctx, d["basis"], d["grid"], 128, 1e-9, 1, keep_fp32=True)
+    P = d["P"]
+    P32 = P.astype(np.float32).astype(np.float64)
+    pot = xc.FuncPotential(bf, "PBE", "FP32")
+    F = pot.getMatrix(P)
+    E = pot.getEnergy()
+    F2 = pot.getMatrix(P32)
+    E2 = pot.getEnergy()
+    E64 = xc.FuncPotential(bf, "PBE", "FP64")
+    E64.getMatrix(P)
+    term = float(np.vdot(F, P - P32))
+    say("  FP32 E(P) - E(fl32 P) %.3e, tr(F dP) %.3e, |E32 - E64| %.2e" % (E - E2, term, abs(E - E64.getEnergy())))
+    assert np.abs(F - F2).max() < 1e-12
+    assert abs((E - E2) - term) < 1e-12 and abs(term) > 1e-6
+    assert abs(E - E64.getEnergy()) < 1e-6
+    bf._release()
+
+
 def test_h2_lda_fock_matrix(ctx):
     """FuncPotential_test.cpp:59-100 through the HIP path (Becke grid, Cartesian def2-TZVP)."""
     fx = common.fixtures()
