@@ -54,9 +54,19 @@ struct FockItem {
 };
 constexpr int kFockRows = 256;        // rows a sub-tile item may stage
 constexpr int kFockPanel = 128;       // panel height of blocks split into panel pairs
-constexpr int kFockNumS = 9;          // slot-count classes of the sub-tile kernel
+#ifndef SXC_FOCK_WAVES
+#define SXC_FOCK_WAVES 16
+#endif
+constexpr int kFockWaves = SXC_FOCK_WAVES;  // wavefronts per workgroup of the sub-tile kernel
+#if SXC_FOCK_WAVES == 16
+constexpr int kFockNumS = 5;          // slot-count classes of the sub-tile kernel
+constexpr int kFockS[kFockNumS] = {1, 2, 3, 4, 5};
+inline int fock_smax(int) { return 5; }  // accumulator budget: kFockWaves * 5 sub-tiles per item
+#else
+constexpr int kFockNumS = 9;
 constexpr int kFockS[kFockNumS] = {1, 2, 3, 4, 5, 6, 7, 8, 10};
-inline int fock_smax(int) { return 10; }  // accumulator budget without spills
+inline int fock_smax(int) { return 10; }
+#endif
 
 // Work item of the density kernel: block, a = first column of the pass,
 // b = columns of the pass (multiple of 16), pass = partial-density slot.

@@ -237,13 +237,16 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
   __syncthreads();
   const int* lidx = idx_lds ? sIdx : idx;
 
-  // staging: phi task = (point pa, AO quad qa); P tasks = (column cb, AO quad qb)
+  // staging: phi task = (point pa, AO quad qa); P tasks = (column cb, AO quad qb), 16
+  // consecutive lanes taking 16 columns of one quad (their LDS stores then cover distinct banks)
   const int pa = tid & (kTile - 1), qa = tid >> 7;
+  const int qb = (tid >> 4) & 3;
+  auto col_of = [](int task) { return (task & 15) + 16 * (task >> 6); };
   int nu[NB];
 #pragma unroll
   for (int m = 0; m < NB; ++m) {
-    const int task = tid + 512 * m, cb = task >> 2;
-    nu[m] = (cb < ncols) ? lidx[col0 + cb] : -1;
+    const int task = tid + 512 * m, cb = col_of(task);
+    nu[m] = (task < 4 * NC && cb < ncols) ? lidx[col0 + cb] : -1;
   }
   vec_t rA, rB[NB];
   auto load = [&](int k0) {
@@ -251,7 +254,6 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
     for (int e = 0; e < KV; ++e) rA[e] = tb[(int64_t)(k0 + KV * qa + e) * kTile + pa];
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
-      const int qb = (tid + 512 * m) & 3;
 #pragma unroll
       for (int e = 0; e < KV; ++e) {
         const int mu = lidx[k0 + KV * qb + e];
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
       const int task = tid + 512 * m;
-      if ((task >> 2) < NC) *reinterpret_cast<vec_t*>(&sB[task >> 2][KV * (task & 3)]) = rB[m];
+      if (task < 4 * NC) *reinterpret_cast<vec_t*>(&sB[col_of(task)][KV * qb]) = rB[m];
     }
   };
 
@@ -592,7 +594,7 @@ __global__ __launch_bounds__(128) void functional_pol_kernel(int func, XcParams 
 // read of KV consecutive points of one AO row.  MFMAs are issued in groups of
 // two independent accumulators.
 template <typename T, int S>
-__global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restrict__ work,
+__global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockItem* __restrict__ work,
                                                        const DevBlock* __restrict__ blocks,
                                                        const int* __restrict__ ao_idx, const T* __restrict__ tiles,
                                                        const double* __restrict__ vs, const double* __restrict__ gw,
@@ -603,12 +605,15 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
   constexpr int KV = KC / 4;
   constexpr int RS = KC + 16 / (int)sizeof(T);  // 144-byte LDS rows: the 16 rows of an operand read hit distinct banks
   constexpr int NCH = kTile / KC;
-  constexpr int NTASK = kFockRows * 4 / 512;
-  constexpr int GS = 2;                          // independent accumulators per MFMA group
+  constexpr int NT = 64 * kFockWaves;            // threads
+  constexpr int NTASK = kFockRows * 4 / NT;
+  constexpr int GS = kFockWaves == 16 ? 1 : 2;   // independent accumulators per MFMA group
   typedef T vec_t __attribute__((ext_vector_type(KV)));
   __shared__ __attribute__((aligned(16))) T sPhi[kFockRows][RS];
   __shared__ __attribute__((aligned(16))) T sG[kFockRows][RS];
-  __shared__ __attribute__((aligned(16))) double spot[kTile][4];  // w v, w g_x, w g_y, w g_z of each point
+  // w v / 2, w g_x, w g_y, w g_z of each point, one plane per quantity: the four point runs a
+  // wavefront reads at once then start on distinct banks
+  __shared__ __attribute__((aligned(16))) double spot[4][kTile];
   __shared__ double red[2];
 
   const FockItem it = work[blockIdx.x];
@@ -622,10 +627,10 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
     const double v1 = (in && gga) ? gw[q] : 0.0;
     const double v2 = (in && gga) ? gw[npts + q] : 0.0;
     const double v3 = (in && gga) ? gw[2 * npts + q] : 0.0;
-    spot[tid][0] = 0.5 * v0;
-    spot[tid][1] = v1;
-    spot[tid][2] = v2;
-    spot[tid][3] = v3;
+    spot[0][tid] = 0.5 * v0;
+    spot[1][tid] = v1;
+    spot[2][tid] = v2;
+    spot[3][tid] = v3;
     double a = fabs(v0) + fabs(v1) + fabs(v2) + fabs(v3);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
@@ -661,7 +666,7 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
   int li[S], lj[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const int t = w + 8 * s;
+    const int t = w + kFockWaves * s;
     int i = 0, j = 0;
     if (t < it.nsub) decode(t, i, j);
     li[s] = __builtin_amdgcn_readfirstlane(t < it.nsub ? 16 * i : 0);
@@ -670,13 +675,17 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
 
   vec_t raw[NTASK][4];
   auto grow_of = [&](int lr) { return lr < it.na ? it.a0 + lr : it.b0 + (lr - it.na); };
+  // staging task -> (row, quarter of the chunk): 16 consecutive lanes take 16 rows of one
+  // quarter, so their 16-byte LDS stores (144-byte row stride) cover distinct banks
+  const int q = (tid >> 4) & 3;
+  auto row_of = [&](int task) { return (task & 15) + 16 * (task >> 6); };
   auto load = [&](int c) {
     const int p0 = c * KC;
 #pragma unroll
     for (int m = 0; m < NTASK; ++m) {
-      const int task = tid + 512 * m;
+      const int task = tid + NT * m;
       if (task < 4 * R) {
-        const T* src = tb + (int64_t)grow_of(task >> 2) * kTile + p0 + KV * (task & 3);
+        const T* src = tb + (int64_t)grow_of(row_of(task)) * kTile + p0 + KV * q;
         raw[m][0] = *reinterpret_cast<const vec_t*>(src);
         if (gga) {
           raw[m][1] = *reinterpret_cast<const vec_t*>(src + cstride);
@@ -687,34 +696,29 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
     }
   };
   auto store = [&](int c) {
-    // the KV points of this thread's tasks are the same for every task (task & 3 == tid & 3)
-    const int p0 = c * KC + KV * (tid & 3);
-    typedef double dvec2 __attribute__((ext_vector_type(2)));
-    double wt[KV][4];
+    // the KV points of this thread's tasks are the same for every task
+    const int p0 = c * KC + KV * q;
+    vec_t gv[NTASK];
 #pragma unroll
     for (int e = 0; e < KV; ++e) {
-      const dvec2 lo = *reinterpret_cast<const dvec2*>(&spot[p0 + e][0]);
-      const dvec2 hi = *reinterpret_cast<const dvec2*>(&spot[p0 + e][2]);
-      wt[e][0] = lo[0];
-      wt[e][1] = lo[1];
-      wt[e][2] = hi[0];
-      wt[e][3] = hi[1];
+      const T w0 = (T)spot[0][p0 + e];
+      const T w1 = gga ? (T)spot[1][p0 + e] : T(0);
+      const T w2 = gga ? (T)spot[2][p0 + e] : T(0);
+      const T w3 = gga ? (T)spot[3][p0 + e] : T(0);
+#pragma unroll
+      for (int m = 0; m < NTASK; ++m) {
+        T v = w0 * raw[m][0][e];
+        if (gga) v += w1 * raw[m][1][e] + w2 * raw[m][2][e] + w3 * raw[m][3][e];
+        gv[m][e] = v;
+      }
     }
 #pragma unroll
     for (int m = 0; m < NTASK; ++m) {
-      const int task = tid + 512 * m;
+      const int task = tid + NT * m;
       if (task < 4 * R) {
-        const int lr = task >> 2, q = task & 3;
-        vec_t gv;
-#pragma unroll
-        for (int e = 0; e < KV; ++e) {
-          double v = wt[e][0] * (double)raw[m][0][e];
-          if (gga)
-            v += wt[e][1] * (double)raw[m][1][e] + wt[e][2] * (double)raw[m][2][e] + wt[e][3] * (double)raw[m][3][e];
-          gv[e] = (T)v;
-        }
+        const int lr = row_of(task);
         *reinterpret_cast<vec_t*>(&sPhi[lr][KV * q]) = raw[m][0];
-        *reinterpret_cast<vec_t*>(&sG[lr][KV * q]) = gv;
+        *reinterpret_cast<vec_t*>(&sG[lr][KV * q]) = gv[m];
       }
     }
   };
@@ -758,7 +762,7 @@ __global__ __launch_bounds__(512) void fock_sub_kernel(const FockItem* __restric
   const int* idx = ao_idx + bk.idx_off;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const int t = w + 8 * s;
+    const int t = w + kFockWaves * s;
     if (t >= it.nsub) continue;
     int i, j;
     decode(t, i, j);
@@ -953,8 +957,8 @@ template <typename T, int S>
 static void launch_fock_sub_s(hipStream_t s, const FockItem* work, int nwork, const DevBlock* blocks,
                               const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
                               double thr, double* U, int n, int64_t npts) {
-  hipLaunchKernelGGL((fock_sub_kernel<T, S>), dim3(nwork), dim3(512), 0, s, work, blocks, ao_idx, tiles, vs, gw, gga,
-                     thr, U, n, npts);
+  hipLaunchKernelGGL((fock_sub_kernel<T, S>), dim3(nwork), dim3(64 * kFockWaves), 0, s, work, blocks, ao_idx, tiles,
+                     vs, gw, gga, thr, U, n, npts);
 }
 
 template <typename T>
@@ -971,10 +975,12 @@ void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, con
     SXC_FOCK_S(3)
     SXC_FOCK_S(4)
     SXC_FOCK_S(5)
+#if SXC_FOCK_WAVES != 16
     SXC_FOCK_S(6)
     SXC_FOCK_S(7)
     SXC_FOCK_S(8)
     SXC_FOCK_S(10)
+#endif
 #undef SXC_FOCK_S
   }
   throw std::runtime_error("launch_fock_sub: slot class beyond the accumulator budget of this precision");

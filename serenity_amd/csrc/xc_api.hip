@@ -768,11 +768,11 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
               const int ta = na / 16, tb = nbr / 16;
               const int nsub = nbr == 0 ? ta * (ta + 1) / 2 : ta * tb;
               int sc = 0;
-              while (8 * kFockS[sc] < nsub) ++sc;
+              while (kFockWaves * kFockS[sc] < nsub) ++sc;
               bf->fock_items[fp64][sc].push_back(FockItem{b, a0, na, b0, nbr, nsub});
             };
             const int nt16 = bk.npad / 16;
-            if (bk.npad <= kFockRows && nt16 * (nt16 + 1) / 2 <= 8 * fock_smax(fp64)) {
+            if (bk.npad <= kFockRows && nt16 * (nt16 + 1) / 2 <= kFockWaves * fock_smax(fp64)) {
               add(0, bk.npad, 0, 0);
             } else {
               const int np = (bk.npad + kFockPanel - 1) / kFockPanel;
