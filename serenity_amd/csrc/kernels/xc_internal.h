@@ -180,12 +180,13 @@ void launch_sum_densities(hipStream_t s, const DensityList& list, int64_t npts, 
 void launch_subtract_scalars(hipStream_t s, const DensityList& list, double* out);
 void launch_to_float(hipStream_t s, const double* src, float* dst, int64_t n);
 // F[i + n j] (+)= U[max(i,j) + n min(i,j)] (U holds the lower triangle)
-void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accumulate);
-// mode 0: *out = sum x; 1: *out -= sum x; 2: *out += sum x
-void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int mode = 0);
-// *out += sum_e F[e] (P[e] - P32[e]) over nn elements, through kRoundBlocks partials in part
-constexpr int kRoundBlocks = 256;
-void launch_rounding_term(hipStream_t s, const double* F, const double* P, const float* P32, int64_t nn, double* part,
-                          double* out);
+// F = mirror of U (accumulate: F += it); with P32, part[b] = block b's sum of F (P - P32)
+// (finalize_blocks(n) partials)
+void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accumulate, const double* P = nullptr,
+                     const float* P32 = nullptr, double* part = nullptr);
+int finalize_blocks(int n);
+// *out = sum of x[0..n) and x2[0..n2) (subtract: *out -= that sum)
+void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int subtract = 0, const double* x2 = nullptr,
+                int64_t n2 = 0);
 
 }  // namespace sxc

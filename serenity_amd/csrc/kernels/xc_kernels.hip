@@ -727,6 +727,8 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
 #pragma unroll
   for (int s = 0; s < S; ++s) acc[s] = M::zero();
   const int r = lane & 15, g = lane >> 4;
+  // slots of this wavefront that hold a sub-tile (an item smaller than its class leaves the rest idle)
+  const int slive = it.nsub > w ? (it.nsub - w + kFockWaves - 1) / kFockWaves : 0;
   load(0);
   for (int c = 0; c < NCH; ++c) {
     store(c);
@@ -734,6 +736,7 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     if (c + 1 < NCH) load(c + 1);
 #pragma unroll
     for (int s0 = 0; s0 < S; s0 += GS) {
+      if (s0 >= slive) break;  // wavefront-uniform
       vec_t aP[GS], aG[GS], bP[GS], bG[GS];
 #pragma unroll
       for (int u = 0; u < GS; ++u)
@@ -832,48 +835,50 @@ __global__ void to_float_kernel(const double* __restrict__ s, float* __restrict_
   if (i < n) d[i] = (float)s[i];
 }
 
-__global__ void finalize_kernel(const double* __restrict__ U, double* __restrict__ F, int n, int accumulate) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)n * n) return;
-  const int i = (int)(e % n), j = (int)(e / n);
-  const int lo = i < j ? i : j, hi = i < j ? j : i;
-  const double u = U[hi + (int64_t)n * lo];
-  F[e] = accumulate ? F[e] + u : u;
-}
-
-// mode 0: out = sum; 1: out -= sum; 2: out += sum
-__global__ __launch_bounds__(256) void sum_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ out,
-                                                  int mode) {
-  __shared__ double red[256];
+// F = mirror of the lower-triangle accumulator U (or F += it).  With P32 given, each block
+// also leaves sum_e F[e] (P[e] - P32[e]) in part[blockIdx.x]: the first-order change of
+// E_xc from the FP32-rounded density matrix P32 back to P, since dE_xc/dP = F.
+__global__ __launch_bounds__(256) void finalize_kernel(const double* __restrict__ U, double* __restrict__ F, int n,
+                                                       int accumulate, const double* __restrict__ P,
+                                                       const float* __restrict__ P32, double* __restrict__ part) {
   const int t = threadIdx.x;
-  double s = 0.0;
-  for (int64_t i = t; i < n; i += 256) s += x[i];
-  red[t] = s;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if (t < k) red[t] += red[t + k];
-    __syncthreads();
+  const int64_t e = (int64_t)blockIdx.x * 256 + t;
+  double c = 0.0;
+  if (e < (int64_t)n * n) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const double u = U[hi + (int64_t)n * lo];
+    const double f = accumulate ? F[e] + u : u;
+    F[e] = f;
+    if (P32) c = f * (P[e] - (double)P32[e]);
   }
-  if (t == 0) out[0] = mode == 0 ? red[0] : mode == 1 ? out[0] - red[0] : out[0] + red[0];
-}
-
-// One partial per block of sum_e F[e] (P[e] - P32[e]): the first-order change of E_xc
-// from the FP32-rounded density matrix P32 back to P, since dE_xc/dP = F.
-__global__ __launch_bounds__(256) void rounding_term_kernel(const double* __restrict__ F, const double* __restrict__ P,
-                                                            const float* __restrict__ P32, int64_t nn,
-                                                            double* __restrict__ part) {
+  if (!P32) return;  // uniform over the launch
   __shared__ double red[256];
-  const int t = threadIdx.x;
-  double s = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * 256 + t; e < nn; e += (int64_t)gridDim.x * 256)
-    s += F[e] * (P[e] - (double)P32[e]);
-  red[t] = s;
+  red[t] = c;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
     if (t < k) red[t] += red[t + k];
     __syncthreads();
   }
   if (t == 0) part[blockIdx.x] = red[0];
+}
+
+// sum of x[0..n) and x2[0..n2); mode 0: out = sum; 1: out -= sum
+__global__ __launch_bounds__(256) void sum_kernel(const double* __restrict__ x, int64_t n,
+                                                  const double* __restrict__ x2, int64_t n2, double* __restrict__ out,
+                                                  int mode) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int64_t i = t; i < n; i += 256) s += x[i];
+  for (int64_t i = t; i < n2; i += 256) s += x2[i];
+  red[t] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) red[t] += red[t + k];
+    __syncthreads();
+  }
+  if (t == 0) out[0] = mode == 0 ? red[0] : out[0] - red[0];
 }
 
 inline unsigned grid1d(int64_t n, int bsz) { return (unsigned)((n + bsz - 1) / bsz); }
@@ -1023,21 +1028,17 @@ void launch_to_float(hipStream_t s, const double* src, float* dst, int64_t n) {
   hipLaunchKernelGGL(to_float_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, src, dst, n);
 }
 
-void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accumulate) {
+void launch_finalize(hipStream_t s, const double* U, double* F, int n, int accumulate, const double* P,
+                     const float* P32, double* part) {
   const int64_t nn = (int64_t)n * n;
   if (nn <= 0) return;
-  hipLaunchKernelGGL(finalize_kernel, dim3(grid1d(nn, 256)), dim3(256), 0, s, U, F, n, accumulate);
+  hipLaunchKernelGGL(finalize_kernel, dim3(grid1d(nn, 256)), dim3(256), 0, s, U, F, n, accumulate, P, P32, part);
 }
 
-void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int mode) {
-  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, x, n, out, mode);
-}
+int finalize_blocks(int n) { return (int)grid1d((int64_t)n * n, 256); }
 
-void launch_rounding_term(hipStream_t s, const double* F, const double* P, const float* P32, int64_t nn, double* part,
-                          double* out) {
-  if (nn <= 0) return;
-  hipLaunchKernelGGL(rounding_term_kernel, dim3(kRoundBlocks), dim3(256), 0, s, F, P, P32, nn, part);
-  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, part, (int64_t)kRoundBlocks, out, 2);
+void launch_sum(hipStream_t s, const double* x, int64_t n, double* out, int subtract, const double* x2, int64_t n2) {
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, x, n, x2, n2, out, subtract);
 }
 
 }  // namespace sxc

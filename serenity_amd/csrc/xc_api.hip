@@ -22,6 +22,32 @@ using namespace sxc;
 
 namespace {
 
+/* Work items of one block are consecutive in a list.  Workgroups of a launch are
+ * dealt to the 8 XCDs round-robin (workgroup w runs on XCD w mod 8, from some
+ * starting XCD), each with its own L2.  Deal whole blocks to 8 queues of balanced
+ * length and interleave them, so that one XCD runs a block's items one after
+ * another and its re-reads of the block's tiles hit that XCD's L2. */
+template <typename Item>
+void xcd_order(std::vector<Item>& v) {
+  constexpr int kXcd = 8;
+  std::vector<std::vector<Item>> q(kXcd);
+  for (size_t i = 0; i < v.size();) {
+    size_t j = i;
+    while (j < v.size() && v[j].block == v[i].block) ++j;
+    size_t x = 0;
+    for (size_t y = 1; y < q.size(); ++y)
+      if (q[y].size() < q[x].size()) x = y;
+    q[x].insert(q[x].end(), v.begin() + i, v.begin() + j);
+    i = j;
+  }
+  size_t len = 0;
+  for (const auto& u : q) len = std::max(len, u.size());
+  v.clear();
+  for (size_t k = 0; k < len; ++k)
+    for (const auto& u : q)
+      if (k < u.size()) v.push_back(u[k]);
+}
+
 thread_local std::string g_last_error;
 
 struct XcError : std::runtime_error {
@@ -227,7 +253,7 @@ struct xc_bfogc {
   // per-build work buffers
   DBuf<double> d_part, d_rho, d_grad, d_vs, d_gw, d_eblk, d_U, d_P, d_F, d_v, d_g, d_E;
   DBuf<float> d_P32;
-  DBuf<double> d_round;  // kRoundBlocks partials of the FP32 rounding term
+  DBuf<double> d_round;  // finalize-block partials of the FP32 rounding term, per spin
   // supersystem density of a non-additive build with this controller active (allocated on first use)
   DBuf<double> d_rho_sup, d_grad_sup;
   // cached grid density (xc_bfogc_update_density) and the energies of it per functional
@@ -755,21 +781,23 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
           const int npass = (bk.npad + 16 * kDensMaxS - 1) / (16 * kDensMaxS);
           const int width = ((bk.npad + npass - 1) / npass + 15) / 16 * 16;
           int s = 0;
+          // every pass of a block in the class of the widest, so that they share a launch
           for (int c0 = 0; c0 < bk.npad; c0 += width, ++s) {
             const int ncols = std::min(width, bk.npad - c0);
-            bf->dens_work[(ncols + 15) / 16].push_back(WorkItem{b, c0, ncols, s});
+            bf->dens_work[width / 16].push_back(WorkItem{b, c0, ncols, s});
           }
           bk.npass = s;
           bf->npass_max = std::max(bf->npass_max, s);
           // Fock: the whole upper triangle of 16 x 16 sub-tiles as one item where
           // its accumulators fit, else pairs of 128-row panels
+          // (all items of a block in one slot class, the largest they need, so that they
+          // share a launch and can be ordered for L2 reuse)
           for (int fp64 = 0; fp64 < 2; ++fp64) {
+            std::vector<FockItem> mine;
             auto add = [&](int a0, int na, int b0, int nbr) {
               const int ta = na / 16, tb = nbr / 16;
               const int nsub = nbr == 0 ? ta * (ta + 1) / 2 : ta * tb;
-              int sc = 0;
-              while (kFockWaves * kFockS[sc] < nsub) ++sc;
-              bf->fock_items[fp64][sc].push_back(FockItem{b, a0, na, b0, nbr, nsub});
+              mine.push_back(FockItem{b, a0, na, b0, nbr, nsub});
             };
             const int nt16 = bk.npad / 16;
             if (bk.npad <= kFockRows && nt16 * (nt16 + 1) / 2 <= kFockWaves * fock_smax(fp64)) {
@@ -785,9 +813,17 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
                 }
               }
             }
+            int nsub_max = 0;
+            for (const FockItem& it : mine) nsub_max = std::max(nsub_max, it.nsub);
+            int sc = 0;
+            while (kFockWaves * kFockS[sc] < nsub_max) ++sc;
+            for (const FockItem& it : mine) bf->fock_items[fp64][sc].push_back(it);
           }
         }
       }
+      for (int c = 0; c <= kDensMaxS; ++c) xcd_order(bf->dens_work[c]);
+      for (int f = 0; f < 2; ++f)
+        for (int c = 0; c < kFockNumS; ++c) xcd_order(bf->fock_items[f][c]);
       bf->tile_elems = tile_off;
       // 3. admission by footprint
       size_t free_b = 0, total_b = 0;
@@ -838,7 +874,7 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       bf->d_E.alloc(ctx, 1);
       if (keep_fp32) {
         bf->d_P32.alloc(ctx, (size_t)n * n);
-        bf->d_round.alloc(ctx, kRoundBlocks);
+        bf->d_round.alloc(ctx, 2 * (size_t)finalize_blocks(n));
       }
       HIPX(hipStreamSynchronize(ctx->stream));
     } catch (...) {
@@ -1067,11 +1103,11 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   else
     run_fock<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr, L);
   if (s == ctx->stream) ctx->mark();
-  launch_finalize(s, bf->d_U.p, dF, n, 0);
-  launch_sum(s, bf->d_eblk.p, nxc, d_energy);
-  // FP32 contractions see fl32(P): E_xc[P] = E_xc[fl32(P)] + tr(F (P - fl32(P))) to first order
-  if (precision == XC_PREC_FP32)
-    launch_rounding_term(s, dF, dP, bf->d_P32.p, (int64_t)n * n, bf->d_round.p, d_energy);
+  // FP32 contractions see fl32(P): E_xc[P] = E_xc[fl32(P)] + tr(F (P - fl32(P))) to first order,
+  // the trace formed by the finalize blocks and added with the energy sum
+  const bool fp32 = precision == XC_PREC_FP32;
+  launch_finalize(s, bf->d_U.p, dF, n, 0, fp32 ? dP : nullptr, fp32 ? bf->d_P32.p : nullptr, bf->d_round.p);
+  launch_sum(s, bf->d_eblk.p, nxc, d_energy, 0, fp32 ? bf->d_round.p : nullptr, fp32 ? finalize_blocks(n) : 0);
   if (s == ctx->stream) ctx->mark();
   HIPX(hipGetLastError());
 }
@@ -1135,13 +1171,13 @@ static void enqueue_build_pol(xc_bfogc* bf, int functional, int precision, doubl
       run_fock<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr, L, vs[sp], gw[sp]);
     else
       run_fock<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr, L, vs[sp], gw[sp]);
-    launch_finalize(s, bf->d_U.p, Fs[sp], n, 0);
+    // with the first-order FP32 rounding term of each spin density matrix (dE_xc/dP_s = F_s)
+    const bool fp32 = precision == XC_PREC_FP32;
+    launch_finalize(s, bf->d_U.p, Fs[sp], n, 0, fp32 ? Ps[sp] : nullptr, fp32 ? P32[sp] : nullptr,
+                    fp32 ? bf->d_round.p + (size_t)sp * finalize_blocks(n) : nullptr);
   }
-  launch_sum(s, bf->d_eblk.p, nxc, d_energy);
-  // first-order FP32 rounding term of each spin density matrix (dE_xc/dP_s = F_s)
-  if (precision == XC_PREC_FP32)
-    for (int sp = 0; sp < 2; ++sp)
-      launch_rounding_term(s, Fs[sp], Ps[sp], P32[sp], (int64_t)n * n, bf->d_round.p, d_energy);
+  launch_sum(s, bf->d_eblk.p, nxc, d_energy, 0, precision == XC_PREC_FP32 ? bf->d_round.p : nullptr,
+             precision == XC_PREC_FP32 ? 2 * finalize_blocks(n) : 0);
   HIPX(hipGetLastError());
 }
 
