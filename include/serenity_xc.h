@@ -244,6 +244,17 @@ int xc_fock_build(xc_bfogc* bfogc, int functional, int precision, double block_a
 int xc_fock_build_device(xc_bfogc* bfogc, int functional, int precision, double block_ave_threshold,
                          const double* dP, double* dF, double* energy);
 
+/* Packed builds: FuncPotential::getMatrix + getEnergy (FuncPotential.cpp:73-111) of
+ * nsys controllers at once, for callers with many small systems (serving, embedding
+ * subsystems).  One functional and precision for all; P_i, F_i device n_i x n_i
+ * matrices, energies[i] host.  The work of every system shares one launch per stage
+ * and work class (merged block and work lists with a per-system table); each F_i and
+ * E_xc,i equal those of xc_fock_build_device on the same controller.  The controllers
+ * must be distinct, of one context, with 128-point blocks.  Synchronises before
+ * returning. */
+int xc_fock_build_packed(int nsys, xc_bfogc* const* bfogcs, int functional, int precision,
+                         double block_ave_threshold, const double* const* dP, double* const* dF, double* energies);
+
 /* FuncPotential<UNRESTRICTED>::getMatrix / getEnergy (FuncPotential.cpp:73-111 with
  * SCFMode = UNRESTRICTED): spin density matrices P_a, P_b in, the alpha and beta
  * XC matrices F_a, F_b (overwritten) and E_xc out.  The functional takes the spin

@@ -97,6 +97,8 @@ def lib():
     _sig("xc_fock_add", C.c_int, vp, C.c_double, dp, dp, dp)
     _sig("xc_fock_build", C.c_int, vp, C.c_int, C.c_int, C.c_double, dp, dp, dp)
     _sig("xc_fock_build_device", C.c_int, vp, C.c_int, C.c_int, C.c_double, vp, vp, dp)
+    _sig("xc_fock_build_packed", C.c_int, C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.c_double, C.POINTER(vp),
+         C.POINTER(vp), dp)
     _sig("xc_last_energy", C.c_int, vp, dp)
     _sig("xc_nadd_fock_build", C.c_int, vp, C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.c_double, dp,
          C.POINTER(dp), dp, dp)

@@ -40,6 +40,31 @@ struct DevBlock {
   int shell_off;      // offset into the significant-shell list
   int nshell;         // significant shells
   int npass;          // density column passes (partial rho/grad slots written)
+  int sys;            // system of a packed build (index into its PackSys table; 0 otherwise)
+};
+
+// One system of a packed build: several controllers' builds in one launch per
+// stage.  Work items index a merged block list whose entries keep their own
+// system's offsets (tile_off, first, idx_off) and name the system in DevBlock::sys.
+struct PackSys {
+  const void* tiles;    // AO tiles in the contraction precision
+  const void* P;        // density matrix in the contraction precision
+  const double* P64;    // density matrix (FP64)
+  float* P32;           // FP32 copy (FP32 builds)
+  const int* ao_idx;
+  double* part;         // partial densities
+  const double* w;      // grid weights
+  double* vs;           // w v
+  double* gw;           // w g
+  double* U;            // Fock accumulator
+  double* F;            // XC matrix out
+  double* eblk;         // per-block energies
+  double* round;        // rounding-term partials (FP32)
+  double* E;            // E_xc out (device)
+  int64_t npts;
+  int n;
+  int blk0;             // first merged block of this system
+  int nxc;              // 128-point tiles of its grid
   int pad_;
 };
 
@@ -90,7 +115,7 @@ constexpr int kDensIdxLds = 2048;  // AO index lists up to this length are stage
 template <typename T>
 void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
                         const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
-                        int64_t npts);
+                        int64_t npts, const PackSys* pack = nullptr);
 
 // rho[q] / grad[.][q] from the partials of the block of q (any block size).
 void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
@@ -101,6 +126,9 @@ void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npt
                        const double* grad, const DevBlock* blocks, const double* part, const double* w, double* eps,
                        double* v, double* g, double* vs, double* gw, double* e_blk,
                        const double* rho_sub = nullptr, const double* grad_sub = nullptr);
+// Functional of a packed build: one 128-point block per workgroup over the merged block list.
+void launch_functional_packed(hipStream_t s, int func, const XcParams& prm, int nblocks, const DevBlock* blocks,
+                              const PackSys* pack);
 
 // Unrestricted functional: spin densities/gradients (3 x npts each) in; w v_s, w g_s and
 // block energies out.
@@ -112,7 +140,14 @@ void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t
 template <typename T>
 void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,
                      const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
-                     double block_ave_thr, double* U, int n, int64_t npts);
+                     double block_ave_thr, double* U, int n, int64_t npts, const PackSys* pack = nullptr);
+// Packed elementwise stages over nsys systems (maxnn = largest n^2): P -> FP32, U = 0,
+// F = mirror of U (+ FP32 rounding partials), E = sum of block energies (+ rounding term).
+constexpr int kPackRoundBlocks = 512;  // finalize workgroups per system of a packed build
+void launch_pack_to_float(hipStream_t s, const PackSys* pack, int nsys, int64_t maxnn);
+void launch_pack_clear(hipStream_t s, const PackSys* pack, int nsys, int64_t maxnn);
+void launch_pack_finalize(hipStream_t s, const PackSys* pack, int nsys, int fp32);
+void launch_pack_sum(hipStream_t s, const PackSys* pack, int nsys, int fp32);
 
 // Work item of the gradient kernel: the block's AO columns [col0, col0 + 64).
 struct GradItem {

@@ -204,7 +204,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
                                                           const int* __restrict__ ao_idx,
                                                           const T* __restrict__ tiles, const T* __restrict__ P,
                                                           int n, int want_grad, double* __restrict__ part,
-                                                          int64_t npts) {
+                                                          int64_t npts, const PackSys* __restrict__ pack) {
   using M = Mfma<T>;
   constexpr int KC = 128 / (int)sizeof(T);      // AOs per chunk (128 bytes of each LDS row)
   constexpr int KV = KC / 4;
@@ -221,6 +221,15 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
 
   const WorkItem wk = work[blockIdx.x];
   const DevBlock bk = blocks[wk.block];
+  if (pack) {
+    const PackSys& ps = pack[bk.sys];
+    tiles = static_cast<const T*>(ps.tiles);
+    P = static_cast<const T*>(ps.P);
+    ao_idx = ps.ao_idx;
+    part = ps.part;
+    npts = ps.npts;
+    n = ps.n;
+  }
   const int col0 = wk.a, ncols = wk.b;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -462,9 +471,21 @@ __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm,
                                                          double* __restrict__ vs, double* __restrict__ gw,
                                                          double* __restrict__ e_blk,
                                                          const double* __restrict__ rho_sub,
-                                                         const double* __restrict__ grad_sub) {
+                                                         const double* __restrict__ grad_sub,
+                                                         const PackSys* __restrict__ pack) {
   __shared__ double red[128];
-  const int b = blockIdx.x, t = threadIdx.x;
+  const int bm = blockIdx.x, t = threadIdx.x;  // bm: block in the (merged) block list
+  int b = bm;                                  // block of this system's grid
+  if (pack) {
+    const PackSys& ps = pack[blocks[bm].sys];
+    b = bm - ps.blk0;
+    npts = ps.npts;
+    part = ps.part;
+    w = ps.w;
+    vs = ps.vs;
+    gw = ps.gw;
+    e_blk = ps.eblk;
+  }
   const int64_t first = (int64_t)b * bs;
   const int B = (int)min((int64_t)bs, npts - first);
   const int64_t q = first + t;
@@ -474,7 +495,7 @@ __global__ __launch_bounds__(128) void functional_kernel(int func, XcParams prm,
   if (in) {
     if (part) {
       // sum the density passes of this block (blocks of 128 points)
-      const int np = blocks[b].npass;
+      const int np = blocks[bm].npass;
       for (int s = 0; s < np; ++s) {
         const double* src = part + (int64_t)s * 4 * npts + q;
         r += src[0];
@@ -599,7 +620,7 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
                                                        const int* __restrict__ ao_idx, const T* __restrict__ tiles,
                                                        const double* __restrict__ vs, const double* __restrict__ gw,
                                                        int gga, double thr, double* __restrict__ U, int n,
-                                                       int64_t npts) {
+                                                       int64_t npts, const PackSys* __restrict__ pack) {
   using M = Mfma<T>;
   constexpr int KC = 128 / (int)sizeof(T);       // points per chunk: one 128-byte line of each AO row
   constexpr int KV = KC / 4;
@@ -618,6 +639,16 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
 
   const FockItem it = work[blockIdx.x];
   const DevBlock bk = blocks[it.block];
+  if (pack) {
+    const PackSys& ps = pack[bk.sys];
+    tiles = static_cast<const T*>(ps.tiles);
+    vs = ps.vs;
+    gw = ps.gw;
+    U = ps.U;
+    n = ps.n;
+    npts = ps.npts;
+    ao_idx = ps.ao_idx;
+  }
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (tid < kTile) {
@@ -881,6 +912,63 @@ __global__ __launch_bounds__(256) void sum_kernel(const double* __restrict__ x, 
   if (t == 0) out[0] = mode == 0 ? red[0] : out[0] - red[0];
 }
 
+// ---- packed builds: elementwise stages over the systems of a pack (blockIdx.y = system) ----
+__global__ void pack_to_float_kernel(const PackSys* __restrict__ pack) {
+  const PackSys& ps = pack[blockIdx.y];
+  const int64_t nn = (int64_t)ps.n * ps.n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x)
+    ps.P32[i] = (float)ps.P64[i];
+}
+
+__global__ void pack_clear_kernel(const PackSys* __restrict__ pack) {
+  const PackSys& ps = pack[blockIdx.y];
+  const int64_t nn = (int64_t)ps.n * ps.n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x)
+    ps.U[i] = 0.0;
+}
+
+// F = mirror of U; FP32 builds also leave one partial of sum F (P - P32) per workgroup
+__global__ __launch_bounds__(256) void pack_finalize_kernel(const PackSys* __restrict__ pack, int fp32) {
+  const PackSys& ps = pack[blockIdx.y];
+  const int n = ps.n, t = threadIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  double c = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + t; e < nn; e += (int64_t)gridDim.x * 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const double u = ps.U[hi + (int64_t)n * lo];
+    ps.F[e] = u;
+    if (fp32) c += u * (ps.P64[e] - (double)ps.P32[e]);
+  }
+  if (!fp32) return;  // uniform over the launch
+  __shared__ double red[256];
+  red[t] = c;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) red[t] += red[t + k];
+    __syncthreads();
+  }
+  if (t == 0) ps.round[blockIdx.x] = red[0];
+}
+
+// E_xc of each system: its block energies (+ the FP32 rounding term), in a fixed order
+__global__ __launch_bounds__(256) void pack_sum_kernel(const PackSys* __restrict__ pack, int fp32) {
+  const PackSys& ps = pack[blockIdx.x];
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int i = t; i < ps.nxc; i += 256) s += ps.eblk[i];
+  if (fp32)
+    for (int i = t; i < kPackRoundBlocks; i += 256) s += ps.round[i];
+  red[t] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) red[t] += red[t + k];
+    __syncthreads();
+  }
+  if (t == 0) ps.E[0] = red[0];
+}
+
 inline unsigned grid1d(int64_t n, int bsz) { return (unsigned)((n + bsz - 1) / bsz); }
 
 }  // namespace
@@ -903,34 +991,30 @@ void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const in
 template <typename T>
 void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
                         const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
-                        int64_t npts) {
+                        int64_t npts, const PackSys* pack) {
   if (nwork <= 0) return;
   switch (S) {
 #define SXC_DENS_S(SV)                                                                                        \
   case SV:                                                                                                    \
     hipLaunchKernelGGL((density_sub_kernel<T, SV>), dim3(nwork), dim3(512), 0, s, work, blocks, ao_idx, tiles, \
-                       P, n, want_grad, part, npts);                                                          \
+                       P, n, want_grad, part, npts, pack);                                                    \
     return;
+    static_assert(kDensMaxS == 6, "one case per column-tile class");
     SXC_DENS_S(1)
     SXC_DENS_S(2)
     SXC_DENS_S(3)
     SXC_DENS_S(4)
     SXC_DENS_S(5)
     SXC_DENS_S(6)
-    SXC_DENS_S(7)
-    SXC_DENS_S(8)
-    SXC_DENS_S(9)
-    SXC_DENS_S(10)
-    SXC_DENS_S(11)
-    SXC_DENS_S(12)
 #undef SXC_DENS_S
   }
   throw std::runtime_error("launch_density_sub: column-tile count out of range");
 }
 template void launch_density_sub<double>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
-                                         const double*, const double*, int, int, double*, int64_t);
+                                         const double*, const double*, int, int, double*, int64_t,
+                                         const PackSys*);
 template void launch_density_sub<float>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
-                                        const float*, const float*, int, int, double*, int64_t);
+                                        const float*, const float*, int, int, double*, int64_t, const PackSys*);
 
 void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
                             int want_grad, double* rho, double* grad, int accumulate) {
@@ -946,7 +1030,41 @@ void launch_functional(hipStream_t s, int func, const XcParams& prm, int64_t npt
   const int nb = (int)((npts + bs - 1) / bs);
   if (nb <= 0) return;
   hipLaunchKernelGGL(functional_kernel, dim3(nb), dim3(128), 0, s, func, prm, npts, bs, rho, grad, blocks, part, w,
-                     eps, v, g, vs, gw, e_blk, rho_sub, grad_sub);
+                     eps, v, g, vs, gw, e_blk, rho_sub, grad_sub, (const PackSys*)nullptr);
+}
+
+void launch_functional_packed(hipStream_t s, int func, const XcParams& prm, int nblocks, const DevBlock* blocks,
+                              const PackSys* pack) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(functional_kernel, dim3(nblocks), dim3(128), 0, s, func, prm, (int64_t)0, kTile,
+                     (const double*)nullptr, (const double*)nullptr, blocks, (const double*)nullptr,
+                     (const double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr,
+                     (double*)nullptr, (double*)nullptr, (const double*)nullptr, (const double*)nullptr, pack);
+}
+
+static unsigned pack_grid(int64_t maxnn) {
+  const int64_t g = (maxnn + 255) / 256;
+  return (unsigned)(g < 1 ? 1 : g > 1024 ? 1024 : g);
+}
+
+void launch_pack_to_float(hipStream_t s, const PackSys* pack, int nsys, int64_t maxnn) {
+  if (nsys <= 0) return;
+  hipLaunchKernelGGL(pack_to_float_kernel, dim3(pack_grid(maxnn), nsys), dim3(256), 0, s, pack);
+}
+
+void launch_pack_clear(hipStream_t s, const PackSys* pack, int nsys, int64_t maxnn) {
+  if (nsys <= 0) return;
+  hipLaunchKernelGGL(pack_clear_kernel, dim3(pack_grid(maxnn), nsys), dim3(256), 0, s, pack);
+}
+
+void launch_pack_finalize(hipStream_t s, const PackSys* pack, int nsys, int fp32) {
+  if (nsys <= 0) return;
+  hipLaunchKernelGGL(pack_finalize_kernel, dim3(kPackRoundBlocks, nsys), dim3(256), 0, s, pack, fp32);
+}
+
+void launch_pack_sum(hipStream_t s, const PackSys* pack, int nsys, int fp32) {
+  if (nsys <= 0) return;
+  hipLaunchKernelGGL(pack_sum_kernel, dim3(nsys), dim3(256), 0, s, pack, fp32);
 }
 
 void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* ra,
@@ -961,20 +1079,20 @@ void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t
 template <typename T, int S>
 static void launch_fock_sub_s(hipStream_t s, const FockItem* work, int nwork, const DevBlock* blocks,
                               const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
-                              double thr, double* U, int n, int64_t npts) {
+                              double thr, double* U, int n, int64_t npts, const PackSys* pack) {
   hipLaunchKernelGGL((fock_sub_kernel<T, S>), dim3(nwork), dim3(64 * kFockWaves), 0, s, work, blocks, ao_idx, tiles,
-                     vs, gw, gga, thr, U, n, npts);
+                     vs, gw, gga, thr, U, n, npts, pack);
 }
 
 template <typename T>
 void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,
                      const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga, double thr,
-                     double* U, int n, int64_t npts) {
+                     double* U, int n, int64_t npts, const PackSys* pack) {
   if (nwork <= 0) return;
   switch (kFockS[sc]) {
 #define SXC_FOCK_S(SV) \
   case SV:             \
-    return launch_fock_sub_s<T, SV>(s, work, nwork, blocks, ao_idx, tiles, vs, gw, gga, thr, U, n, npts);
+    return launch_fock_sub_s<T, SV>(s, work, nwork, blocks, ao_idx, tiles, vs, gw, gga, thr, U, n, npts, pack);
     SXC_FOCK_S(1)
     SXC_FOCK_S(2)
     SXC_FOCK_S(3)
@@ -992,9 +1110,10 @@ void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, con
 }
 template void launch_fock_sub<double>(hipStream_t, int, const FockItem*, int, const DevBlock*, const int*,
                                       const double*, const double*, const double*, int, double, double*, int,
-                                      int64_t);
+                                      int64_t, const PackSys*);
 template void launch_fock_sub<float>(hipStream_t, int, const FockItem*, int, const DevBlock*, const int*,
-                                     const float*, const double*, const double*, int, double, double*, int, int64_t);
+                                     const float*, const double*, const double*, int, double, double*, int, int64_t,
+                                     const PackSys*);
 
 void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
                              double* vs, double* gw) {

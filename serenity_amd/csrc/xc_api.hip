@@ -128,8 +128,11 @@ struct Lanes {
   }
 };
 
+struct PackWork;
+
 struct xc_ctx {
   int device = 0;
+  PackWork* pack = nullptr;  // device lists of packed builds (allocated on first use)
   hipStream_t stream = nullptr;
   Lanes lanes;  // helper lanes of builds on the context stream
   std::recursive_mutex mu;
@@ -197,7 +200,25 @@ struct DBuf {
   ~DBuf() { free_(); }
 };
 
+
 }  // namespace
+
+// Merged lists and per-system table of packed builds, grown on demand; a packed build
+// synchronises before it returns, so they are free again for the next one.
+struct PackWork {
+  DBuf<DevBlock> blocks;
+  DBuf<PackSys> sys;
+  DBuf<WorkItem> dens[kDensMaxS + 1];
+  DBuf<FockItem> fock[kFockNumS];
+  DBuf<double> round, E;
+};
+
+template <typename T>
+void upload_grow(xc_ctx* ctx, DBuf<T>& b, const std::vector<T>& v) {
+  if (v.empty()) return;
+  if (b.n < v.size()) b.alloc(ctx, std::max(v.size(), 2 * b.n));
+  HIPX(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+}
 
 struct xc_basis {
   xc_ctx* ctx = nullptr;
@@ -378,6 +399,8 @@ int xc_ctx_destroy(xc_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     ctx->release(ctx->d_energy, sizeof(double));
+    delete ctx->pack;
+    ctx->pack = nullptr;
     ctx->lanes.destroy();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1435,6 +1458,114 @@ int xc_fock_build_device(xc_bfogc* bf, int functional, int precision, double thr
       HIPX(hipStreamSynchronize(ctx->stream));
       *energy = ctx->last_energy;
     }
+  });
+}
+
+int xc_fock_build_packed(int nsys, xc_bfogc* const* bfs, int functional, int precision, double thr,
+                         const double* const* dP, double* const* dF, double* energies) {
+  return guarded([&] {
+    require(nsys >= 1 && bfs && dP && dF && energies, XC_ERR_ARG, "xc_fock_build_packed: bad arguments");
+    xc_ctx* ctx = bfs[0] ? bfs[0]->ctx : nullptr;
+    require(ctx != nullptr, XC_ERR_ARG, "xc_fock_build_packed: null controller");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
+    require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+    const bool gga = functional_is_gga(functional);
+    const bool fp32 = precision == XC_PREC_FP32;
+    std::vector<DevBlock> blocks;
+    std::vector<PackSys> sys(nsys);
+    std::vector<WorkItem> dens[kDensMaxS + 1];
+    std::vector<FockItem> fock[kFockNumS];
+    int64_t maxnn = 0;
+    for (int i = 0; i < nsys; ++i) {
+      xc_bfogc* bf = bfs[i];
+      require(bf && dP[i] && dF[i], XC_ERR_ARG, "xc_fock_build_packed: null controller or matrix");
+      require(bf->ctx == ctx, XC_ERR_ARG, "xc_fock_build_packed: controllers of different contexts");
+      for (int j = 0; j < i; ++j)
+        require(bfs[j] != bf, XC_ERR_ARG, "xc_fock_build_packed: a controller appears twice (shared scratch)");
+      require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
+      require(bf->bs == kTile, XC_ERR_STATE, "xc_fock_build_packed: controllers need 128-point blocks");
+      require(!gga || bf->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
+      require(!fp32 || bf->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+      require(fp32 || bf->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
+      const int n = bf->basis->nao;
+      const int blk0 = (int)blocks.size();
+      for (DevBlock b : bf->blocks) {
+        b.sys = i;
+        blocks.push_back(b);
+      }
+      for (int c = 1; c <= kDensMaxS; ++c)
+        for (WorkItem w : bf->dens_work[c]) {
+          w.block += blk0;
+          dens[c].push_back(w);
+        }
+      for (int c = 0; c < kFockNumS; ++c)
+        for (FockItem f : bf->fock_items[fp32 ? 0 : 1][c]) {
+          f.block += blk0;
+          fock[c].push_back(f);
+        }
+      PackSys& ps = sys[i];
+      ps.tiles = fp32 ? (const void*)bf->d_t32.p : (const void*)bf->d_t64.p;
+      ps.P = fp32 ? (const void*)bf->d_P32.p : (const void*)dP[i];
+      ps.P64 = dP[i];
+      ps.P32 = fp32 ? bf->d_P32.p : nullptr;
+      ps.ao_idx = bf->d_ao_idx.p;
+      ps.part = bf->d_part.p;
+      ps.w = bf->grid->d_w.p;
+      ps.vs = bf->d_vs.p;
+      ps.gw = bf->d_gw.p;
+      ps.U = bf->d_U.p;
+      ps.F = dF[i];
+      ps.eblk = bf->d_eblk.p;
+      ps.npts = bf->grid->npts;
+      ps.n = n;
+      ps.blk0 = blk0;
+      ps.nxc = (int)((bf->grid->npts + kTile - 1) / kTile);
+      maxnn = std::max(maxnn, (int64_t)n * n);
+    }
+    if (!ctx->pack) ctx->pack = new PackWork();
+    PackWork& pw = *ctx->pack;
+    if (pw.round.n < (size_t)nsys * kPackRoundBlocks) pw.round.alloc(ctx, (size_t)2 * nsys * kPackRoundBlocks);
+    if (pw.E.n < (size_t)nsys) pw.E.alloc(ctx, (size_t)2 * nsys);
+    for (int i = 0; i < nsys; ++i) {
+      sys[i].round = pw.round.p + (size_t)i * kPackRoundBlocks;
+      sys[i].E = pw.E.p + i;
+    }
+    upload_grow(ctx, pw.blocks, blocks);
+    upload_grow(ctx, pw.sys, sys);
+    for (int c = 1; c <= kDensMaxS; ++c) upload_grow(ctx, pw.dens[c], dens[c]);
+    for (int c = 0; c < kFockNumS; ++c) upload_grow(ctx, pw.fock[c], fock[c]);
+    hipStream_t s = ctx->stream;
+    const Lanes& L = ctx->lanes;
+    const PackSys* dsys = pw.sys.p;
+    const int g = gga ? 1 : 0;
+    // one launch per stage and work class for every system of the pack
+    if (fp32) launch_pack_to_float(s, dsys, nsys, maxnn);
+    over_lanes(L, kDensMaxS + 1, [&](int c) { return c >= 1 ? (int)dens[c].size() : 0; },
+               [&](int c, hipStream_t st) {
+      if (fp32)
+        launch_density_sub<float>(st, c, pw.dens[c].p, (int)dens[c].size(), pw.blocks.p, nullptr, nullptr, nullptr,
+                                  0, g, nullptr, 0, dsys);
+      else
+        launch_density_sub<double>(st, c, pw.dens[c].p, (int)dens[c].size(), pw.blocks.p, nullptr, nullptr, nullptr,
+                                   0, g, nullptr, 0, dsys);
+    });
+    launch_functional_packed(s, functional, ctx->prm, (int)blocks.size(), pw.blocks.p, dsys);
+    launch_pack_clear(s, dsys, nsys, maxnn);
+    over_lanes(L, kFockNumS, [&](int c) { return (int)fock[c].size(); }, [&](int c, hipStream_t st) {
+      if (fp32)
+        launch_fock_sub<float>(st, c, pw.fock[c].p, (int)fock[c].size(), pw.blocks.p, nullptr, nullptr, nullptr,
+                               nullptr, g, thr, nullptr, 0, 0, dsys);
+      else
+        launch_fock_sub<double>(st, c, pw.fock[c].p, (int)fock[c].size(), pw.blocks.p, nullptr, nullptr, nullptr,
+                                nullptr, g, thr, nullptr, 0, 0, dsys);
+    });
+    launch_pack_finalize(s, dsys, nsys, fp32 ? 1 : 0);
+    launch_pack_sum(s, dsys, nsys, fp32 ? 1 : 0);
+    HIPX(hipGetLastError());
+    HIPX(hipMemcpyAsync(energies, pw.E.p, sizeof(double) * nsys, hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    ctx->last_energy = energies[nsys - 1];
   });
 }
 

@@ -34,7 +34,7 @@ __all__ = [
     "Context", "BasisController", "GridController", "BasisFunctionOnGridController",
     "DensityOnGridCalculator", "FunctionalLibrary", "FunctionalData", "ScalarOperatorToMatrixAdder",
     "FuncPotential", "molecular_grid", "lebedev", "overlap_matrix", "XCError", "ANGSTROM_TO_BOHR",
-    "DensityFittedCoulomb",
+    "DensityFittedCoulomb", "fock_build_packed",
 ]
 
 
@@ -582,6 +582,30 @@ class FuncPotential:
             self._energy = E.value
             return E.value
         return None
+
+
+def fock_build_packed(potentials, dPs, dFs):
+    """XC matrices and energies of several controllers in one packed launch per stage
+    (xc_fock_build_packed): ``potentials`` are FuncPotential objects of distinct controllers
+    sharing one functional, precision and block-average threshold; ``dPs``/``dFs`` device
+    pointers of their density and XC matrices.  Returns the energies (also kept on each
+    potential)."""
+    k = len(potentials)
+    if k == 0:
+        return []
+    p0 = potentials[0]
+    for p in potentials:
+        if (p.functional, p.precision, p.blockAveThreshold) != (p0.functional, p0.precision, p0.blockAveThreshold):
+            raise XCError(1, "fock_build_packed: the potentials differ in functional, precision or threshold")
+    hs = (C.c_void_p * k)(*[p.bfogc.handle for p in potentials])
+    Ps = (C.c_void_p * k)(*[C.c_void_p(x.value if isinstance(x, C.c_void_p) else x) for x in dPs])
+    Fs = (C.c_void_p * k)(*[C.c_void_p(x.value if isinstance(x, C.c_void_p) else x) for x in dFs])
+    E = np.zeros(k)
+    check(_lib.lib().xc_fock_build_packed(k, hs, FUNCTIONALS[p0.functional], PRECISIONS[p0.precision],
+                                          float(p0.blockAveThreshold), Ps, Fs, _d(E)))
+    for p, e in zip(potentials, E):
+        p._energy = float(e)
+    return E.tolist()
 
 
 class NAddFuncPotential:
