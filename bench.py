@@ -149,6 +149,57 @@ def fde_sweep(ctx, steps, warmup, check_oracle):
     return out
 
 
+def packed_leg(ctx, reps=20):
+    """32 small systems of the serving mix ((H2O)n n = 1..8 and CkH2k+2 k = 1..8, def2-SVP and
+    def2-TZVP, PBE): one build each on the context stream (launches of one build after another)
+    vs one packed build of all 32 (one launch per stage and work class), per precision."""
+    from serenity_amd import workloads as W, xc
+    systems = []
+    for basis in ("DEF2-SVP", "DEF2-TZVP"):
+        for kind in ("water", "alkane"):
+            for size in range(1, 9):
+                Z, X = W.water_chain(size) if kind == "water" else W.alkane(size)
+                b = xc.BasisController.from_file(basis, Z, X)
+                g = xc.molecular_grid(Z, X, ctx=ctx)
+                P = W.synthetic_density(b.overlap(), W.n_electrons(Z) // 2, seed=size)
+                bf = xc.BasisFunctionOnGridController(ctx, b, g, 128, 1e-9, 1, keep_fp32=True)
+                n = b.nao
+                dP, dF = ctx.alloc(8 * n * n), ctx.alloc(8 * n * n)
+                ctx.upload(dP, np.asfortranarray(P))
+                systems.append(dict(bf=bf, n=n, dP=dP, dF=dF, npts=g.getNGridPoints()))
+    out = dict(workload="32 small systems: (H2O)n n=1..8 and CkH2k+2 k=1..8, def2-SVP and def2-TZVP, PBE, "
+                        "accuracy-4 grids", systems=len(systems), nao_range=[min(s["n"] for s in systems),
+                                                                            max(s["n"] for s in systems)],
+               grid_points=int(sum(s["npts"] for s in systems)))
+    for prec in ("FP32", "FP64"):
+        pots = [xc.FuncPotential(s["bf"], "PBE", prec) for s in systems]
+        dPs, dFs = [s["dP"] for s in systems], [s["dF"] for s in systems]
+        alone = [p.build_device(s["dP"], s["dF"]) for p, s in zip(pots, systems)]
+        for _ in range(3):
+            xc.fock_build_packed(pots, dPs, dFs)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for p, s in zip(pots, systems):
+                p.build_device(s["dP"], s["dF"], sync=False)
+        ctx.synchronize()
+        t_seq = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            E = xc.fock_build_packed(pots, dPs, dFs)
+        t_pack = (time.perf_counter() - t0) / reps
+        out[prec.lower()] = dict(ms_one_by_one=round(t_seq * 1e3, 4), ms_packed=round(t_pack * 1e3, 4),
+                                 builds_per_s_one_by_one=round(len(systems) / t_seq, 1),
+                                 builds_per_s_packed=round(len(systems) / t_pack, 1),
+                                 speedup=round(t_seq / t_pack, 3),
+                                 max_dE_packed_vs_alone=float(max(abs(a - b) for a, b in zip(E, alone))))
+    for s in systems:
+        ctx.free(s["dP"])
+        ctx.free(s["dF"])
+        s["bf"]._release()
+    return out
+
+
 def scf_leg(ctx, check_oracle, config=1, cycle_log=None):
     """Full SCF of a benchmark configuration (RI-J): FP64, mixed with FP64 finish, and (optionally) the
     oracle-driven SCF.  Config 1 computes J on the host, config 3 with the GPU J engine."""
@@ -208,6 +259,7 @@ def main():
     ap.add_argument("--no-fde", action="store_true", help="skip the embedding (non-additive potential) sweep")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-scf", action="store_true", help="skip the full SCFs of configs 1 and 3")
+    ap.add_argument("--no-packed", action="store_true", help="skip the packed-build comparison of small systems")
     ap.add_argument("--scf3", action="store_true",
                     help="also run the full SCF of (H2O)64/def2-TZVP/PBE (FP64 and mixed; a few minutes)")
     ap.add_argument("--scf3-oracle", action="store_true",
@@ -314,6 +366,15 @@ def main():
              fde["ms_direct_nadd_build"]))
         out_extra["fde"] = fde
 
+    if not args.no_packed:
+        pk = packed_leg(ctx)
+        log("packed builds of %d small systems (%d-%d AOs, %d points): FP32 %.3f ms one by one, %.3f ms packed "
+            "(%.2fx); FP64 %.3f / %.3f ms (%.2fx)" %
+            (pk["systems"], pk["nao_range"][0], pk["nao_range"][1], pk["grid_points"], pk["fp32"]["ms_one_by_one"],
+             pk["fp32"]["ms_packed"], pk["fp32"]["speedup"], pk["fp64"]["ms_one_by_one"], pk["fp64"]["ms_packed"],
+             pk["fp64"]["speedup"]))
+        out_extra["packed"] = pk
+
     if not args.no_scf:
         sc = scf_leg(ctx, not args.no_cpu_baseline)
         log("SCF (H2O)8/SVP/LDA: FP64 E %.10f (%d cycles, %.2f s); mixed before finish dE %.2e, after %.2e "
@@ -344,6 +405,18 @@ def main():
         log("serving: %d requests at %.0f QPS: p50 %.3f ms, p99 %.3f ms, %.1f builds/s, max dE vs FP64 %.2e" %
             (sv["requests"], sv["qps_offered"], sv["p50_ms"], sv["p99_ms"], sv["builds_per_s"],
              sv["max_dE_xc_fp32_vs_fp64"]))
+        # the same stream with waiting small requests packed into one launch per stage
+        svp = SV.run_stream(ctx, stream, nslots=4, log=log, pack_nao=SV.PACK_NAO, host=host)
+        ep = svp.pop("_energies")
+        svp.pop("_host")
+        svp["max_dE_packed_vs_slots"] = float(max(abs(ep[k] - energies[k]) for k in ep))
+        log("serving with packing (<= %d AOs): p50 %.3f ms, p99 %.3f ms, %.1f builds/s, %d requests in %d packs, "
+            "max dE vs the unpacked run %.2e" %
+            (svp["pack_max_nao"], svp["p50_ms"], svp["p99_ms"], svp["builds_per_s"], svp["packed_requests"],
+             svp["packed_launches"], svp["max_dE_packed_vs_slots"]))
+        sv["packed"] = {k: v for k, v in svp.items() if k in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "builds_per_s",
+                                                               "pack_max_nao", "packed_requests", "packed_launches",
+                                                               "max_dE_packed_vs_slots")}
         out_extra["serving"] = sv
 
     line = dict(

@@ -12,7 +12,9 @@ build of its system with FP32-MFMA contractions.
 Continuous batching: arrived requests are dispatched in arrival order whenever a
 server slot is free and their system has no build in flight; builds of
 different systems overlap on the device (one stream per slot, HIP-graph replay
-of each system's build).  Latency = completion - arrival.
+of each system's build).  With packing on, waiting requests of small systems
+(idle, one functional) are first gathered into one packed build: one launch
+per stage and work class for all of them.  Latency = completion - arrival.
 
     python -m serenity_amd.serving [--requests 2000] [--qps 400] [--slots 4]
 """
@@ -28,6 +30,7 @@ from . import workloads as W
 from . import xc
 
 BASES = ("DEF2-SVP", "DEF2-TZVP")
+PACK_NAO = 250  # largest system (basis functions) whose waiting requests are packed
 FUNCTIONALS = ("LDA", "PBE")
 
 
@@ -101,10 +104,15 @@ class ResidentSystems:
         return bf
 
 
-def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="FP32", log=None):
-    """Serve the stream in real time; returns latency percentiles, throughput and counts."""
+def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="FP32", log=None, pack_nao=0,
+               host=None):
+    """Serve the stream in real time; returns latency percentiles, throughput and counts.
+
+    pack_nao > 0: pending requests of idle systems with at most that many basis functions are
+    gathered (per functional) and built together with one packed launch per stage
+    (xc.fock_build_packed) when two or more are waiting; the rest go to the server slots."""
     say = log or (lambda *a: None)
-    host = HostSystems(ctx=ctx)
+    host = host or HostSystems(ctx=ctx)
     res = ResidentSystems(ctx, host, budget_gb * 1e9, tiles="only" if precision == "FP32" else False)
     keys = sorted({k for _, k, _ in stream})
     t0 = time.perf_counter()
@@ -134,6 +142,33 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
     inflight = {}  # ticket -> request index
     busy = set()
     energies = {}
+    packbuf = {}  # key -> (dP, dF) device matrices of systems built in packs
+    npacked = npacks = 0
+
+    def run_pack(group):
+        """One packed build of the pending requests in group (distinct idle systems, one functional)."""
+        pots, dPs, dFs = [], [], []
+        keep = busy | {stream[j][1] for j in group}  # no member of the pack may be evicted for another
+        for j in group:
+            _, key, func = stream[j]
+            d = host.get(key)
+            nao = d["basis"].nao
+            if key not in packbuf:
+                dP, dF = ctx.alloc(8 * nao * nao), ctx.alloc(8 * nao * nao)
+                ctx.upload(dP, np.asfortranarray(d["P"]))
+                packbuf[key] = (dP, dF)
+            pots.append(xc.FuncPotential(res.get(key, keep), func, precision))
+            dPs.append(packbuf[key][0])
+            dFs.append(packbuf[key][1])
+        E = xc.fock_build_packed(pots, dPs, dFs)
+        for j, e, dF in zip(group, E, dFs):
+            _, key, func = stream[j]
+            nao = host.get(key)["basis"].nao
+            F = np.empty((nao, nao), order="F")
+            ctx.download(F, dF)
+            energies[(key, func)] = e
+        return time.perf_counter() - t0
+
     i = nxt = 0
     t0 = time.perf_counter()
     while nxt < n:
@@ -141,6 +176,25 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
         while i < n and arrive[i] <= now:
             pending.append(i)
             i += 1
+        if pack_nao:
+            groups = {}
+            for j in pending:
+                _, key, func = stream[j]
+                if key in busy or host.get(key)["basis"].nao > pack_nao:
+                    continue
+                g = groups.setdefault(func, {})
+                g.setdefault(key, j)  # one request per system in a pack
+            for func, g in groups.items():
+                if len(g) < 2:
+                    continue
+                group = list(g.values())
+                t_done = run_pack(group)
+                for j in group:
+                    done_at[j] = t_done
+                    pending.remove(j)
+                    nxt += 1
+                npacked += len(group)
+                npacks += 1
         # dispatch in arrival order; a request whose system is busy waits, later ones may pass it
         for j in list(pending):
             _, key, func = stream[j]
@@ -166,6 +220,9 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
     lat = (done_at - arrive) * 1e3
     st = srv.stats()
     srv._release()
+    for dP, dF in packbuf.values():
+        ctx.free(dP)
+        ctx.free(dF)
     for bf, _ in res.lru.values():
         bf._release()
     return dict(requests=n, qps_offered=round(n / arrive[-1], 2), slots=nslots, precision=precision,
@@ -174,6 +231,7 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
                 builds_per_s=round(n / wall, 2), wall_s=round(wall, 3), distinct_systems=len(keys),
                 resident_gb=round(res.bytes / 1e9, 2), created=res.created, evicted=res.evicted,
                 graph_launches=st["graph_launches"], host_setup_s=round(t_host, 1), device_setup_s=round(t_dev, 1),
+                pack_max_nao=pack_nao, packed_requests=npacked, packed_launches=npacks,
                 _energies=energies, _host=host)
 
 
@@ -200,11 +258,13 @@ def main():
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--fp64-sample", type=int, default=16)
+    ap.add_argument("--pack-nao", type=int, default=0,
+                    help="pack waiting requests of systems with at most this many basis functions (0: off)")
     args = ap.parse_args()
     log = lambda *a: print("[serving]", *a, file=sys.stderr, flush=True)
     ctx = xc.Context(0)
     stream = make_stream(args.requests, args.qps)
-    out = run_stream(ctx, stream, nslots=args.slots, budget_gb=args.budget_gb, log=log)
+    out = run_stream(ctx, stream, nslots=args.slots, budget_gb=args.budget_gb, log=log, pack_nao=args.pack_nao)
     energies, host = out.pop("_energies"), out.pop("_host")
     out["max_dE_xc_fp32_vs_fp64"], out["fp64_sample"] = fp64_check(ctx, host, energies, args.fp64_sample)
     print(json.dumps(out), flush=True)

@@ -123,3 +123,19 @@ def test_device_grid_partition_matches_host(ctx, grid_type):
         assert nh == nd and gh.getNGridPoints() == gd.getNGridPoints()
         assert np.array_equal(gh.points, gd.points)
         assert np.abs(gh.weights - gd.weights).max() <= 1e-14 * np.abs(gh.weights).max()
+
+
+def test_serving_with_packing_equals_unpacked(ctx):
+    """A short burst of the serving mix served with and without packing of small requests:
+    every request's E_xc is the same (packed builds run the same kernels on the same items)."""
+    from serenity_amd import serving as SV
+    stream = SV.make_stream(48, 1.0e6)  # all arrive at once: small requests wait together
+    host = SV.HostSystems(ctx=ctx)
+    plain = SV.run_stream(ctx, stream, nslots=2, pack_nao=0, host=host)
+    packed = SV.run_stream(ctx, stream, nslots=2, pack_nao=SV.PACK_NAO, host=host)
+    e0, e1 = plain.pop("_energies"), packed.pop("_energies")
+    say("serving burst: %d requests packed in %d launches; p50 %.2f ms packed vs %.2f ms" %
+        (packed["packed_requests"], packed["packed_launches"], packed["p50_ms"], plain["p50_ms"]))
+    assert packed["packed_requests"] >= 2 and packed["packed_launches"] >= 1
+    assert set(e0) == set(e1)
+    assert max(abs(e0[k] - e1[k]) for k in e0) < 1e-10
