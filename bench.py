@@ -91,7 +91,7 @@ def fde_sweep(ctx, steps, warmup, check_oracle):
     from serenity_amd import xc, workloads as W
     emb = W.build_embedding(24, accuracy=4)
     g, subs = emb["grid"], emb["subsystems"]
-    bfs = [xc.BasisFunctionOnGridController(ctx, s["basis"], g, 128, 1e-9, 1) for s in subs]
+    bfs = [xc.BasisFunctionOnGridController(ctx, s["basis"], g, 128, 1e-9, 1, keep_fp32=True) for s in subs]
     dP, dF = [], []
     for s in subs:
         n = s["basis"].nao
@@ -129,11 +129,34 @@ def fde_sweep(ctx, steps, warmup, check_oracle):
         direct.build_device(dP[a], [dP[i] for i in env], dF[a], sync=False)
     ctx.synchronize()
     ms_direct = (time.perf_counter() - t0) * 1e3 / steps
+    # each subsystem's own PBE XC matrix, all 25 in one packed build per cycle (FP32 contractions; an
+    # FP64 packed build finishes), against the same builds issued one by one
+    dG = [ctx.alloc(8 * s["basis"].nao ** 2) for s in subs]
+    own = {}
+    for prec in ("FP32", "FP64"):
+        pots_own = [xc.FuncPotential(bf, "PBE", prec) for bf in bfs]
+        alone = [p.build_device(dP[i], dG[i]) for i, p in enumerate(pots_own)]
+        xc.fock_build_packed(pots_own, dP, dG)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for i, p in enumerate(pots_own):
+                p.build_device(dP[i], dG[i], sync=False)
+        ctx.synchronize()
+        t_one = (time.perf_counter() - t0) * 1e3 / steps
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            E = xc.fock_build_packed(pots_own, dP, dG)
+        t_pack = (time.perf_counter() - t0) * 1e3 / steps
+        own[prec] = dict(E=E, ms_one_by_one=round(t_one, 4), ms_packed=round(t_pack, 4),
+                         max_dE_packed_vs_alone=float(max(abs(a - b) for a, b in zip(E, alone))))
+    own_xc = dict(fp32=dict((k, v) for k, v in own["FP32"].items() if k != "E"),
+                  fp64=dict((k, v) for k, v in own["FP64"].items() if k != "E"),
+                  max_dE_fp32_vs_fp64=float(max(abs(a - b) for a, b in zip(own["FP32"]["E"], own["FP64"]["E"]))))
     out = dict(workload="C6H6+24H2O/def2-SVP, PBE + TF non-additive potentials, supersystem SSF accuracy-4 grid, "
                         "25 subsystems, synthetic subsystem densities",
                npoints=g.getNGridPoints(), nao_total=int(sum(s["basis"].nao for s in subs)),
                density_updates_per_sweep=len(subs), nadd_builds_per_sweep=2 * len(subs), ms_per_sweep=round(ms, 4),
-               ms_direct_nadd_build=round(ms_direct, 4))
+               ms_direct_nadd_build=round(ms_direct, 4), subsystem_xc_builds=own_xc)
     if check_oracle:
         from oracle import oracle as O
         E = pots[0].build_cached(dF[a], sync=True)
@@ -144,7 +167,7 @@ def fde_sweep(ctx, steps, warmup, check_oracle):
                                    subs[a]["P"], [subs[i]["P"] for i in env], "PBE")
         out["benzene_pbe_nadd_dE_vs_oracle"] = abs(E - Eo)
         out["benzene_pbe_nadd_max_dF_vs_oracle"] = float(np.abs(F.reshape(n, n, order="F") - Fo).max())
-    for d in dP + dF:
+    for d in dP + dF + dG:
         ctx.free(d)
     return out
 
@@ -364,6 +387,11 @@ def main():
         log("FDE sweep: %.3f ms (%d density updates + %d cached NAdd builds); direct NAdd build %.3f ms" %
             (fde["ms_per_sweep"], fde["density_updates_per_sweep"], fde["nadd_builds_per_sweep"],
              fde["ms_direct_nadd_build"]))
+        ox = fde["subsystem_xc_builds"]
+        log("FDE subsystem XC builds (25, packed): FP32 %.3f ms packed vs %.3f ms one by one; FP64 %.3f / %.3f ms; "
+            "max dE FP32 vs FP64 %.2e" % (ox["fp32"]["ms_packed"], ox["fp32"]["ms_one_by_one"],
+                                         ox["fp64"]["ms_packed"], ox["fp64"]["ms_one_by_one"],
+                                         ox["max_dE_fp32_vs_fp64"]))
         out_extra["fde"] = fde
 
     if not args.no_packed:
