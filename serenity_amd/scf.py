@@ -27,6 +27,8 @@ The linear algebra (eigh, DIIS, the density-fitted J contractions) is numpy on
 the host: at the sizes the SCF targets it is small beside the XC builds.
 """
 import ctypes as C
+import json
+import os
 import time
 
 import numpy as np
@@ -313,6 +315,51 @@ def gpu_xc_build(pot):
     return build
 
 
+GUESS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "initial_guess.json")
+ELEMENT_SYMBOLS = {1: "H", 2: "HE", 6: "C", 7: "N", 8: "O"}
+
+
+def atomic_density_guess(basis_name, Z, spherical=True):
+    """Serenity's ATOM_SCF initial guess (AtomicDensityGuessCalculator.cpp:84-181): for every element the
+    stored spherically averaged free-atom density matrix in the MINAO basis is projected into the
+    atom's target basis, P_t = X P_m X^T with X the least-squares solution of S_tt X = S_tm (SVD,
+    singular values below 1e-6 of the largest dropped, as the reference's JacobiSVD threshold), and
+    the atom blocks are placed on the diagonal in atom order."""
+    table = json.load(open(GUESS_FILE))["spherical" if spherical else "cartesian"]
+    origin = np.zeros((1, 3))
+    per_element, blocks = {}, []
+    for z in np.asarray(Z, dtype=int):
+        sym = ELEMENT_SYMBOLS.get(int(z))
+        if sym is None or sym not in table:
+            raise xc.XCError(1, "atomic_density_guess: no stored free-atom density for Z = %d" % z)
+        if sym not in per_element:
+            bt = xc.BasisController.from_file(basis_name, [z], origin, spherical=spherical)
+            bm = xc.BasisController.from_file("MINAO", [z], origin, spherical=spherical)
+            both = xc.BasisController(np.concatenate([bt.l, bm.l]), np.concatenate([bt.pure, bm.pure]),
+                                      np.concatenate([bt.nprim, bm.nprim]), np.vstack([bt.center, bm.center]),
+                                      np.concatenate([bt.alpha, bm.alpha]), np.concatenate([bt.coef, bm.coef]))
+            S = both.overlap()
+            nt = bt.nao
+            U, sv, Vt = np.linalg.svd(S[:nt, :nt])
+            keep = sv > 1e-6 * sv[0]
+            X = Vt[keep].T @ ((U[:, keep].T @ S[:nt, nt:]) / sv[keep][:, None])
+            d = table[sym]
+            Pm = np.asarray(d["P"], dtype=np.float64).reshape(d["n"], d["n"])
+            if Pm.shape[0] != bm.nao:
+                raise xc.XCError(1, "atomic_density_guess: stored %s density has %d functions, MINAO %d"
+                                 % (sym, Pm.shape[0], bm.nao))
+            per_element[sym] = X @ Pm @ X.T
+        blocks.append(per_element[sym])
+    n = sum(b.shape[0] for b in blocks)
+    P = np.zeros((n, n))
+    o = 0
+    for b in blocks:
+        k = b.shape[0]
+        P[o:o + k, o:o + k] = b
+        o += k
+    return P
+
+
 class ScfTask:
     """ScfTask<RESTRICTED>::run for a KS-DFT system on the GPU XC build.
 
@@ -323,7 +370,7 @@ class ScfTask:
     def __init__(self, ctx, Z, coords_bohr, basis="DEF2-SVP", functional="LDA", ri=True, aux="RI_J_WEIGEND",
                  accuracy=4, small_accuracy=2, spherical=True, schwarz=1e-14, fp32_tiles=True, log=None,
                  aux_spherical=None, radial_threshold=1e-9, block_ave_threshold=1e-11, highest_derivative=1,
-                 spin=0, unrestricted=False, gpu_coulomb=False):
+                 spin=0, unrestricted=False, gpu_coulomb=False, guess="ATOM_SCF"):
         self.ctx = ctx
         self.Z = np.ascontiguousarray(Z, dtype=np.int32)
         self.X = np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
@@ -348,6 +395,13 @@ class ScfTask:
         self.grid_small = xc.molecular_grid(self.Z, self.X, accuracy=small_accuracy, ctx=ctx)
         self.grid_default = (self.grid_small if accuracy == small_accuracy
                              else xc.molecular_grid(self.Z, self.X, accuracy=accuracy, ctx=ctx))
+        # initial guess (Settings::scf.initialguess, default ATOM_SCF); H_CORE: core Hamiltonian orbitals
+        if guess not in ("ATOM_SCF", "H_CORE"):
+            raise xc.XCError(1, "ScfTask: unknown initial guess " + str(guess))
+        self.guess = guess
+        self.P_guess = atomic_density_guess(basis, self.Z, spherical) if guess == "ATOM_SCF" else None
+        if self.P_guess is not None and self.P_guess.shape[0] != self.basis.nao:
+            raise xc.XCError(1, "ScfTask: guess density does not match the basis")
         self.setup_s = time.perf_counter() - t0
         self.fp32_tiles = fp32_tiles
         self._bf = {}
@@ -405,6 +459,8 @@ class ScfTask:
         t0 = time.perf_counter()
         out = dict(mode=mode, nao=self.basis.nao, naux=None if self.auxbasis is None else self.auxbasis.nao,
                    npts_small=self.grid_small.getNGridPoints(), npts_default=self.grid_default.getNGridPoints())
+        if P0 is None:
+            P0 = self.P_guess
         if mode == "FP64":
             r = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "FP64")), P0=P0,
                         max_cycles=max_cycles, log=self.say)

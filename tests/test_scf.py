@@ -20,7 +20,11 @@ energy from the converged density, ScfTask.cpp:127-153, 358-359):
 * ethane/def2-SVP Hartree-Fock with exact four-centre J and K, Cartesian and spherical d
   shells, src/tasks/ScfTask_test.cpp:43-88: -79.1786812939 and -79.1728833551 at 1e-8 --
   no XC involved, so this pins the integrals (s, p, d; both AO conventions) and the SCF.
+* the ATOM_SCF initial guess, src/scf/initialGuess/AtomicDensityGuessCalculator_test.cpp:42-56:
+  O2/STO-6G density elements after one HF Fock diagonalisation of the guess, at 1e-6.
 """
+import json
+
 import numpy as np
 import pytest
 
@@ -140,3 +144,46 @@ def test_nuclear_repulsion():
     e = scf.nuclear_repulsion(Z, X)
     d = lambda a, b: np.linalg.norm(X[a] - X[b])
     assert abs(e - (1 / d(0, 1) + 8 / d(0, 2) + 8 / d(1, 2))) < 1e-13
+
+
+def test_atom_scf_guess_matches_reference():
+    """ATOM_SCF initial guess (AtomicDensityGuessCalculator_test.cpp:42-56): O2 / STO-6G at 1.21 A; the
+    stored MINAO free-atom densities projected into STO-6G, one HF Fock matrix of that density
+    (Settings' default method) diagonalised; the reference's density elements to 1e-6.  The projection
+    of MINAO onto itself is the identity, and the guess of water holds ~10 electrons."""
+    Z = np.array([8, 8])
+    X = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.21 * xc.ANGSTROM_TO_BOHR]])
+    b = xc.BasisController.from_file("STO-6G", Z, X)
+    ints = scf.ScfIntegrals(b, Z, X)
+    P0 = scf.atomic_density_guess("STO-6G", Z)
+    assert np.abs(P0[:5, 5:]).max() == 0.0  # atom blocks only
+    F = ints.h + ints.coulomb(P0) - 0.5 * ints.exchange(P0)
+    Xo = scf.canonical_orthogonalization(ints.S)
+    _, Cp = np.linalg.eigh(Xo.T @ F @ Xo)
+    C = Xo @ Cp[:, :8]
+    P = 2.0 * C @ C.T
+    ref = {(0, 0): 2.1071094855194836, (1, 0): -0.46128984211922747, (3, 3): 0.88024781372049266,
+           (5, 0): -7.3034439855780155e-05, (0, 2): 0.0}
+    for k, v in ref.items():
+        assert abs(P[k] - v) < 1e-6, (k, P[k], v)
+    Pm = scf.atomic_density_guess("MINAO", [8])
+    stored = json.load(open(scf.GUESS_FILE))["spherical"]["O"]
+    assert np.abs(Pm - np.reshape(stored["P"], (5, 5))).max() < 1e-14
+    Zw, Xw = water_geometry()
+    bw = xc.BasisController.from_file("DEF2-SVP", Zw, Xw)
+    assert abs(np.vdot(scf.atomic_density_guess("DEF2-SVP", Zw), bw.overlap()) - 10.0) < 1e-2
+
+
+def test_atom_scf_guess_converges_to_the_same_energy():
+    """Water / def2-SVP / LDA with RI-J from the ATOM_SCF and the core-Hamiltonian guesses: the same
+    SCF energy, in fewer cycles from the atomic densities."""
+    Z, X = water_geometry()
+    b = xc.BasisController.from_file("DEF2-SVP", Z, X)
+    a = xc.BasisController.from_file("RI_J_WEIGEND", Z, X)
+    ints = scf.ScfIntegrals(b, Z, X, a)
+    grid = xc.molecular_grid(Z, X, accuracy=2)
+    build = oracle_xc(b, grid, "LDA")
+    r_core = scf.perform(ints, 5, build)
+    r_atom = scf.perform(ints, 5, build, P0=scf.atomic_density_guess("DEF2-SVP", Z))
+    assert abs(r_core["energy"] - r_atom["energy"]) < 1e-9
+    assert r_atom["cycles"] < r_core["cycles"]
