@@ -260,7 +260,7 @@ def scf_leg(ctx, check_oracle, config=1, cycle_log=None):
         def build(P):
             F, E, _, _ = O.fock_build(b, gs.points, gs.weights, P, f)
             return E, F
-        r = SCF.perform(task.ints, task.nocc, build, log=cycle_log)
+        r = SCF.perform(task.ints, task.nocc, build, P0=task.P_guess, log=cycle_log)
         _, e_xc, _, _ = O.fock_build(b, gd.points, gd.weights, r["P"], f)
         e = SCF.total_energy(task.ints, r["P"], e_xc)
         out["cpu_oracle"] = dict(energy=e, cycles=r["cycles"], wall_s=round(time.perf_counter() - t0, 3),

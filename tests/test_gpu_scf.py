@@ -26,7 +26,7 @@ def ctx():
 
 def oracle_scf(task):
     """The same SCF (integrals, grids, protocol) with the oracle's XC build."""
-    r = scf.perform(task.ints, task.nocc, oracle_xc(task.basis, task.grid_small, task.functional))
+    r = scf.perform(task.ints, task.nocc, oracle_xc(task.basis, task.grid_small, task.functional), P0=task.P_guess)
     _, e_xc, _, _ = O.fock_build(task.basis, task.grid_default.points, task.grid_default.weights, r["P"],
                                  task.functional)
     return scf.total_energy(task.ints, r["P"], e_xc), r
