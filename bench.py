@@ -427,6 +427,7 @@ def main():
         stream = SV.make_stream(args.serving_requests, args.serving_qps)
         sv = SV.run_stream(ctx, stream, nslots=4, log=log)
         energies, host = sv.pop("_energies"), sv.pop("_host")
+        sv.pop("_failed")
         sv["max_dE_xc_fp32_vs_fp64"], sv["fp64_sample"] = SV.fp64_check(ctx, host, energies, 8)
         sv["workload"] = ("Poisson stream: (H2O)_n n~U{1..48} / CkH2k+2 k~U{1..30}, def2-SVP/TZVP, LDA/PBE, "
                           "FP32-MFMA builds on resident FP32 tiles")
@@ -437,6 +438,7 @@ def main():
         svp = SV.run_stream(ctx, stream, nslots=4, log=log, pack_nao=SV.PACK_NAO, host=host)
         ep = svp.pop("_energies")
         svp.pop("_host")
+        svp.pop("_failed")
         svp["max_dE_packed_vs_slots"] = float(max(abs(ep[k] - energies[k]) for k in ep))
         log("serving with packing (<= %d AOs): p50 %.3f ms, p99 %.3f ms, %.1f builds/s, %d requests in %d packs, "
             "max dE vs the unpacked run %.2e" %

@@ -322,10 +322,10 @@ GridProblem grid_problem(const std::vector<int>& Z, const std::vector<double>& x
   if (nAtoms == 0) throw std::runtime_error("grid: empty geometry");
   if ((int)xyz.size() != 3 * nAtoms) throw std::runtime_error("grid: coordinate array size mismatch");
   for (int i = 0; i < nAtoms; ++i) check_Z(Z[i]);
+  // atoms closer than 1e-5 bohr count as duplicates (Geometry::hasIdenticalAtoms, Geometry.cpp:540-549)
   for (int i = 0; i < nAtoms; ++i)
     for (int j = 0; j < i; ++j)
-      if (xyz[3 * i] == xyz[3 * j] && xyz[3 * i + 1] == xyz[3 * j + 1] && xyz[3 * i + 2] == xyz[3 * j + 2])
-        throw std::runtime_error("grid construction with duplicated atoms");
+      if (dist3(&xyz[3 * i], &xyz[3 * j]) < 1e-5) throw std::runtime_error("grid construction with duplicated atoms");
 
   std::vector<double> dist((size_t)nAtoms * nAtoms);
   for (int i = 0; i < nAtoms; ++i)

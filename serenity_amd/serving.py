@@ -133,7 +133,10 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
         for k in keys:
             d = host.get(k)
             for f in {f for _, kk, f in stream if kk == k}:
-                t = srv.submit(res.get(k, ()), f, precision, d["P"])
+                try:
+                    t = srv.submit(res.get(k, ()), f, precision, d["P"])
+                except xc.XCError:
+                    continue  # the request fails when it is served
                 srv.wait(t)
     n = len(stream)
     arrive = np.array([a for a, _, _ in stream])
@@ -142,6 +145,8 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
     inflight = {}  # ticket -> request index
     busy = set()
     energies = {}
+    failed = {}     # request -> error message: a failing request ends alone, the others go on
+    nopack = set()  # requests of a pack that failed: served one by one
     packbuf = {}  # key -> (dP, dF) device matrices of systems built in packs
     npacked = npacks = 0
 
@@ -180,7 +185,7 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
             groups = {}
             for j in pending:
                 _, key, func = stream[j]
-                if key in busy or host.get(key)["basis"].nao > pack_nao:
+                if j in nopack or key in busy or host.get(key)["basis"].nao > pack_nao:
                     continue
                 g = groups.setdefault(func, {})
                 g.setdefault(key, j)  # one request per system in a pack
@@ -188,7 +193,11 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
                 if len(g) < 2:
                     continue
                 group = list(g.values())
-                t_done = run_pack(group)
+                try:
+                    t_done = run_pack(group)
+                except xc.XCError:
+                    nopack.update(group)  # each request then succeeds or fails on its own
+                    continue
                 for j in group:
                     done_at[j] = t_done
                     pending.remove(j)
@@ -201,7 +210,14 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
             if key in busy:
                 continue
             bf = res.get(key, busy)
-            t = srv.submit(bf, func, precision, host.get(key)["P"])
+            try:
+                t = srv.submit(bf, func, precision, host.get(key)["P"])
+            except xc.XCError as e:
+                failed[j] = str(e)
+                done_at[j] = time.perf_counter() - t0
+                pending.remove(j)
+                nxt += 1
+                continue
             if t is None:
                 break
             inflight[t] = j
@@ -217,7 +233,8 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
                 energies[(key, func)] = out[1]
                 nxt += 1
     wall = time.perf_counter() - t0
-    lat = (done_at - arrive) * 1e3
+    ok = np.array([j not in failed for j in range(n)])
+    lat = ((done_at - arrive) * 1e3)[ok]
     st = srv.stats()
     srv._release()
     for dP, dF in packbuf.values():
@@ -231,7 +248,8 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
                 builds_per_s=round(n / wall, 2), wall_s=round(wall, 3), distinct_systems=len(keys),
                 resident_gb=round(res.bytes / 1e9, 2), created=res.created, evicted=res.evicted,
                 graph_launches=st["graph_launches"], host_setup_s=round(t_host, 1), device_setup_s=round(t_dev, 1),
-                pack_max_nao=pack_nao, packed_requests=npacked, packed_launches=npacks,
+                pack_max_nao=pack_nao, packed_requests=npacked, packed_launches=npacks, failed=len(failed),
+                _failed=failed,
                 _energies=energies, _host=host)
 
 
@@ -266,6 +284,7 @@ def main():
     stream = make_stream(args.requests, args.qps)
     out = run_stream(ctx, stream, nslots=args.slots, budget_gb=args.budget_gb, log=log, pack_nao=args.pack_nao)
     energies, host = out.pop("_energies"), out.pop("_host")
+    out.pop("_failed")
     out["max_dE_xc_fp32_vs_fp64"], out["fp64_sample"] = fp64_check(ctx, host, energies, args.fp64_sample)
     print(json.dumps(out), flush=True)
     ctx.close()

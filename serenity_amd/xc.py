@@ -707,6 +707,8 @@ class Server(_Handle):
         ctx._register(self)
 
     def submit(self, bfogc, functional, precision, P, block_ave_threshold=1e-11):
+        if functional not in FUNCTIONALS or precision not in PRECISIONS:
+            raise XCError(1, "Server.submit: unknown functional or precision %s / %s" % (functional, precision))
         n = bfogc.getNBasisFunctions()
         Pf = np.asfortranarray(P, dtype=np.float64)
         if Pf.shape != (n, n):

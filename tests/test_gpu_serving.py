@@ -139,3 +139,24 @@ def test_serving_with_packing_equals_unpacked(ctx):
     assert packed["packed_requests"] >= 2 and packed["packed_launches"] >= 1
     assert set(e0) == set(e1)
     assert max(abs(e0[k] - e1[k]) for k in e0) < 1e-10
+
+
+def test_serving_failed_request_is_isolated(ctx):
+    """A request that cannot be built (an unknown functional, rejected on the host before any
+    launch) fails alone: in a packed group it sends its group back to one-by-one builds, and every
+    other request of the burst gets the energy of the clean run."""
+    from serenity_amd import serving as SV
+    stream = SV.make_stream(24, 1.0e6)
+    host = SV.HostSystems(ctx=ctx)
+    small = [j for j, (_, key, _) in enumerate(stream) if host.get(key)["basis"].nao <= SV.PACK_NAO]
+    bad = small[0]
+    broken = list(stream)
+    broken[bad] = (stream[bad][0], stream[bad][1], "NOPE")
+    clean = SV.run_stream(ctx, stream, nslots=2, pack_nao=SV.PACK_NAO, host=host)
+    out = SV.run_stream(ctx, broken, nslots=2, pack_nao=SV.PACK_NAO, host=host)
+    say("failed requests: %d (%s)" % (out["failed"], list(out["_failed"].values())[:1]))
+    assert out["failed"] == 1 and bad in out["_failed"] and "unknown functional" in out["_failed"][bad]
+    e0, e1 = clean["_energies"], out["_energies"]
+    for k, e in e1.items():
+        assert abs(e - e0[k]) < 1e-10, k
+    assert len(e1) >= len(e0) - 1

@@ -176,6 +176,10 @@ def test_host_error_paths():
     """Errors come back as status codes with a message (SerenityError counterpart)."""
     with pytest.raises(xc.XCError, match="duplicated atoms"):
         xc.molecular_grid([8, 8], np.zeros((2, 3)))
+    # closer than 1e-5 bohr is a duplicate (Geometry.cpp:540-549), regardless of the element
+    with pytest.raises(xc.XCError, match="duplicated atoms"):
+        xc.molecular_grid([8, 1], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 5e-6]]), accuracy=1)
+    assert xc.molecular_grid([8, 1], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.8]]), accuracy=1).getNGridPoints() > 0
     with pytest.raises(xc.XCError, match="no entry for element"):
         xc.BasisController.from_file("DEF2-SVP", [11], np.zeros((1, 3)))
     with pytest.raises(xc.XCError, match="accuracy"):
