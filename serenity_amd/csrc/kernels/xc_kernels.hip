@@ -266,7 +266,9 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
 #pragma unroll
       for (int e = 0; e < KV; ++e) {
         const int mu = lidx[k0 + KV * qb + e];
-        rB[m][e] = (mu >= 0 && nu[m] >= 0) ? P[mu + (int64_t)n * nu[m]] : T(0);
+        // P is symmetric: read P[nu][mu] so that the 16 lanes of a group (consecutive columns
+        // nu, one mu) read consecutive addresses
+        rB[m][e] = (mu >= 0 && nu[m] >= 0) ? P[nu[m] + (int64_t)n * mu] : T(0);
       }
     }
   };
