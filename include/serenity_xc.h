@@ -296,6 +296,16 @@ int xc_last_energy(xc_ctx* ctx, double* energy);
 int xc_nadd_fock_build(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
                        double block_ave_threshold, const double* P_active, const double* const* P_env, double* F,
                        double* energy);
+/* NAddFuncPotential<UNRESTRICTED>::getMatrix / getEnergy (NAddFuncPotential.cpp:192-301
+ * with SCFMode = UNRESTRICTED): spin density matrices of the active (Pa_active,
+ * Pb_active) and environment subsystems in; F_a, F_b = the integrals of
+ * v_s[rho_sup] - v_s[rho_act] in the active basis and the energy
+ * E[sup] - E[act] - sum_env E[env] of the spin-polarised functional out (host
+ * buffers).  Same grid and controller rules as xc_nadd_fock_build. */
+int xc_nadd_fock_build_unrestricted(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
+                                    double block_ave_threshold, const double* Pa_active, const double* Pb_active,
+                                    const double* const* Pa_env, const double* const* Pb_env, double* Fa,
+                                    double* Fb, double* energy);
 /* DensityMatrixDensityOnGridController::calculateData (DensityMatrixDensityOnGrid
  * Controller.cpp:134-170), explicit form of its lazy cache: computes rho (and
  * grad rho if with_gradient) of the device density matrix dP and keeps them

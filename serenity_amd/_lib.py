@@ -102,6 +102,8 @@ def lib():
     _sig("xc_last_energy", C.c_int, vp, dp)
     _sig("xc_nadd_fock_build", C.c_int, vp, C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.c_double, dp,
          C.POINTER(dp), dp, dp)
+    _sig("xc_nadd_fock_build_unrestricted", C.c_int, vp, C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.c_double, dp,
+         dp, C.POINTER(dp), C.POINTER(dp), dp, dp, dp)
     _sig("xc_server_create", C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp))
     _sig("xc_server_destroy", C.c_int, vp)
     _sig("xc_server_submit", C.c_int, vp, vp, C.c_int, C.c_int, C.c_double, dp, dp, dp, C.POINTER(C.c_int64))

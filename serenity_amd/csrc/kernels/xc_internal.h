@@ -134,7 +134,9 @@ void launch_functional_packed(hipStream_t s, int func, const XcParams& prm, int 
 // block energies out.
 void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* ra,
                            const double* ga, const double* rb, const double* gb, const double* w, double* vsa,
-                           double* gwa, double* vsb, double* gwb, double* e_blk);
+                           double* gwa, double* vsb, double* gwb, double* e_blk,
+                           const double* ra2 = nullptr, const double* ga2 = nullptr, const double* rb2 = nullptr,
+                           const double* gb2 = nullptr);
 
 // Sub-tile Fock kernel: 8 wavefronts, slot class sc (kFockS[sc] sub-tiles per wavefront).
 template <typename T>

@@ -571,7 +571,11 @@ __global__ __launch_bounds__(128) void functional_pol_kernel(int func, XcParams 
                                                              const double* __restrict__ gb,
                                                              const double* __restrict__ w, double* __restrict__ vsa,
                                                              double* __restrict__ gwa, double* __restrict__ vsb,
-                                                             double* __restrict__ gwb, double* __restrict__ e_blk) {
+                                                             double* __restrict__ gwb, double* __restrict__ e_blk,
+                                                             const double* __restrict__ ra2,
+                                                             const double* __restrict__ ga2,
+                                                             const double* __restrict__ rb2,
+                                                             const double* __restrict__ gb2) {
   __shared__ double red[128];
   const int b = blockIdx.x, t = threadIdx.x;
   const int64_t first = (int64_t)b * bs;
@@ -590,8 +594,28 @@ __global__ __launch_bounds__(128) void functional_pol_kernel(int func, XcParams 
   const bool skip_b = block_sum_128(fabs(bb), red) < B * 1e-12;
   double e = 0.0, v[2] = {0.0, 0.0}, g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (in && !(skip_a && skip_b)) e = xc_point_pol(func, a, bb, xa, xb, prm, v, g);
+  if (ra2) {
+    // non-additive form: the same functional of a second pair of spin densities is subtracted
+    // (its own block skip applies to it)
+    const double a2 = in ? ra2[q] : 0.0, b2 = in ? rb2[q] : 0.0;
+    double ya[3] = {0.0, 0.0, 0.0}, yb[3] = {0.0, 0.0, 0.0};
+    if (in && gga)
+      for (int c = 0; c < 3; ++c) {
+        ya[c] = ga2[c * npts + q];
+        yb[c] = gb2[c * npts + q];
+      }
+    const bool skip2_a = block_sum_128(fabs(a2), red) < B * 1e-12;
+    const bool skip2_b = block_sum_128(fabs(b2), red) < B * 1e-12;
+    if (in && !(skip2_a && skip2_b)) {
+      double v2[2] = {0.0, 0.0}, g2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      e -= xc_point_pol(func, a2, b2, ya, yb, prm, v2, g2);
+      v[0] -= v2[0];
+      v[1] -= v2[1];
+      for (int c = 0; c < 6; ++c) g[c] -= g2[c];
+    }
+  }
   const double wq = in ? w[q] : 0.0;
-  if (in) {
+  if (in && vsa) {
     vsa[q] = wq * v[0];
     vsb[q] = wq * v[1];
     if (gga)
@@ -604,7 +628,7 @@ __global__ __launch_bounds__(128) void functional_pol_kernel(int func, XcParams 
   if (t == 0) e_blk[b] = eb;
 }
 
-// Sub-tile Fock contraction.  One workgroup of 8 wavefronts per work item; the
+// Sub-tile Fock contraction.  One workgroup of kFockWaves wavefronts per work item; the
 // item's 16 x 16 AO sub-tiles (i, j) are dealt to the wavefronts round-robin,
 // S per wavefront, each accumulating S_ij = phi_i^T G_j + G_i^T phi_j over the
 // block's 128 points in registers (dummy slots of a short item compute on row
@@ -615,7 +639,7 @@ __global__ __launch_bounds__(128) void functional_pol_kernel(int func, XcParams 
 // on its way in.  Inside a chunk, k-step t of lane group g takes point
 // KV g + t (KV = KC / 4), so each MFMA operand of a chunk is one 32-byte LDS
 // read of KV consecutive points of one AO row.  MFMAs are issued in groups of
-// two independent accumulators.
+// GS independent accumulators.
 template <typename T, int S>
 __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockItem* __restrict__ work,
                                                        const DevBlock* __restrict__ blocks,
@@ -1071,11 +1095,12 @@ void launch_pack_sum(hipStream_t s, const PackSys* pack, int nsys, int fp32) {
 
 void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t npts, int bs, const double* ra,
                            const double* ga, const double* rb, const double* gb, const double* w, double* vsa,
-                           double* gwa, double* vsb, double* gwb, double* e_blk) {
+                           double* gwa, double* vsb, double* gwb, double* e_blk, const double* ra2,
+                           const double* ga2, const double* rb2, const double* gb2) {
   const int nb = (int)((npts + bs - 1) / bs);
   if (nb <= 0) return;
   hipLaunchKernelGGL(functional_pol_kernel, dim3(nb), dim3(128), 0, s, func, prm, npts, bs, ra, ga, rb, gb, w, vsa,
-                     gwa, vsb, gwb, e_blk);
+                     gwa, vsb, gwb, e_blk, ra2, ga2, rb2, gb2);
 }
 
 template <typename T, int S>

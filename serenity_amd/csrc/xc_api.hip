@@ -276,7 +276,7 @@ struct xc_bfogc {
   DBuf<float> d_P32;
   DBuf<double> d_round;  // finalize-block partials of the FP32 rounding term, per spin
   // supersystem density of a non-additive build with this controller active (allocated on first use)
-  DBuf<double> d_rho_sup, d_grad_sup;
+  DBuf<double> d_rho_sup, d_grad_sup, d_rho_sup_b, d_grad_sup_b;
   // cached grid density (xc_bfogc_update_density) and the energies of it per functional
   DBuf<double> d_rho_c, d_grad_c, d_ecache;
   bool rho_c_valid = false, rho_c_grad = false;
@@ -1280,6 +1280,144 @@ static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int func
   launch_finalize(s, act->d_U.p, dF, n, 0);
   ctx->mark();
   HIPX(hipGetLastError());
+}
+
+/* Unrestricted non-additive build (NAddFuncPotential<UNRESTRICTED>::getMatrix): spin
+ * densities of every subsystem on the shared grid, rho_sup^s = sum of them,
+ * v^s = v^s[rho_sup] - v^s[rho_act] integrated with the active basis per spin,
+ * E = E[sup] - E[act] - sum_env E[env] (left in ctx->d_energy). */
+static void enqueue_nadd_pol(xc_bfogc* act, int nenv, xc_bfogc* const* env, int functional, int precision,
+                             double thr, const double* dPa_act, const double* dPb_act, const double* const* dPa_env,
+                             const double* const* dPb_env, double* dFa, double* dFb) {
+  xc_ctx* ctx = act->ctx;
+  require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
+  require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+  require(nenv >= 0 && (nenv == 0 || (env && dPa_env && dPb_env)), XC_ERR_ARG,
+          "non-additive build: null environment list");
+  const bool gga = functional_is_gga(functional);
+  const bool fp32 = precision == XC_PREC_FP32;
+  std::vector<xc_bfogc*> sub(1, act);
+  std::vector<const double*> Pa(1, dPa_act), Pb(1, dPb_act);
+  for (int i = 0; i < nenv; ++i) {
+    require(env[i] && dPa_env[i] && dPb_env[i], XC_ERR_ARG,
+            "non-additive build: null environment controller or density");
+    require(env[i] != act, XC_ERR_ARG, "non-additive build: the active controller is also listed as environment");
+    for (int j = 0; j < i; ++j)
+      require(env[j] != env[i], XC_ERR_ARG, "non-additive build: an environment controller is listed twice");
+    sub.push_back(env[i]);
+    Pa.push_back(dPa_env[i]);
+    Pb.push_back(dPb_env[i]);
+  }
+  for (xc_bfogc* b : sub) {
+    require(b->ctx == ctx, XC_ERR_ARG, "non-additive build: controllers live on different contexts");
+    require(b->grid == act->grid, XC_ERR_ARG,
+            "non-additive build: all subsystem controllers must be defined on the same (supersystem) grid");
+    require(b->bs == act->bs, XC_ERR_ARG, "non-additive build: controllers differ in block size");
+    require(!b->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
+    require(!gga || b->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
+    require(!fp32 || b->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+    require(fp32 || b->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
+    ensure_pol_buffers(b, fp32);
+  }
+  const int n = act->basis->nao;
+  const int64_t N = act->grid->npts;
+  const int nxc = (int)((N + kTile - 1) / kTile);
+  if (!act->d_rho_sup.p) {
+    act->d_rho_sup.alloc(ctx, N);
+    act->d_grad_sup.alloc(ctx, 3 * N);
+  }
+  if (!act->d_rho_sup_b.p) {
+    act->d_rho_sup_b.alloc(ctx, N);
+    act->d_grad_sup_b.alloc(ctx, 3 * N);
+  }
+  hipStream_t s = ctx->stream;
+  const double* w = act->grid->d_w.p;
+  const int g = gga ? 1 : 0;
+  double* sup_r[2] = {act->d_rho_sup.p, act->d_rho_sup_b.p};
+  double* sup_g[2] = {act->d_grad_sup.p, act->d_grad_sup_b.p};
+  for (size_t i = 0; i < sub.size(); ++i) {
+    xc_bfogc* b = sub[i];
+    const double* P[2] = {Pa[i], Pb[i]};
+    float* P32[2] = {b->d_P32.p, b->d_P32b.p};
+    double* rho[2] = {b->d_rho.p, b->d_rho_b.p};
+    double* grd[2] = {b->d_grad.p, b->d_grad_b.p};
+    for (int sp = 0; sp < 2; ++sp) {
+      if (fp32) {
+        launch_to_float(s, P[sp], P32[sp], (int64_t)b->basis->nao * b->basis->nao);
+        run_density<float>(b, b->d_t32.p, P32[sp], g, ctx->lanes);
+      } else {
+        run_density<double>(b, b->d_t64.p, P[sp], g, ctx->lanes);
+      }
+      launch_reduce_partials(s, b->d_blocks.p, b->bs, N, b->d_part.p, g, rho[sp], grd[sp], 0);
+      launch_reduce_partials(s, b->d_blocks.p, b->bs, N, b->d_part.p, g, sup_r[sp], sup_g[sp], i == 0 ? 0 : 1);
+    }
+  }
+  // v^s[sup] - v^s[act] and E[sup] - E[act]
+  launch_functional_pol(s, functional, ctx->prm, N, kTile, sup_r[0], sup_g[0], sup_r[1], sup_g[1], w, act->d_vs.p,
+                        act->d_gw.p, act->d_vs_b.p, act->d_gw_b.p, act->d_eblk.p, act->d_rho.p, act->d_grad.p,
+                        act->d_rho_b.p, act->d_grad_b.p);
+  launch_sum(s, act->d_eblk.p, nxc, ctx->d_energy, 0);
+  for (size_t i = 1; i < sub.size(); ++i) {
+    xc_bfogc* b = sub[i];
+    launch_functional_pol(s, functional, ctx->prm, N, kTile, b->d_rho.p, b->d_grad.p, b->d_rho_b.p, b->d_grad_b.p, w,
+                          nullptr, nullptr, nullptr, nullptr, b->d_eblk.p);
+    launch_sum(s, b->d_eblk.p, nxc, ctx->d_energy, 1);
+  }
+  const double* vs[2] = {act->d_vs.p, act->d_vs_b.p};
+  const double* gw[2] = {act->d_gw.p, act->d_gw_b.p};
+  double* Fs[2] = {dFa, dFb};
+  for (int sp = 0; sp < 2; ++sp) {
+    launch_fill(s, act->d_U.p, (int64_t)n * n, 0.0);
+    if (fp32)
+      run_fock<float>(act, act->d_t32.p, g, thr, ctx->lanes, vs[sp], gw[sp]);
+    else
+      run_fock<double>(act, act->d_t64.p, g, thr, ctx->lanes, vs[sp], gw[sp]);
+    launch_finalize(s, act->d_U.p, Fs[sp], n, 0);
+  }
+  HIPX(hipGetLastError());
+}
+
+int xc_nadd_fock_build_unrestricted(xc_bfogc* active, int nenv, xc_bfogc* const* env, int functional, int precision,
+                                    double thr, const double* Pa_active, const double* Pb_active,
+                                    const double* const* Pa_env, const double* const* Pb_env, double* Fa, double* Fb,
+                                    double* energy) {
+  return guarded([&] {
+    require(active && Pa_active && Pb_active && Fa && Fb, XC_ERR_ARG, "xc_nadd_fock_build_unrestricted: null argument");
+    require(nenv >= 0 && (nenv == 0 || (env && Pa_env && Pb_env)), XC_ERR_ARG,
+            "xc_nadd_fock_build_unrestricted: null environment list");
+    xc_ctx* ctx = active->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    std::vector<xc_bfogc*> sub(1, active);
+    for (int i = 0; i < nenv; ++i) {
+      require(env[i] && Pa_env[i] && Pb_env[i], XC_ERR_ARG,
+              "xc_nadd_fock_build_unrestricted: null environment controller or density");
+      require(env[i]->ctx == ctx, XC_ERR_ARG, "xc_nadd_fock_build_unrestricted: controllers live on different contexts");
+      sub.push_back(env[i]);
+    }
+    // host spin densities into each controller's P (alpha) and Pb (beta) buffers
+    std::vector<const double*> dPa(nenv), dPb(nenv);
+    for (size_t i = 0; i < sub.size(); ++i) {
+      xc_bfogc* b = sub[i];
+      ensure_pol_buffers(b, precision == XC_PREC_FP32);
+      const int m = b->basis->nao;
+      const double* ha = i == 0 ? Pa_active : Pa_env[i - 1];
+      const double* hb = i == 0 ? Pb_active : Pb_env[i - 1];
+      HIPX(hipMemcpyAsync(b->d_P.p, ha, sizeof(double) * m * m, hipMemcpyHostToDevice, ctx->stream));
+      HIPX(hipMemcpyAsync(b->d_Pb.p, hb, sizeof(double) * m * m, hipMemcpyHostToDevice, ctx->stream));
+      if (i > 0) {
+        dPa[i - 1] = b->d_P.p;
+        dPb[i - 1] = b->d_Pb.p;
+      }
+    }
+    enqueue_nadd_pol(active, nenv, env, functional, precision, thr, active->d_P.p, active->d_Pb.p, dPa.data(),
+                     dPb.data(), active->d_F.p, active->d_Fb.p);
+    const int n = active->basis->nao;
+    HIPX(hipMemcpyAsync(Fa, active->d_F.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipMemcpyAsync(Fb, active->d_Fb.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
+    if (energy) *energy = ctx->last_energy;
+  });
 }
 
 int xc_bfogc_update_density(xc_bfogc* bf, int precision, const double* dP, int with_gradient) {

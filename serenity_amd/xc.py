@@ -656,6 +656,34 @@ class NAddFuncPotential:
         self._energy = E.value
         return F
 
+    def getMatrixUnrestricted(self, Pa_active, Pb_active, Pa_environment, Pb_environment):
+        """NAddFuncPotential<UNRESTRICTED>::getMatrix: spin density matrices of the active and
+        environment subsystems in, (F_alpha, F_beta) of the active subsystem out
+        (xc_nadd_fock_build_unrestricted); getEnergy() then returns the non-additive energy."""
+        n = self.active.getNBasisFunctions()
+        if len(Pa_environment) != len(self.environment) or len(Pb_environment) != len(self.environment):
+            raise XCError(1, "NAddFuncPotential: environment spin densities do not match the environment controllers")
+
+        def mat(P, m, what):
+            Pf = np.asfortranarray(P, dtype=np.float64)
+            if Pf.shape != (m, m):
+                raise XCError(1, "NAddFuncPotential: %s density matrix is %s, basis has %d functions"
+                              % (what, Pf.shape, m))
+            return Pf
+        Pa, Pb = mat(Pa_active, n, "active alpha"), mat(Pb_active, n, "active beta")
+        Pea = [mat(P, c.getNBasisFunctions(), "environment alpha") for c, P in zip(self.environment, Pa_environment)]
+        Peb = [mat(P, c.getNBasisFunctions(), "environment beta") for c, P in zip(self.environment, Pb_environment)]
+        pa = (_lib.dp * max(1, len(Pea)))(*[p.ctypes.data_as(_lib.dp) for p in Pea])
+        pb = (_lib.dp * max(1, len(Peb)))(*[p.ctypes.data_as(_lib.dp) for p in Peb])
+        Fa, Fb = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+        E = C.c_double()
+        check(_lib.lib().xc_nadd_fock_build_unrestricted(
+            self.active.handle, len(Pea), self._env_handles(), FUNCTIONALS[self.functional],
+            PRECISIONS[self.precision], float(self.blockAveThreshold), Pa.ctypes.data_as(_lib.dp),
+            Pb.ctypes.data_as(_lib.dp), pa, pb, Fa.ctypes.data_as(_lib.dp), Fb.ctypes.data_as(_lib.dp), C.byref(E)))
+        self._energy = E.value
+        return Fa, Fb
+
     def getEnergy(self):
         if self._energy is None:
             raise XCError(4, "NAddFuncPotential: getMatrix has not been called")

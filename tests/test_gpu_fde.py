@@ -177,3 +177,73 @@ def test_nadd_cached_densities_match_direct_build(ctx):
         assert abs(E2 - Eo) <= TOL_E and np.abs(F2.reshape(n0, n0, order="F") - Fo).max() <= TOL_F
     for d in dP + [dF1, dF2]:
         ctx.free(d)
+
+
+def _oracle_nadd_pol(bases, Pas, Pbs, grid, functional):
+    """The unrestricted non-additive build composed from oracle pieces: subsystem spin densities,
+    their sums, the spin-polarised functional of the supersystem and of the active densities, and
+    the per-spin Fock assembly of the difference potential in the active basis."""
+    pts, w = grid.points, grid.weights
+    gga = functional in O.GGA
+    dens = []
+    for b, Pa, Pb in zip(bases, Pas, Pbs):
+        da, db = O.density(b, pts, Pa), O.density(b, pts, Pb)
+        dens.append((da["rho"], db["rho"], da["grad"], db["grad"]))
+    sup = [dens[0][k].copy() for k in range(4)]
+    for d in dens[1:]:
+        for k in range(4):
+            sup[k] = sup[k] + d[k]
+    _, va_s, vb_s, ga_s, gb_s, E_s = O.functional_pol(functional, sup[0], sup[1], sup[2], sup[3], w)
+    _, va_a, vb_a, ga_a, gb_a, E_a = O.functional_pol(functional, *dens[0], w)
+    E = E_s - E_a - sum(O.functional_pol(functional, *d, w)[5] for d in dens[1:])
+    Fa = O.fock_add(bases[0], pts, w, va_s - va_a, (ga_s - ga_a) if gga else None)
+    Fb = O.fock_add(bases[0], pts, w, vb_s - vb_a, (gb_s - gb_a) if gga else None)
+    return Fa, Fb, E
+
+
+def test_nadd_unrestricted_h2_lda_known_answers(ctx):
+    """NAddFuncPotential_test.cpp:87-116 (LDA, UNRESTRICTED, 1e-5): the closed-shell H2 dimer with
+    P_a = P_b = P / 2 gives F_a = F_b = the restricted matrix and the same energy."""
+    s = common.h2_fde_system()
+    (ba, Pa), (be, Pe), g = s["act"], s["env"], s["grid"]
+    bfa = xc.BasisFunctionOnGridController(ctx, ba, g, 128, 1e-9, 1)
+    bfe = xc.BasisFunctionOnGridController(ctx, be, g, 128, 1e-9, 1)
+    pot = xc.NAddFuncPotential(bfa, [bfe], "LDA")
+    F = pot.getMatrix(Pa, [Pe])
+    E = pot.getEnergy()
+    Fa, Fb = pot.getMatrixUnrestricted(Pa / 2, Pa / 2, [Pe / 2], [Pe / 2])
+    kat = common.load_json("kat_nadd_h2.json")
+    for e in kat["entries"]:
+        assert abs(Fa[e["row"], e["col"]] - e["expected"]) < kat["tolerance"], e
+        assert abs(Fb[e["row"], e["col"]] - e["expected"]) < kat["tolerance"], e
+    say("NAdd H2 dimer LDA unrestricted: max|F_a - F| %.2e |dE| %.2e" % (np.abs(Fa - F).max(),
+                                                                       abs(pot.getEnergy() - E)))
+    assert np.abs(Fa - F).max() < 1e-12 and np.abs(Fb - F).max() < 1e-12
+    assert abs(pot.getEnergy() - E) < 1e-12
+
+
+@pytest.mark.parametrize("functional", ["PBE", "LDA", "TF"])
+def test_nadd_unrestricted_open_shell_matches_oracle(ctx, functional):
+    """Benzene active, 6 waters in the environment, synthetic open-shell spin densities per subsystem:
+    F_a, F_b and E against the oracle composition (FP64, 1e-9); FP32 contractions within 1e-5."""
+    emb = embedding(6, 3)
+    g, subs = emb["grid"], emb["subsystems"]
+    bfs = controllers(ctx, emb, keep_fp32=True)
+    Pas, Pbs = [], []
+    for i, s in enumerate(subs):
+        S = s["basis"].overlap()
+        nel = int(round(np.vdot(s["P"], S)))
+        Pas.append(W.synthetic_density(S, nel // 2 + 1, seed=40 + i) / 2)
+        Pbs.append(W.synthetic_density(S, nel // 2 - 1, seed=80 + i) / 2)
+    Fo_a, Fo_b, Eo = _oracle_nadd_pol([s["basis"] for s in subs], Pas, Pbs, g, functional)
+    pot = xc.NAddFuncPotential(bfs[0], bfs[1:], functional, "FP64")
+    Fa, Fb = pot.getMatrixUnrestricted(Pas[0], Pbs[0], Pas[1:], Pbs[1:])
+    dF = max(np.abs(Fa - Fo_a).max(), np.abs(Fb - Fo_b).max())
+    say("NAdd unrestricted %s: |dE| %.2e max|dF| %.2e (E %.8f)" % (functional, abs(pot.getEnergy() - Eo), dF, Eo))
+    assert dF < 1e-9 and abs(pot.getEnergy() - Eo) < 1e-9
+    assert np.abs(Fa - Fb).max() > 1e-6  # the spins differ
+    p32 = xc.NAddFuncPotential(bfs[0], bfs[1:], functional, "FP32")
+    Ga, Gb = p32.getMatrixUnrestricted(Pas[0], Pbs[0], Pas[1:], Pbs[1:])
+    assert max(np.abs(Ga - Fa).max(), np.abs(Gb - Fb).max()) < 1e-5
+    for b in bfs:
+        b._release()
