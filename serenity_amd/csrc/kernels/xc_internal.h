@@ -138,7 +138,7 @@ void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t
                            const double* ra2 = nullptr, const double* ga2 = nullptr, const double* rb2 = nullptr,
                            const double* gb2 = nullptr);
 
-// Sub-tile Fock kernel: 8 wavefronts, slot class sc (kFockS[sc] sub-tiles per wavefront).
+// Sub-tile Fock kernel: kFockWaves wavefronts, slot class sc (kFockS[sc] sub-tiles per wavefront).
 template <typename T>
 void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,
                      const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
