@@ -133,6 +133,9 @@ struct PackWork;
 struct xc_ctx {
   int device = 0;
   PackWork* pack = nullptr;  // device lists of packed builds (allocated on first use)
+  // packed-build compositions seen before (controllers, functional, precision, threshold):
+  // their own lists and, from the second build on, a captured graph
+  std::map<std::vector<uint64_t>, PackWork*> pack_graphs;
   hipStream_t stream = nullptr;
   Lanes lanes;  // helper lanes of builds on the context stream
   std::recursive_mutex mu;
@@ -211,7 +214,25 @@ struct PackWork {
   DBuf<WorkItem> dens[kDensMaxS + 1];
   DBuf<FockItem> fock[kFockNumS];
   DBuf<double> round, E;
+  hipGraphExec_t exec = nullptr;  // captured build of a repeated composition
+  ~PackWork() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
 };
+
+// drop every cached packed composition that names bf (its buffers are going away)
+void forget_packs_of(xc_ctx* ctx, const void* bf) {
+  for (auto it = ctx->pack_graphs.begin(); it != ctx->pack_graphs.end();) {
+    bool uses = false;
+    for (size_t i = 0; i + 3 < it->first.size(); ++i) uses = uses || it->first[i] == (uint64_t)(uintptr_t)bf;
+    if (uses) {
+      delete it->second;
+      it = ctx->pack_graphs.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
 
 template <typename T>
 void upload_grow(xc_ctx* ctx, DBuf<T>& b, const std::vector<T>& v) {
@@ -401,6 +422,8 @@ int xc_ctx_destroy(xc_ctx* ctx) {
     ctx->release(ctx->d_energy, sizeof(double));
     delete ctx->pack;
     ctx->pack = nullptr;
+    for (auto& kv : ctx->pack_graphs) delete kv.second;
+    ctx->pack_graphs.clear();
     ctx->lanes.destroy();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -914,6 +937,7 @@ int xc_bfogc_destroy(xc_bfogc* bfogc) {
     if (!bfogc) return;
     require(!bfogc->in_flight, XC_ERR_STATE, "xc_bfogc_destroy: a server build of this controller is in flight");
     (void)hipStreamSynchronize(bfogc->ctx->stream);
+    forget_packs_of(bfogc->ctx, bfogc);
     --bfogc->ctx->live_handles;
     delete bfogc;
   });
@@ -1661,45 +1685,95 @@ int xc_fock_build_packed(int nsys, xc_bfogc* const* bfs, int functional, int pre
       ps.nxc = (int)((bf->grid->npts + kTile - 1) / kTile);
       maxnn = std::max(maxnn, (int64_t)n * n);
     }
-    if (!ctx->pack) ctx->pack = new PackWork();
-    PackWork& pw = *ctx->pack;
+    // a composition seen once before runs as a captured graph; the first build is eager
+    std::vector<uint64_t> key;
+    for (int i = 0; i < nsys; ++i) key.push_back((uint64_t)(uintptr_t)bfs[i]);
+    uint64_t thr_bits = 0;
+    std::memcpy(&thr_bits, &thr, sizeof(thr_bits));
+    key.push_back((uint64_t)functional);
+    key.push_back((uint64_t)precision);
+    key.push_back(thr_bits);
+    auto hit = ctx->pack_graphs.find(key);
+    bool capture = false;
+    PackWork* pwp = nullptr;
+    if (hit == ctx->pack_graphs.end()) {
+      if (ctx->pack_graphs.size() >= 16) {  // bounded cache: forget the oldest-keyed composition
+        delete ctx->pack_graphs.begin()->second;
+        ctx->pack_graphs.erase(ctx->pack_graphs.begin());
+      }
+      ctx->pack_graphs.emplace(key, new PackWork());
+      if (!ctx->pack) ctx->pack = new PackWork();
+      pwp = ctx->pack;
+    } else {
+      pwp = hit->second;
+      capture = pwp->exec == nullptr;
+    }
+    PackWork& pw = *pwp;
     if (pw.round.n < (size_t)nsys * kPackRoundBlocks) pw.round.alloc(ctx, (size_t)2 * nsys * kPackRoundBlocks);
     if (pw.E.n < (size_t)nsys) pw.E.alloc(ctx, (size_t)2 * nsys);
     for (int i = 0; i < nsys; ++i) {
       sys[i].round = pw.round.p + (size_t)i * kPackRoundBlocks;
       sys[i].E = pw.E.p + i;
     }
-    upload_grow(ctx, pw.blocks, blocks);
-    upload_grow(ctx, pw.sys, sys);
-    for (int c = 1; c <= kDensMaxS; ++c) upload_grow(ctx, pw.dens[c], dens[c]);
-    for (int c = 0; c < kFockNumS; ++c) upload_grow(ctx, pw.fock[c], fock[c]);
     hipStream_t s = ctx->stream;
     const Lanes& L = ctx->lanes;
-    const PackSys* dsys = pw.sys.p;
     const int g = gga ? 1 : 0;
-    // one launch per stage and work class for every system of the pack
-    if (fp32) launch_pack_to_float(s, dsys, nsys, maxnn);
-    over_lanes(L, kDensMaxS + 1, [&](int c) { return c >= 1 ? (int)dens[c].size() : 0; },
-               [&](int c, hipStream_t st) {
-      if (fp32)
-        launch_density_sub<float>(st, c, pw.dens[c].p, (int)dens[c].size(), pw.blocks.p, nullptr, nullptr, nullptr,
-                                  0, g, nullptr, 0, dsys);
-      else
-        launch_density_sub<double>(st, c, pw.dens[c].p, (int)dens[c].size(), pw.blocks.p, nullptr, nullptr, nullptr,
-                                   0, g, nullptr, 0, dsys);
-    });
-    launch_functional_packed(s, functional, ctx->prm, (int)blocks.size(), pw.blocks.p, dsys);
-    launch_pack_clear(s, dsys, nsys, maxnn);
-    over_lanes(L, kFockNumS, [&](int c) { return (int)fock[c].size(); }, [&](int c, hipStream_t st) {
-      if (fp32)
-        launch_fock_sub<float>(st, c, pw.fock[c].p, (int)fock[c].size(), pw.blocks.p, nullptr, nullptr, nullptr,
-                               nullptr, g, thr, nullptr, 0, 0, dsys);
-      else
-        launch_fock_sub<double>(st, c, pw.fock[c].p, (int)fock[c].size(), pw.blocks.p, nullptr, nullptr, nullptr,
-                                nullptr, g, thr, nullptr, 0, 0, dsys);
-    });
-    launch_pack_finalize(s, dsys, nsys, fp32 ? 1 : 0);
-    launch_pack_sum(s, dsys, nsys, fp32 ? 1 : 0);
+    // the per-system table (matrices of this call) is uploaded every time; a replayed graph reads it
+    upload_grow(ctx, pw.sys, sys);
+    if (pw.exec) {
+      HIPX(hipGraphLaunch(pw.exec, s));
+    } else {
+      upload_grow(ctx, pw.blocks, blocks);
+      for (int c = 1; c <= kDensMaxS; ++c) upload_grow(ctx, pw.dens[c], dens[c]);
+      for (int c = 0; c < kFockNumS; ++c) upload_grow(ctx, pw.fock[c], fock[c]);
+      const PackSys* dsys = pw.sys.p;
+      // one launch per stage and work class for every system of the pack
+      auto enqueue = [&]() {
+        if (fp32) launch_pack_to_float(s, dsys, nsys, maxnn);
+        over_lanes(L, kDensMaxS + 1, [&](int c) { return c >= 1 ? (int)dens[c].size() : 0; },
+                   [&](int c, hipStream_t st) {
+          if (fp32)
+            launch_density_sub<float>(st, c, pw.dens[c].p, (int)dens[c].size(), pw.blocks.p, nullptr, nullptr,
+                                      nullptr, 0, g, nullptr, 0, dsys);
+          else
+            launch_density_sub<double>(st, c, pw.dens[c].p, (int)dens[c].size(), pw.blocks.p, nullptr, nullptr,
+                                       nullptr, 0, g, nullptr, 0, dsys);
+        });
+        launch_functional_packed(s, functional, ctx->prm, (int)blocks.size(), pw.blocks.p, dsys);
+        launch_pack_clear(s, dsys, nsys, maxnn);
+        over_lanes(L, kFockNumS, [&](int c) { return (int)fock[c].size(); }, [&](int c, hipStream_t st) {
+          if (fp32)
+            launch_fock_sub<float>(st, c, pw.fock[c].p, (int)fock[c].size(), pw.blocks.p, nullptr, nullptr,
+                                   nullptr, nullptr, g, thr, nullptr, 0, 0, dsys);
+          else
+            launch_fock_sub<double>(st, c, pw.fock[c].p, (int)fock[c].size(), pw.blocks.p, nullptr, nullptr,
+                                    nullptr, nullptr, g, thr, nullptr, 0, 0, dsys);
+        });
+        launch_pack_finalize(s, dsys, nsys, fp32 ? 1 : 0);
+        launch_pack_sum(s, dsys, nsys, fp32 ? 1 : 0);
+      };
+      if (capture) {
+        // second build of this composition: capture it (kernel nodes only, no memsets) into its own lists
+        hipGraph_t graph = nullptr;
+        HIPX(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue();
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+          throw;
+        }
+        HIPX(hipStreamEndCapture(s, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPX(e);
+        pw.exec = exec;
+        HIPX(hipGraphLaunch(pw.exec, s));
+      } else {
+        enqueue();
+      }
+    }
     HIPX(hipGetLastError());
     HIPX(hipMemcpyAsync(energies, pw.E.p, sizeof(double) * nsys, hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));

@@ -93,3 +93,36 @@ def test_packed_single_system_and_errors(ctx):
         xc.fock_build_packed([pot, xc.FuncPotential(systems[1]["bf"], "LDA", "FP64")],
                              [s["dP"], systems[1]["dP"]], [s["dF"], systems[1]["dF"]])
     _release(ctx, systems)
+
+
+def test_packed_repeated_composition_replays_graph(ctx):
+    """A composition built again is captured once into a HIP graph and then replayed: every replay,
+    with other builds and new density matrices in between, equals the builds alone."""
+    systems = _systems(ctx)[:4]
+    pots = [xc.FuncPotential(s["bf"], "PBE", "FP32") for s in systems]
+    other = _systems(ctx)[4]
+    for rep in range(4):
+        alone = []
+        for i, s in enumerate(systems):
+            Pr = W.synthetic_density(s["basis"].overlap(), max(1, int(round(np.vdot(s["P"], s["basis"].overlap()))) // 2),
+                                     seed=100 * rep + i)
+            ctx.upload(s["dP"], np.asfortranarray(Pr))
+            p = xc.FuncPotential(s["bf"], "PBE", "FP32")
+            p.build_device(s["dP"], s["dF"])
+            F = np.zeros((s["n"], s["n"]), order="F")
+            ctx.download(F, s["dF"])
+            alone.append((F, p.getEnergy()))
+        xc.FuncPotential(other["bf"], "LDA", "FP64").build_device(other["dP"], other["dF"])  # eager work between
+        E = xc.fock_build_packed(pots, [s["dP"] for s in systems], [s["dF"] for s in systems])
+        worst = 0.0
+        for s, (Fa, Ea), e in zip(systems, alone, E):
+            F = np.zeros((s["n"], s["n"]), order="F")
+            ctx.download(F, s["dF"])
+            worst = max(worst, np.abs(F - Fa).max(), abs(e - Ea))
+        say("packed build %d of one composition (%s): max diff %.2e" % (rep, "eager" if rep == 0 else
+                                                                         "captured" if rep == 1 else "replay", worst))
+        assert worst < 1e-12
+    _release(ctx, systems)
+    ctx.free(other["dP"])
+    ctx.free(other["dF"])
+    other["bf"]._release()
