@@ -105,7 +105,7 @@ class ResidentSystems:
 
 
 def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="FP32", log=None, pack_nao=0,
-               host=None):
+               host=None, use_graphs=True):
     """Serve the stream in real time; returns latency percentiles, throughput and counts.
 
     pack_nao > 0: pending requests of idle systems with at most that many basis functions are
@@ -127,7 +127,7 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
         ctx.synchronize()
     t_dev = time.perf_counter() - t0
     say("resident: %d systems, %.1f GB of device memory, setup %.1f s" % (len(res.lru), res.bytes / 1e9, t_dev))
-    srv = xc.Server(ctx, nslots=nslots, use_graphs=True)
+    srv = xc.Server(ctx, nslots=nslots, use_graphs=use_graphs)
     # capture every resident system's build once (first-call graph capture is setup, not serving)
     if prewarm:
         for k in keys:
