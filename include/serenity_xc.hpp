@@ -363,6 +363,32 @@ class NAddFuncPotential {
     _built = true;
     return F;
   }
+  /* NAddFuncPotential<UNRESTRICTED>::getMatrix: alpha and beta density matrices of the active
+   * and environment subsystems in, the alpha and beta non-additive potentials out. */
+  std::pair<Matrix, Matrix> getMatrix(const Matrix& PactA, const Matrix& PactB, const std::vector<Matrix>& PenvA,
+                                      const std::vector<Matrix>& PenvB) {
+    const int n = (int)_act->getNBasisFunctions();
+    if (PactA.rows != n || PactA.cols != n || PactB.rows != n || PactB.cols != n)
+      throw SerenityError(XC_ERR_ARG, "NAddFuncPotential: active density matrices and basis differ in size");
+    if (PenvA.size() != _env.size() || PenvB.size() != _env.size())
+      throw SerenityError(XC_ERR_ARG, "NAddFuncPotential: two density matrices per environment subsystem are needed");
+    std::vector<xc_bfogc*> env;
+    std::vector<const double*> Pa, Pb;
+    for (size_t i = 0; i < _env.size(); ++i) {
+      const int m = (int)_env[i]->getNBasisFunctions();
+      if (PenvA[i].rows != m || PenvA[i].cols != m || PenvB[i].rows != m || PenvB[i].cols != m)
+        throw SerenityError(XC_ERR_ARG, "NAddFuncPotential: environment density matrix and basis differ in size");
+      env.push_back(_env[i]->get());
+      Pa.push_back(PenvA[i].data.data());
+      Pb.push_back(PenvB[i].data.data());
+    }
+    Matrix Fa(n, n), Fb(n, n);
+    check(xc_nadd_fock_build_unrestricted(_act->get(), (int)env.size(), env.data(), (int)_f, (int)_p, _thr,
+                                          PactA.data.data(), PactB.data.data(), Pa.data(), Pb.data(),
+                                          Fa.data.data(), Fb.data.data(), &_energy));
+    _built = true;
+    return {std::move(Fa), std::move(Fb)};
+  }
   double getEnergy() const {
     if (!_built) throw SerenityError(XC_ERR_STATE, "NAddFuncPotential: getMatrix has not been called");
     return _energy;
