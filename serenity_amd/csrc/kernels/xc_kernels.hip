@@ -677,6 +677,34 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   }
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int npad = bk.npad;
+  const bool diag = it.nb == 0;
+  const int R = it.na + it.nb;
+  const T* tb = tiles + bk.tile_off;
+  const int64_t cstride = (int64_t)npad * kTile;
+  vec_t raw[NTASK][4];
+  auto grow_of = [&](int lr) { return lr < it.na ? it.a0 + lr : it.b0 + (lr - it.na); };
+  // staging task -> (row, quarter of the chunk): 16 consecutive lanes take 16 rows of one
+  // quarter, so their 16-byte LDS stores (144-byte row stride) cover distinct banks
+  const int q = (tid >> 4) & 3;
+  auto row_of = [&](int task) { return (task & 15) + 16 * (task >> 6); };
+  auto load = [&](int c) {
+    const int p0 = c * KC;
+#pragma unroll
+    for (int m = 0; m < NTASK; ++m) {
+      const int task = tid + NT * m;
+      if (task < 4 * R) {
+        const T* src = tb + (int64_t)grow_of(row_of(task)) * kTile + p0 + KV * q;
+        raw[m][0] = *reinterpret_cast<const vec_t*>(src);
+        if (gga) {
+          raw[m][1] = *reinterpret_cast<const vec_t*>(src + cstride);
+          raw[m][2] = *reinterpret_cast<const vec_t*>(src + 2 * cstride);
+          raw[m][3] = *reinterpret_cast<const vec_t*>(src + 3 * cstride);
+        }
+      }
+    }
+  };
+  load(0);  // in flight while the potentials are staged and the block skip is decided
   if (tid < kTile) {
     const bool in = tid < bk.npts;
     const int64_t q = bk.first + tid;
@@ -696,11 +724,6 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   __syncthreads();
   if ((red[0] + red[1]) / bk.npts < thr) return;
 
-  const int npad = bk.npad;
-  const bool diag = it.nb == 0;
-  const int R = it.na + it.nb;
-  const T* tb = tiles + bk.tile_off;
-  const int64_t cstride = (int64_t)npad * kTile;
 
   // sub-tile t of the item -> (i, j) in 16-row units (i <= j on a diagonal item)
   auto decode = [&](int t, int& i, int& j) {
@@ -730,28 +753,6 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     lj[s] = __builtin_amdgcn_readfirstlane(t < it.nsub ? (diag ? 16 * j : it.na + 16 * j) : 0);
   }
 
-  vec_t raw[NTASK][4];
-  auto grow_of = [&](int lr) { return lr < it.na ? it.a0 + lr : it.b0 + (lr - it.na); };
-  // staging task -> (row, quarter of the chunk): 16 consecutive lanes take 16 rows of one
-  // quarter, so their 16-byte LDS stores (144-byte row stride) cover distinct banks
-  const int q = (tid >> 4) & 3;
-  auto row_of = [&](int task) { return (task & 15) + 16 * (task >> 6); };
-  auto load = [&](int c) {
-    const int p0 = c * KC;
-#pragma unroll
-    for (int m = 0; m < NTASK; ++m) {
-      const int task = tid + NT * m;
-      if (task < 4 * R) {
-        const T* src = tb + (int64_t)grow_of(row_of(task)) * kTile + p0 + KV * q;
-        raw[m][0] = *reinterpret_cast<const vec_t*>(src);
-        if (gga) {
-          raw[m][1] = *reinterpret_cast<const vec_t*>(src + cstride);
-          raw[m][2] = *reinterpret_cast<const vec_t*>(src + 2 * cstride);
-          raw[m][3] = *reinterpret_cast<const vec_t*>(src + 3 * cstride);
-        }
-      }
-    }
-  };
   auto store = [&](int c) {
     // the KV points of this thread's tasks are the same for every task
     const int p0 = c * KC + KV * q;
@@ -786,7 +787,6 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   const int r = lane & 15, g = lane >> 4;
   // slots of this wavefront that hold a sub-tile (an item smaller than its class leaves the rest idle)
   const int slive = it.nsub > w ? (it.nsub - w + kFockWaves - 1) / kFockWaves : 0;
-  load(0);
   for (int c = 0; c < NCH; ++c) {
     store(c);
     __syncthreads();
