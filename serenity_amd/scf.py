@@ -316,7 +316,8 @@ def gpu_xc_build(pot):
 
 
 GUESS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "initial_guess.json")
-ELEMENT_SYMBOLS = {1: "H", 2: "HE", 6: "C", 7: "N", 8: "O"}
+ELEMENT_SYMBOLS = {z + 1: s for z, s in enumerate(("H", "HE", "LI", "BE", "B", "C", "N", "O", "F", "NE", "NA", "MG", "AL",
+                                                     "SI", "P", "S", "CL", "AR"))}
 
 
 def atomic_density_guess(basis_name, Z, spherical=True):

@@ -131,6 +131,11 @@ def _system(kind):
     elif kind == "h2o4_tzvp_pbe":
         Z, X = W.water_cluster(2, 2, 1)
         func, basis = "PBE", "DEF2-TZVP"
+    elif kind == "li_ar_tzvp_pbe":
+        # one atom of every element Li..Ar plus two H, on a 2.6 A cubic lattice
+        Z = np.array(list(range(3, 19)) + [1, 1])
+        X = np.array([[i % 3, (i // 3) % 3, i // 9] for i in range(len(Z))], dtype=float) * 2.6 * xc.ANGSTROM_TO_BOHR
+        func, basis = "PBE", "DEF2-TZVP"
     elif kind == "config1_h2o8_svp_lda":
         Z, X = W.water_cluster(2, 2, 2)
         func, basis = "LDA", "DEF2-SVP"
@@ -148,7 +153,7 @@ def _system(kind):
     return b, g, P, func
 
 
-SYSTEMS = ["h2o_fixture_pbe", "h2o2_svp_lda", "butane_svp_pbe", "h2o4_tzvp_pbe", "config1_h2o8_svp_lda",
+SYSTEMS = ["h2o_fixture_pbe", "h2o2_svp_lda", "butane_svp_pbe", "h2o4_tzvp_pbe", "li_ar_tzvp_pbe", "config1_h2o8_svp_lda",
            "config2_c20h42_svp_pbe", "config3_h2o64_tzvp_pbe"]
 
 

@@ -181,6 +181,6 @@ def test_host_error_paths():
         xc.molecular_grid([8, 1], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 5e-6]]), accuracy=1)
     assert xc.molecular_grid([8, 1], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.8]]), accuracy=1).getNGridPoints() > 0
     with pytest.raises(xc.XCError, match="no entry for element"):
-        xc.BasisController.from_file("DEF2-SVP", [11], np.zeros((1, 3)))
+        xc.BasisController.from_file("DEF2-SVP", [19], np.zeros((1, 3)))  # the tables hold H..Ar
     with pytest.raises(xc.XCError, match="accuracy"):
         xc.molecular_grid([8], np.zeros((1, 3)), accuracy=9)

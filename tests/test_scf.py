@@ -174,16 +174,35 @@ def test_atom_scf_guess_matches_reference():
     assert abs(np.vdot(scf.atomic_density_guess("DEF2-SVP", Zw), bw.overlap()) - 10.0) < 1e-2
 
 
-def test_atom_scf_guess_converges_to_the_same_energy():
-    """Water / def2-SVP / LDA with RI-J from the ATOM_SCF and the core-Hamiltonian guesses: the same
-    SCF energy, in fewer cycles from the atomic densities."""
-    Z, X = water_geometry()
+@pytest.mark.parametrize("molecule", ["water", "hcl"])
+def test_atom_scf_guess_converges_to_the_same_energy(molecule):
+    """Water and HCl (1.2746 A) / def2-SVP / LDA with RI-J from the ATOM_SCF and the core-Hamiltonian
+    guesses: the same SCF energy, in fewer cycles from the atomic densities."""
+    if molecule == "water":
+        Z, X = water_geometry()
+    else:
+        Z, X = np.array([17, 1]), np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.2746 * xc.ANGSTROM_TO_BOHR]])
     b = xc.BasisController.from_file("DEF2-SVP", Z, X)
     a = xc.BasisController.from_file("RI_J_WEIGEND", Z, X)
     ints = scf.ScfIntegrals(b, Z, X, a)
     grid = xc.molecular_grid(Z, X, accuracy=2)
     build = oracle_xc(b, grid, "LDA")
-    r_core = scf.perform(ints, 5, build)
-    r_atom = scf.perform(ints, 5, build, P0=scf.atomic_density_guess("DEF2-SVP", Z))
+    nocc = int(np.sum(Z)) // 2
+    r_core = scf.perform(ints, nocc, build)
+    r_atom = scf.perform(ints, nocc, build, P0=scf.atomic_density_guess("DEF2-SVP", Z))
     assert abs(r_core["energy"] - r_atom["energy"]) < 1e-9
     assert r_atom["cycles"] < r_core["cycles"]
+
+
+@pytest.mark.parametrize("z", range(1, 19))
+def test_free_atom_guess_holds_z_electrons(z):
+    """The stored free-atom density of every element H..Ar holds Z electrons in the MINAO basis it is
+    stored in (tr P S = Z to rounding); projected into def2-TZVP it keeps Z to 2e-3 (def2-TZVP spans the
+    MINAO orbitals of H..Ar to that accuracy).  A wrong element, AO order or normalisation in the
+    basis or guess tables changes the count."""
+    Z, X = np.array([z]), np.zeros((1, 3))
+    for basis, tol in (("MINAO", 1e-10), ("DEF2-TZVP", 2e-3)):
+        b = xc.BasisController.from_file(basis, Z, X)
+        n = np.vdot(scf.atomic_density_guess(basis, Z), b.overlap())
+        assert abs(n - z) < tol, (basis, z, n)
+
