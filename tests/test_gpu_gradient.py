@@ -70,6 +70,41 @@ def test_gradient_matches_oracle_water8_pbe(ctx):
     bf._release()
 
 
+def second_row_molecules():
+    """NaF, H2S, HCl and SiH4 about 3.5 A apart (Angstrom geometry, converted to bohr)."""
+    Z = np.array([11, 9, 16, 1, 1, 17, 1, 14, 1, 1, 1, 1])
+    X = np.array([[0, 0, 0], [0, 0, 1.93], [3.5, 0, 0], [3.5, 1.34, 0], [4.8, 0, 0.4], [0, 3.6, 0], [0, 3.6, 1.27],
+                  [-3.5, 0, 0], [-3.5, 1.48, 0], [-4.0, -0.7, 1.2], [-4.0, -0.7, -1.2], [-2.0, -0.5, 0]])
+    return Z, X * xc.ANGSTROM_TO_BOHR
+
+
+def test_gradient_matches_oracle_second_row_tzvp(ctx):
+    """Second-row atoms in def2-TZVP (f shells on every heavy atom): AO second-derivative tiles and
+    the PBE gradient against the oracle."""
+    Z, X = second_row_molecules()
+    b = xc.BasisController.from_file("DEF2-TZVP", Z, X)
+    g = xc.molecular_grid(Z, X, accuracy=2)
+    P = W.synthetic_density(b.overlap(), W.n_electrons(Z) // 2, seed=5)
+    bf = xc.BasisFunctionOnGridController(ctx, b, g, 128, 1e-9, 2)
+    worst = 0.0
+    for blk in range(0, bf.getNBlocks(), max(1, bf.getNBlocks() // 12)):
+        d = bf.getBlockOnGridData(blk)
+        B = d["functionValues"].shape[0]
+        o = O.ao_block(b, g.points[128 * blk:128 * blk + B], thr=1e-9, deriv=2)
+        assert np.array_equal(d["negligible"], o["negligible"])
+        for c in range(6):
+            worst = max(worst, np.abs(d["secondDerivativeValues"][c] - o["hess"][c]).max())
+    assert worst < 1e-10
+    pot = xc.FuncPotential(bf, "PBE", "FP64")
+    G = pot.getGeomGradients(P)
+    Go, Eo = O.xc_gradient(b, g.points, g.weights, P, "PBE", b.getAtomIndicesOfBasis(), len(Z))
+    print("NaF/H2S/HCl/SiH4 TZVP PBE: tiles %.2e max|dG| %.2e |dE| %.2e" % (worst, np.abs(G - Go).max(),
+                                                                            abs(pot.getEnergy() - Eo)))
+    assert np.abs(G - Go).max() < 1e-9
+    assert abs(pot.getEnergy() - Eo) < 1e-9
+    bf._release()
+
+
 def test_h2_tzvp_pbe_gradient_pin(ctx):
     Z, X = h2()
     t = scf.ScfTask(ctx, Z, X, basis="DEF2-TZVP", functional="PBE", ri=False, accuracy=7, small_accuracy=7,

@@ -59,6 +59,26 @@ def test_open_shell_alkane_matches_oracle(ctx, functional):
     bf._release()
 
 
+def test_open_shell_second_row_matches_oracle(ctx):
+    """PH2 and ClO radicals 4 A apart / def2-TZVP / PBE (f shells on P, Cl and O), 22 alpha and 20 beta
+    synthetic orbitals: FP64 vs oracle."""
+    Z = np.array([15, 1, 1, 17, 8])
+    X = np.array([[0, 0, 0], [1.42, 0, 0], [-0.3, 1.38, 0], [4.0, 0, 0], [4.0, 1.57, 0]]) * xc.ANGSTROM_TO_BOHR
+    b = xc.BasisController.from_file("DEF2-TZVP", Z, X)
+    g = xc.molecular_grid(Z, X, accuracy=3)
+    S = b.overlap()
+    Pa = W.synthetic_density(S, 22, seed=21) / 2.0
+    Pb = W.synthetic_density(S, 20, seed=22) / 2.0
+    bf = xc.BasisFunctionOnGridController(ctx, b, g, 128, 1e-9, 1)
+    pot = xc.FuncPotential(bf, "PBE", "FP64")
+    Fa, Fb = pot.getMatrixUnrestricted(Pa, Pb)
+    Oa, Ob, Eo = O.fock_build_pol(b, g.points, g.weights, Pa, Pb, "PBE")
+    dF = max(np.abs(Fa - Oa).max(), np.abs(Fb - Ob).max())
+    print("PH2 + ClO open shell PBE: max|dF| %.2e |dE| %.2e" % (dF, abs(pot.getEnergy() - Eo)))
+    assert dF < 1e-9 and abs(pot.getEnergy() - Eo) < 1e-9
+    bf._release()
+
+
 def test_closed_shell_equals_restricted(ctx):
     d = W.build_system(1)
     bf = xc.BasisFunctionOnGridController(ctx, d["basis"], d["grid"], 128, 1e-9, 1)
