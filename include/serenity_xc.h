@@ -45,8 +45,15 @@ enum { XC_FUNC_LDA = 0, XC_FUNC_PBE = 1, XC_FUNC_TF = 2, XC_FUNC_SLATER = 3, XC_
 /* Precision of the two dense contractions (phi.P and phi^T.(w v phi)).  The
  * functional, the E_xc reduction and the F accumulation are FP64 in every mode.
  * XC_PREC_FP32 contracts with fl32(P) and adds the FP64 term tr(F (P - fl32(P)))
- * to E_xc (dE_xc/dP = F), which removes the first-order error of rounding P. */
-enum { XC_PREC_FP64 = 0, XC_PREC_FP32 = 1 };
+ * to E_xc (dE_xc/dP = F), which removes the first-order error of rounding P.
+ * XC_PREC_MIXED is XC_PREC_FP32 plus an FP64 refinement of the density of the
+ * blocks whose largest density after the FP32 pass exceeds 2 bohr^-3 (the
+ * heavy-atom cores, about a tenth of the density work of a water cluster),
+ * contracted again with FP64 tiles and fl32(P) in FP64: it removes the FP32
+ * rounding bias of E_xc that grows with the number of heavy atoms.  It needs a
+ * controller with both tile sets (keep_fp32 = 1) and is available for the
+ * restricted single-system builds (xc_fock_build[_device], serving). */
+enum { XC_PREC_FP64 = 0, XC_PREC_FP32 = 1, XC_PREC_MIXED = 2 };
 
 /* Grid partition and radial quadrature (Serenity Options::GRID_TYPES / RADIAL_GRID_TYPES). */
 enum { XC_GRID_BECKE = 0, XC_GRID_VORONOI = 1, XC_GRID_SSF = 2 };
@@ -346,9 +353,10 @@ int xc_server_stats(const xc_server* server, int* busy_slots, int64_t* submitted
 /* Stage timing with HIP events on the context stream.  While enabled, every
  * build records an event between its stages; xc_profile_collect waits for
  * them, returns the summed milliseconds of each stage over the builds since
- * the last collect (index 0 P to FP32, 1 density, 2 functional, 3 clear of
- * the Fock accumulator, 4 fock, 5 finalize + energy sum) and resets. */
-enum { XC_STAGE_CONVERT = 0, XC_STAGE_DENSITY = 1, XC_STAGE_FUNCTIONAL = 2, XC_STAGE_CLEAR = 3, XC_STAGE_FOCK = 4,
+ * the last collect (index 0 P to FP32, 1 density, 2 functional, 3 fock (the
+ * blocks' sub-tiles), 4 reduce of the sub-tiles into F's rows, 5 finalize +
+ * energy sum) and resets. */
+enum { XC_STAGE_CONVERT = 0, XC_STAGE_DENSITY = 1, XC_STAGE_FUNCTIONAL = 2, XC_STAGE_FOCK = 3, XC_STAGE_REDUCE = 4,
        XC_STAGE_FINALIZE = 5, XC_NSTAGES = 6 };
 int xc_profile(xc_ctx* ctx, int enable);
 int xc_profile_collect(xc_ctx* ctx, double* stage_ms, int* nbuilds);

@@ -13,8 +13,8 @@ XC_OK = 0
 XC_BUSY = 6
 FUNCTIONALS = {"LDA": 0, "PBE": 1, "TF": 2, "SLATER": 3, "VWN5": 4, "PBEX": 5, "PBEC": 6, "PBE_LIBXC": 7}
 GGA = {"PBE", "PBEX", "PBEC", "PBE_LIBXC"}
-PRECISIONS = {"FP64": 0, "FP32": 1}
-STAGES = ["convert", "density", "functional", "clear", "fock", "finalize"]
+PRECISIONS = {"FP64": 0, "FP32": 1, "MIXED": 2}
+STAGES = ["convert", "density", "functional", "fock", "reduce", "finalize"]
 GRID_TYPES = {"BECKE": 0, "VORONOI": 1, "SSF": 2}
 RADIAL_TYPES = {"BECKE": 0, "HANDY": 1, "AHLRICHS": 2, "KNOWLES": 3, "EQUI": 4}
 

@@ -32,6 +32,7 @@ struct DevShell {
 
 struct DevBlock {
   int64_t tile_off;   // element offset of the block's tiles
+  int64_t toff;       // element offset of the block's Fock sub-tiles in the sub-tile scratch
   int64_t first;      // first grid point
   int npts;           // points in the block (<= 128)
   int nb;             // non-negligible AOs
@@ -41,6 +42,27 @@ struct DevBlock {
   int nshell;         // significant shells
   int npass;          // density column passes (partial rho/grad slots written)
   int sys;            // system of a packed build (index into its PackSys table; 0 otherwise)
+};
+
+// Fock sub-tiles of a block: the npad / 16 = nt sub-tile rows and columns of its upper
+// triangle, (I, J) with I <= J stored at sub-tile index I nt - I (I - 1) / 2 + (J - I), each as
+// 16 x 16 row-major values, in the block's slice of the scratch (DevBlock::toff).
+__host__ __device__ inline int64_t fock_subtile_index(int I, int J, int nt) {
+  return (int64_t)I * nt - (int64_t)I * (I - 1) / 2 + (J - I);
+}
+__host__ __device__ inline int64_t fock_subtile_elems(int npad) {
+  const int64_t nt = npad / 16;
+  return nt * (nt + 1) / 2 * 256;
+}
+
+// One contribution to a row of the Fock matrix: block `block` holds AO `row` at compact position
+// c0; its sub-tile row for that AO starts at element rowbase of the scratch (sub-tile (c0 / 16,
+// c0 / 16), row c0 % 16), and the compact columns c0 .. c1 - 1 (AOs idx[idx_off + c], in
+// increasing global order) carry the values F[row][idx[c]] of the block.  Column c lies in
+// sub-tile c / 16, at rowbase + (c / 16 - c0 / 16) * 256 + c % 16.
+struct RowEnt {
+  int64_t rowbase;
+  int c0, c1, idx_off, block;
 };
 
 // One system of a packed build: several controllers' builds in one launch per
@@ -56,7 +78,10 @@ struct PackSys {
   const double* w;      // grid weights
   double* vs;           // w v
   double* gw;           // w g
-  double* U;            // Fock accumulator
+  double* U;            // Fock accumulator (lower triangle, written by the row reduce)
+  void* T;              // Fock sub-tile scratch (contraction precision)
+  const RowEnt* ent;    // row entries of the reduce (sorted by row, then block)
+  const int* row_ptr;   // n + 1 offsets into ent
   double* F;            // XC matrix out
   double* eblk;         // per-block energies
   double* round;        // rounding-term partials (FP32)
@@ -111,11 +136,31 @@ void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const in
 // Partial densities: part[(pass * 4 + c) * npts + q], c = rho, d/dx, d/dy, d/dz
 // (gradient slots hold sum T grad phi without the factor 2).
 constexpr int kDensMaxS = 6;
+// Mixed precision: blocks whose largest density (bohr^-3) after the FP32 density pass exceeds
+// this are contracted again in FP64 (the cores of the heavy atoms, where FP32 rounding of the AO
+// values and of phi P gives a bias that does not cancel over the grid).
+constexpr double kMixedRhoThreshold = 2.0;
 constexpr int kDensIdxLds = 2048;  // AO index lists up to this length are staged in LDS
+// With nwork_dev, the launch covers a list filled on the device: only its first *nwork_dev
+// items run (nwork is the list's capacity).
 template <typename T>
 void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
                         const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
-                        int64_t npts, const PackSys* pack = nullptr);
+                        int64_t npts, const PackSys* pack = nullptr, const int* nwork_dev = nullptr);
+
+// Dense-block refinement of the mixed-precision density: the work lists (one per column-tile
+// class) and their device-side counts, filled by launch_mark_dense.
+struct DenseLists {
+  WorkItem* item[kDensMaxS + 1];
+  int* count;  // kDensMaxS + 1 counters
+};
+// Clears the counts, then appends the density items of every block whose largest density in the
+// partials part (one pass per item, 128-point blocks) exceeds rho_thr; by_block holds every block's
+// items contiguously from item0[b] (blocks[b].npass of them), cls[b] is their class.
+void launch_mark_dense(hipStream_t s, const DevBlock* blocks, int nblocks, int64_t npts, const double* part,
+                       double rho_thr, const WorkItem* by_block, const int* item0, const int* cls,
+                       const DenseLists& lists);
+void launch_to_double(hipStream_t s, const float* src, double* dst, int64_t n);
 
 // rho[q] / grad[.][q] from the partials of the block of q (any block size).
 void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
@@ -139,15 +184,24 @@ void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t
                            const double* gb2 = nullptr);
 
 // Sub-tile Fock kernel: kFockWaves wavefronts, slot class sc (kFockS[sc] sub-tiles per wavefront).
+// Each item stores its sub-tiles of phi^T G + G^T phi into the block's scratch slice Tscr + toff
+// (zeros when the block-average test skips the block).
 template <typename T>
 void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,
                      const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
-                     double block_ave_thr, double* U, int n, int64_t npts, const PackSys* pack = nullptr);
-// Packed elementwise stages over nsys systems (maxnn = largest n^2): P -> FP32, U = 0,
+                     double block_ave_thr, T* Tscr, int64_t npts, const PackSys* pack = nullptr);
+// U[j + n i] = sum over the row entries of AO i (block order) of the stored sub-tile values at
+// columns j >= i, for every i of [0, n): the lower triangle of the Fock accumulator, each element
+// written once and summed in a fixed order.  Packed: blockIdx.y = system, n = the largest n.
+constexpr int kReduceWaves = 8;
+constexpr int kReduceWindow = 8192;  // row columns accumulated in LDS per pass
+template <typename T>
+void launch_fock_reduce(hipStream_t s, const RowEnt* ent, const int* row_ptr, const T* Tscr, const int* ao_idx,
+                        double* U, int n, const PackSys* pack = nullptr, int nsys = 1);
+// Packed elementwise stages over nsys systems (maxnn = largest n^2): P -> FP32,
 // F = mirror of U (+ FP32 rounding partials), E = sum of block energies (+ rounding term).
 constexpr int kPackRoundBlocks = 512;  // finalize workgroups per system of a packed build
 void launch_pack_to_float(hipStream_t s, const PackSys* pack, int nsys, int64_t maxnn);
-void launch_pack_clear(hipStream_t s, const PackSys* pack, int nsys, int64_t maxnn);
 void launch_pack_finalize(hipStream_t s, const PackSys* pack, int nsys, int fp32);
 void launch_pack_sum(hipStream_t s, const PackSys* pack, int nsys, int fp32);
 

@@ -7,8 +7,11 @@
 //               (MatrixOperatorToGridTransformer.cpp:103-165)
 //   functional  per-point XC energy density and potential, block skip and E_xc partials
 //               (XCFun.cpp:129-158, 751-765)
-//   fock        F += phi_b^T G_b + G_b^T phi_b on matrix cores, scattered into the lower
-//               triangle of U with FP64 atomics (ScalarOperatorToMatrixAdder.cpp:179-303)
+//   fock        S_b = phi_b^T G_b + G_b^T phi_b on matrix cores, the upper-triangle sub-tiles
+//               of every block stored to a scratch (ScalarOperatorToMatrixAdder.cpp:179-303)
+//   fock_reduce F = sum over blocks of S_b scattered by the blocks' AO lists, one AO row per
+//               workgroup summing its blocks in a fixed order (the reference's thread-private
+//               F plus serial sum, ScalarOperatorToMatrixAdder.cpp:101-110)
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
@@ -204,7 +207,10 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
                                                           const int* __restrict__ ao_idx,
                                                           const T* __restrict__ tiles, const T* __restrict__ P,
                                                           int n, int want_grad, double* __restrict__ part,
-                                                          int64_t npts, const PackSys* __restrict__ pack) {
+                                                          int64_t npts, const PackSys* __restrict__ pack,
+                                                          const int* __restrict__ nwork_dev) {
+  // a launch over a list filled on the device runs only its first *nwork_dev items
+  if (nwork_dev && (int)blockIdx.x >= *nwork_dev) return;
   using M = Mfma<T>;
   constexpr int KC = 128 / (int)sizeof(T);      // AOs per chunk (128 bytes of each LDS row)
   constexpr int KV = KC / 4;
@@ -645,8 +651,8 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
                                                        const DevBlock* __restrict__ blocks,
                                                        const int* __restrict__ ao_idx, const T* __restrict__ tiles,
                                                        const double* __restrict__ vs, const double* __restrict__ gw,
-                                                       int gga, double thr, double* __restrict__ U, int n,
-                                                       int64_t npts, const PackSys* __restrict__ pack) {
+                                                       int gga, double thr, T* __restrict__ Tscr, int64_t npts,
+                                                       const PackSys* __restrict__ pack) {
   using M = Mfma<T>;
   constexpr int KC = 128 / (int)sizeof(T);       // points per chunk: one 128-byte line of each AO row
   constexpr int KV = KC / 4;
@@ -670,10 +676,8 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     tiles = static_cast<const T*>(ps.tiles);
     vs = ps.vs;
     gw = ps.gw;
-    U = ps.U;
-    n = ps.n;
+    Tscr = static_cast<T*>(ps.T);
     npts = ps.npts;
-    ao_idx = ps.ao_idx;
   }
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -722,7 +726,6 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     if (lane == 0) red[w] = a;
   }
   __syncthreads();
-  if ((red[0] + red[1]) / bk.npts < thr) return;
 
 
   // sub-tile t of the item -> (i, j) in 16-row units (i <= j on a diagonal item)
@@ -742,6 +745,20 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
       j = t - i * ntb;
     }
   };
+  // a skipped block (every item of it takes the same decision from the same values) leaves zeros
+  // in its sub-tiles, so the reduce needs no per-block test
+  if ((red[0] + red[1]) / bk.npts < thr) {
+    const int nt0 = npad / 16;
+    T* out0 = Tscr + bk.toff;
+    for (int t = w; t < it.nsub; t += kFockWaves) {
+      int i, j;
+      decode(t, i, j);
+      const int I = it.a0 / 16 + i, J = (diag ? it.a0 : it.b0) / 16 + j;
+      T* sub = out0 + fock_subtile_index(I, J, nt0) * 256;
+      for (int e = lane; e < 256; e += 64) sub[e] = T(0);
+    }
+    return;
+  }
   // this wavefront's sub-tiles as LDS row offsets (wavefront-uniform)
   int li[S], lj[S];
 #pragma unroll
@@ -819,23 +836,81 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     __syncthreads();
   }
 
-  const int* idx = ao_idx + bk.idx_off;
+  // store the sub-tiles (block sub-tile (I, J), I <= J; 16 lanes write 16 consecutive values of a row)
+  const int nt = npad / 16;
+  T* tb_out = Tscr + bk.toff;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int t = w + kFockWaves * s;
     if (t >= it.nsub) continue;
     int i, j;
     decode(t, i, j);
-    const int gi = it.a0 + 16 * i, gj = (diag ? it.a0 : it.b0) + 16 * j;
-    const bool dsub = diag && i == j;
-    const int gc = idx[gj + r];
+    const int I = it.a0 / 16 + i, J = (diag ? it.a0 : it.b0) / 16 + j;
+    T* sub = tb_out + fock_subtile_index(I, J, nt) * 256;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int gr = idx[gi + M::row(lane, e)];
-      if (gr < 0 || gc < 0) continue;
-      if (dsub && gr > gc) continue;
-      unsafeAtomicAdd(U + gc + (int64_t)n * gr, (double)acc[s][e]);
+    for (int e = 0; e < 4; ++e) sub[M::row(lane, e) * 16 + r] = acc[s][e];
+  }
+}
+
+// One AO row i of the Fock accumulator per workgroup: its entries (the blocks holding AO i, in
+// block order) are dealt to the kReduceWaves wavefronts round-robin; a wavefront adds each entry's
+// stored row segment into the workgroup's LDS copy of the row (lanes take consecutive columns; four
+// 64-column chunks are loaded before their LDS additions, so that many loads are in flight), and
+// the row is written once to U[j + n i], j >= i.  Rows longer than the LDS window are done in
+// several passes over the entries.  The LDS additions of different wavefronts meet in an order
+// that varies from run to run, so F varies at the rounding level.
+template <typename T>
+__global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const RowEnt* __restrict__ ent,
+                                                                        const int* __restrict__ row_ptr,
+                                                                        const T* __restrict__ Tscr,
+                                                                        const int* __restrict__ ao_idx,
+                                                                        double* __restrict__ U, int n, int window,
+                                                                        const PackSys* __restrict__ pack) {
+  extern __shared__ double racc[];  // window
+  if (pack) {
+    const PackSys& ps = pack[blockIdx.y];
+    ent = ps.ent;
+    row_ptr = ps.row_ptr;
+    Tscr = static_cast<const T*>(ps.T);
+    ao_idx = ps.ao_idx;
+    U = ps.U;
+    n = ps.n;
+  }
+  const int i = blockIdx.x;
+  if (i >= n) return;  // uniform over the workgroup
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
+  for (int w0 = 0; w0 < n - i; w0 += window) {
+    const int wl = min(window, n - i - w0);
+    for (int k = tid; k < wl; k += 64 * kReduceWaves) racc[k] = 0.0;
+    __syncthreads();
+    for (int e = e0 + w; e < e1; e += kReduceWaves) {
+      const RowEnt E = ent[e];
+      const int* idx = ao_idx + E.idx_off;
+      const T* trow = Tscr + E.rowbase;
+      const int I = E.c0 >> 4;
+      for (int c = E.c0 + lane; c < E.c1; c += 256) {
+        int g[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int cc = c + 64 * u;
+          g[u] = -1;
+          v[u] = 0.0;
+          if (cc < E.c1) {
+            g[u] = idx[cc] - i - w0;  // >= -w0: the AOs of a block are in increasing order
+            v[u] = (double)trow[(int64_t)((cc >> 4) - I) * 256 + (cc & 15)];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (g[u] >= 0 && g[u] < wl) atomicAdd(racc + g[u], v[u]);
+      }
     }
+    __syncthreads();
+    for (int j = tid; j < wl; j += 64 * kReduceWaves) U[(int64_t)(i + w0 + j) + (int64_t)n * i] = racc[j];
+    __syncthreads();
   }
 }
 
@@ -946,13 +1021,6 @@ __global__ void pack_to_float_kernel(const PackSys* __restrict__ pack) {
     ps.P32[i] = (float)ps.P64[i];
 }
 
-__global__ void pack_clear_kernel(const PackSys* __restrict__ pack) {
-  const PackSys& ps = pack[blockIdx.y];
-  const int64_t nn = (int64_t)ps.n * ps.n;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x)
-    ps.U[i] = 0.0;
-}
-
 // F = mirror of U; FP32 builds also leave one partial of sum F (P - P32) per workgroup
 __global__ __launch_bounds__(256) void pack_finalize_kernel(const PackSys* __restrict__ pack, int fp32) {
   const PackSys& ps = pack[blockIdx.y];
@@ -1017,13 +1085,13 @@ void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const in
 template <typename T>
 void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
                         const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
-                        int64_t npts, const PackSys* pack) {
+                        int64_t npts, const PackSys* pack, const int* nwork_dev) {
   if (nwork <= 0) return;
   switch (S) {
 #define SXC_DENS_S(SV)                                                                                        \
   case SV:                                                                                                    \
     hipLaunchKernelGGL((density_sub_kernel<T, SV>), dim3(nwork), dim3(512), 0, s, work, blocks, ao_idx, tiles, \
-                       P, n, want_grad, part, npts, pack);                                                    \
+                       P, n, want_grad, part, npts, pack, nwork_dev);                                         \
     return;
     static_assert(kDensMaxS == 6, "one case per column-tile class");
     SXC_DENS_S(1)
@@ -1038,9 +1106,59 @@ void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, c
 }
 template void launch_density_sub<double>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
                                          const double*, const double*, int, int, double*, int64_t,
-                                         const PackSys*);
+                                         const PackSys*, const int*);
 template void launch_density_sub<float>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
-                                        const float*, const float*, int, int, double*, int64_t, const PackSys*);
+                                        const float*, const float*, int, int, double*, int64_t, const PackSys*,
+                                        const int*);
+
+// Blocks whose largest density (from the partials of a first pass) exceeds rho_thr: their density
+// work items are appended to the list of their column-tile class, for a second pass.
+__global__ __launch_bounds__(128) void mark_dense_kernel(const DevBlock* __restrict__ blocks, int64_t npts,
+                                                         const double* __restrict__ part, double rho_thr,
+                                                         const WorkItem* __restrict__ by_block,
+                                                         const int* __restrict__ item0, const int* __restrict__ cls,
+                                                         DenseLists lists) {
+  __shared__ double red[128];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const DevBlock bk = blocks[b];
+  double r = 0.0;
+  if (t < bk.npts)
+    for (int s = 0; s < bk.npass; ++s) r += part[(int64_t)s * 4 * npts + bk.first + t];
+  red[t] = r;
+  __syncthreads();
+  for (int k = 64; k > 0; k >>= 1) {
+    if (t < k) red[t] = fmax(red[t], red[t + k]);
+    __syncthreads();
+  }
+  if (t == 0 && bk.npass > 0 && red[0] > rho_thr) {
+    const int c = cls[b];
+    const int pos = atomicAdd(lists.count + c, bk.npass);
+    for (int k = 0; k < bk.npass; ++k) lists.item[c][pos + k] = by_block[item0[b] + k];
+  }
+}
+
+__global__ void clear_counts_kernel(int* __restrict__ c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+__global__ void to_double_kernel(const float* __restrict__ s, double* __restrict__ d, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (double)s[i];
+}
+
+void launch_mark_dense(hipStream_t s, const DevBlock* blocks, int nblocks, int64_t npts, const double* part,
+                       double rho_thr, const WorkItem* by_block, const int* item0, const int* cls,
+                       const DenseLists& lists) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(clear_counts_kernel, dim3(1), dim3(64), 0, s, lists.count, kDensMaxS + 1);
+  hipLaunchKernelGGL(mark_dense_kernel, dim3(nblocks), dim3(128), 0, s, blocks, npts, part, rho_thr, by_block, item0,
+                     cls, lists);
+}
+
+void launch_to_double(hipStream_t s, const float* src, double* dst, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(to_double_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+}
 
 void launch_reduce_partials(hipStream_t s, const DevBlock* blocks, int bs, int64_t npts, const double* part,
                             int want_grad, double* rho, double* grad, int accumulate) {
@@ -1078,11 +1196,6 @@ void launch_pack_to_float(hipStream_t s, const PackSys* pack, int nsys, int64_t 
   hipLaunchKernelGGL(pack_to_float_kernel, dim3(pack_grid(maxnn), nsys), dim3(256), 0, s, pack);
 }
 
-void launch_pack_clear(hipStream_t s, const PackSys* pack, int nsys, int64_t maxnn) {
-  if (nsys <= 0) return;
-  hipLaunchKernelGGL(pack_clear_kernel, dim3(pack_grid(maxnn), nsys), dim3(256), 0, s, pack);
-}
-
 void launch_pack_finalize(hipStream_t s, const PackSys* pack, int nsys, int fp32) {
   if (nsys <= 0) return;
   hipLaunchKernelGGL(pack_finalize_kernel, dim3(kPackRoundBlocks, nsys), dim3(256), 0, s, pack, fp32);
@@ -1106,20 +1219,20 @@ void launch_functional_pol(hipStream_t s, int func, const XcParams& prm, int64_t
 template <typename T, int S>
 static void launch_fock_sub_s(hipStream_t s, const FockItem* work, int nwork, const DevBlock* blocks,
                               const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga,
-                              double thr, double* U, int n, int64_t npts, const PackSys* pack) {
+                              double thr, T* Tscr, int64_t npts, const PackSys* pack) {
   hipLaunchKernelGGL((fock_sub_kernel<T, S>), dim3(nwork), dim3(64 * kFockWaves), 0, s, work, blocks, ao_idx, tiles,
-                     vs, gw, gga, thr, U, n, npts, pack);
+                     vs, gw, gga, thr, Tscr, npts, pack);
 }
 
 template <typename T>
 void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, const DevBlock* blocks,
                      const int* ao_idx, const T* tiles, const double* vs, const double* gw, int gga, double thr,
-                     double* U, int n, int64_t npts, const PackSys* pack) {
+                     T* Tscr, int64_t npts, const PackSys* pack) {
   if (nwork <= 0) return;
   switch (kFockS[sc]) {
 #define SXC_FOCK_S(SV) \
   case SV:             \
-    return launch_fock_sub_s<T, SV>(s, work, nwork, blocks, ao_idx, tiles, vs, gw, gga, thr, U, n, npts, pack);
+    return launch_fock_sub_s<T, SV>(s, work, nwork, blocks, ao_idx, tiles, vs, gw, gga, thr, Tscr, npts, pack);
     SXC_FOCK_S(1)
     SXC_FOCK_S(2)
     SXC_FOCK_S(3)
@@ -1136,11 +1249,42 @@ void launch_fock_sub(hipStream_t s, int sc, const FockItem* work, int nwork, con
   throw std::runtime_error("launch_fock_sub: slot class beyond the accumulator budget of this precision");
 }
 template void launch_fock_sub<double>(hipStream_t, int, const FockItem*, int, const DevBlock*, const int*,
-                                      const double*, const double*, const double*, int, double, double*, int,
-                                      int64_t, const PackSys*);
+                                      const double*, const double*, const double*, int, double, double*, int64_t,
+                                      const PackSys*);
 template void launch_fock_sub<float>(hipStream_t, int, const FockItem*, int, const DevBlock*, const int*,
-                                     const float*, const double*, const double*, int, double, double*, int, int64_t,
+                                     const float*, const double*, const double*, int, double, float*, int64_t,
                                      const PackSys*);
+
+// Row columns per LDS pass of the reduce: kReduceWindow, or less where the device's LDS per
+// workgroup cannot hold kReduceWaves copies of it.
+static int reduce_window_limit() {
+  static int cached[64] = {};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev < 0 || dev >= 64) dev = 0;
+  if (!cached[dev]) {
+    int lds = 65536;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lds = 65536;
+    const int w = lds / (int)sizeof(double);
+    cached[dev] = (w < kReduceWindow ? w : kReduceWindow) / 64 * 64;
+  }
+  return cached[dev];
+}
+
+template <typename T>
+void launch_fock_reduce(hipStream_t s, const RowEnt* ent, const int* row_ptr, const T* Tscr, const int* ao_idx,
+                        double* U, int n, const PackSys* pack, int nsys) {
+  if (n <= 0 || nsys <= 0) return;
+  const int lim = reduce_window_limit();
+  const int window = n < lim ? n : lim;
+  const size_t lds = sizeof(double) * (size_t)window;
+  hipLaunchKernelGGL((fock_reduce_kernel<T>), dim3(n, nsys), dim3(64 * kReduceWaves), lds, s, ent, row_ptr, Tscr,
+                     ao_idx, U, n, window, pack);
+}
+template void launch_fock_reduce<double>(hipStream_t, const RowEnt*, const int*, const double*, const int*, double*,
+                                         int, const PackSys*, int);
+template void launch_fock_reduce<float>(hipStream_t, const RowEnt*, const int*, const float*, const int*, double*,
+                                        int, const PackSys*, int);
 
 void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
                              double* vs, double* gw) {

@@ -136,6 +136,7 @@ struct xc_ctx {
   // packed-build compositions seen before (controllers, functional, precision, threshold):
   // their own lists and, from the second build on, a captured graph
   std::map<std::vector<uint64_t>, PackWork*> pack_graphs;
+  uint64_t pack_calls = 0;  // packed builds so far (recency of the cached compositions)
   hipStream_t stream = nullptr;
   Lanes lanes;  // helper lanes of builds on the context stream
   std::recursive_mutex mu;
@@ -215,6 +216,7 @@ struct PackWork {
   DBuf<FockItem> fock[kFockNumS];
   DBuf<double> round, E;
   hipGraphExec_t exec = nullptr;  // captured build of a repeated composition
+  uint64_t last_use = 0;          // packed-build counter at its last use (least recently used is evicted)
   ~PackWork() {
     if (exec) (void)hipGraphExecDestroy(exec);
   }
@@ -291,9 +293,23 @@ struct xc_bfogc {
   std::vector<WorkItem> dens_work[kDensMaxS + 1];        // by column tiles per pass
   std::vector<FockItem> fock_items[2][kFockNumS];       // sub-tile Fock kernel, [fp64][slot class]
   DBuf<WorkItem> d_dens_work[kDensMaxS + 1];
+  // mixed precision: every block's density items contiguously (from dens_item0[b], class
+  // dens_cls[b]); the device lists of the dense blocks refined in FP64 and their counts; the
+  // FP32-rounded density matrix in FP64
+  std::vector<WorkItem> dens_by_block;
+  std::vector<int> dens_item0, dens_cls;
+  DBuf<WorkItem> d_dens_by_block, d_dense_list[kDensMaxS + 1];
+  DBuf<int> d_dens_item0, d_dens_cls, d_dense_cnt;
+  DBuf<double> d_P32d;
   DBuf<FockItem> d_fock_items[2][kFockNumS];
   // per-build work buffers
   DBuf<double> d_part, d_rho, d_grad, d_vs, d_gw, d_eblk, d_U, d_P, d_F, d_v, d_g, d_E;
+  // Fock sub-tile scratch (FP64 size when FP64 tiles are kept, else FP32) and the row entries of
+  // the reduce (sorted by AO row, then block)
+  int64_t t_elems = 0;
+  DBuf<double> d_T;
+  DBuf<RowEnt> d_rowent;
+  DBuf<int> d_rowptr;
   DBuf<float> d_P32;
   DBuf<double> d_round;  // finalize-block partials of the FP32 rounding term, per spin
   // supersystem density of a non-additive build with this controller active (allocated on first use)
@@ -344,17 +360,63 @@ void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad, const 
   });
 }
 
+// Mixed precision: FP64 density items (FP64 tiles, the FP32-rounded P in FP64) for the blocks
+// whose largest density after the FP32 pass exceeds kMixedRhoThreshold; they overwrite those
+// blocks' partials.  The lists are filled on the device, so the launches cover their capacity
+// and run only the filled part.
+void run_density_refine(xc_bfogc* bf, int want_grad, const Lanes& L) {
+  DenseLists lists{};
+  for (int c = 0; c <= kDensMaxS; ++c) lists.item[c] = bf->d_dense_list[c].p;
+  lists.count = bf->d_dense_cnt.p;
+  launch_mark_dense(L.main, bf->d_blocks.p, bf->nblocks, bf->grid->npts, bf->d_part.p, kMixedRhoThreshold,
+                    bf->d_dens_by_block.p, bf->d_dens_item0.p, bf->d_dens_cls.p, lists);
+  over_lanes(L, kDensMaxS + 1, [&](int c) { return c >= 1 ? (int)bf->dens_work[c].size() : 0; },
+             [&](int c, hipStream_t s) {
+    launch_density_sub<double>(s, c, bf->d_dense_list[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
+                               bf->d_ao_idx.p, bf->d_t64.p, bf->d_P32d.p, bf->basis->nao, want_grad, bf->d_part.p,
+                               bf->grid->npts, nullptr, bf->d_dense_cnt.p + c);
+  });
+}
+
+// A controller whose server build is still running owns its work buffers (P, partial
+// densities, potentials, U) until the build completes: refuse any other use first.
+void require_idle(const xc_bfogc* bf, const char* who) {
+  require(!bf->in_flight, XC_ERR_STATE, std::string(who) + ": a server build of this controller is in flight");
+}
+
+void require_idle(const xc_bfogc* act, int nenv, xc_bfogc* const* env, const char* who) {
+  require_idle(act, who);
+  for (int i = 0; i < nenv; ++i)
+    if (env && env[i]) require_idle(env[i], who);
+}
+
+// The Fock stage in two steps: the blocks' sub-tiles into the scratch (run_fock_tiles), then the
+// row reduce into the lower triangle of U (run_fock_reduce; every element of that triangle is
+// written, so U needs no clearing).  run_fock does both.
 template <typename T>
-void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr, const Lanes& L, const double* vs = nullptr,
-              const double* gw = nullptr) {
+void run_fock_tiles(xc_bfogc* bf, const T* tiles, int gga, double thr, const Lanes& L, const double* vs = nullptr,
+                    const double* gw = nullptr) {
   const int f = sizeof(T) == 8 ? 1 : 0;
   if (!vs) vs = bf->d_vs.p;
   if (!gw) gw = bf->d_gw.p;
+  T* Tscr = reinterpret_cast<T*>(bf->d_T.p);
   over_lanes(L, kFockNumS, [&](int c) { return (int)bf->fock_items[f][c].size(); }, [&](int c, hipStream_t s) {
     launch_fock_sub<T>(s, c, bf->d_fock_items[f][c].p, (int)bf->fock_items[f][c].size(),
-                       bf->d_blocks.p, bf->d_ao_idx.p, tiles, vs, gw, gga, thr, bf->d_U.p,
-                       bf->basis->nao, bf->grid->npts);
+                       bf->d_blocks.p, bf->d_ao_idx.p, tiles, vs, gw, gga, thr, Tscr, bf->grid->npts);
   });
+}
+
+template <typename T>
+void run_fock_reduce(xc_bfogc* bf, const Lanes& L) {
+  launch_fock_reduce<T>(L.main, bf->d_rowent.p, bf->d_rowptr.p, reinterpret_cast<const T*>(bf->d_T.p),
+                        bf->d_ao_idx.p, bf->d_U.p, bf->basis->nao);
+}
+
+template <typename T>
+void run_fock(xc_bfogc* bf, const T* tiles, int gga, double thr, const Lanes& L, const double* vs = nullptr,
+              const double* gw = nullptr) {
+  run_fock_tiles<T>(bf, tiles, gga, thr, L, vs, gw);
+  run_fock_reduce<T>(bf, L);
 }
 
 }  // namespace
@@ -796,6 +858,8 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       // 2. block lists (compact AO index lists padded to kAoPad) and tile offsets
       std::vector<int> sig_shell, sig_pos;
       bf->blocks.resize(nb);
+      bf->dens_item0.assign(nb, 0);
+      bf->dens_cls.assign(nb, 1);
       int64_t tile_off = 0;
       for (int b = 0; b < nb; ++b) {
         DevBlock& bk = bf->blocks[b];
@@ -819,6 +883,8 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
         for (int i = cnt; i < bk.npad; ++i) bf->ao_idx.push_back(-1);
         bk.tile_off = tile_off;
         tile_off += (int64_t)bk.npad * kTile * bf->ncomp;
+        bk.toff = bf->t_elems;
+        bf->t_elems += fock_subtile_elems(bk.npad);
         bf->sum_nb += cnt;
         bf->sum_B_nb2 += (double)bk.npts * cnt * cnt;
         bk.npass = 0;
@@ -828,9 +894,12 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
           const int width = ((bk.npad + npass - 1) / npass + 15) / 16 * 16;
           int s = 0;
           // every pass of a block in the class of the widest, so that they share a launch
+          bf->dens_item0[b] = (int)bf->dens_by_block.size();
+          bf->dens_cls[b] = width / 16;
           for (int c0 = 0; c0 < bk.npad; c0 += width, ++s) {
             const int ncols = std::min(width, bk.npad - c0);
             bf->dens_work[width / 16].push_back(WorkItem{b, c0, ncols, s});
+            bf->dens_by_block.push_back(WorkItem{b, c0, ncols, s});
           }
           bk.npass = s;
           bf->npass_max = std::max(bf->npass_max, s);
@@ -871,21 +940,56 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       for (int f = 0; f < 2; ++f)
         for (int c = 0; c < kFockNumS; ++c) xcd_order(bf->fock_items[f][c]);
       bf->tile_elems = tile_off;
+      // row entries of the Fock reduce: for every AO, the blocks holding it (in block order) with
+      // the position of its sub-tile row in the scratch
+      const int nao = basis->nao;
+      std::vector<int> rowptr(nao + 1, 0);
+      for (int b = 0; b < nb; ++b)
+        for (int l = 0; l < bf->blocks[b].nb; ++l) ++rowptr[bf->ao_idx[bf->blocks[b].idx_off + l] + 1];
+      for (int i = 0; i < nao; ++i) rowptr[i + 1] += rowptr[i];
+      std::vector<RowEnt> rowent(std::max(rowptr[nao], 1));
+      {
+        std::vector<int> fill(rowptr.begin(), rowptr.end() - 1);
+        for (int b = 0; b < nb; ++b) {
+          const DevBlock& bk = bf->blocks[b];
+          const int nt = bk.npad / 16;
+          for (int l = 0; l < bk.nb; ++l) {
+            RowEnt e;
+            e.rowbase = bk.toff + fock_subtile_index(l / 16, l / 16, nt) * 256 + (l % 16) * 16;
+            e.c0 = l;
+            e.c1 = bk.nb;
+            e.idx_off = bk.idx_off;
+            e.block = b;
+            rowent[fill[bf->ao_idx[bk.idx_off + l]]++] = e;
+          }
+        }
+      }
       // 3. admission by footprint
       size_t free_b = 0, total_b = 0;
       HIPX(hipMemGetInfo(&free_b, &total_b));
       const double need =
-          (double)tile_off * ((keep_fp32 != 2 ? sizeof(double) : 0) + (keep_fp32 ? sizeof(float) : 0));
+          (double)tile_off * ((keep_fp32 != 2 ? sizeof(double) : 0) + (keep_fp32 ? sizeof(float) : 0)) +
+          (double)bf->t_elems * (keep_fp32 != 2 ? sizeof(double) : sizeof(float));
       require(need < 0.9 * (double)free_b, XC_ERR_MEMORY,
-              "xc_bfogc_create: AO tiles need " + std::to_string(need / 1e9) + " GB, " +
+              "xc_bfogc_create: AO tiles and Fock scratch need " + std::to_string(need / 1e9) + " GB, " +
                   std::to_string(free_b / 1e9) + " GB free");
       // 4. upload lists, evaluate AO tiles
       bf->d_blocks.upload(ctx, bf->blocks.data(), bf->blocks.size());
+      bf->d_rowptr.upload(ctx, rowptr.data(), rowptr.size());
+      bf->d_rowent.upload(ctx, rowent.data(), rowent.size());
       bf->d_ao_idx.upload(ctx, bf->ao_idx.data(), bf->ao_idx.size());
       bf->d_sig_shell.upload(ctx, sig_shell.data(), sig_shell.size());
       bf->d_sig_pos.upload(ctx, sig_pos.data(), sig_pos.size());
       for (int c = 0; c <= kDensMaxS; ++c)
         bf->d_dens_work[c].upload(ctx, bf->dens_work[c].data(), bf->dens_work[c].size());
+      if (keep_fp32 == 1) {  // both tile sets: the mixed-precision build is available
+        bf->d_dens_by_block.upload(ctx, bf->dens_by_block.data(), bf->dens_by_block.size());
+        bf->d_dens_item0.upload(ctx, bf->dens_item0.data(), bf->dens_item0.size());
+        bf->d_dens_cls.upload(ctx, bf->dens_cls.data(), bf->dens_cls.size());
+        for (int c = 1; c <= kDensMaxS; ++c)
+          bf->d_dense_list[c].alloc(ctx, std::max<size_t>(bf->dens_work[c].size(), 1));
+        bf->d_dense_cnt.alloc(ctx, kDensMaxS + 1);
+      }
       for (int f = 0; f < 2; ++f)
         for (int c = 0; c < kFockNumS; ++c)
           bf->d_fock_items[f][c].upload(ctx, bf->fock_items[f][c].data(), bf->fock_items[f][c].size());
@@ -915,6 +1019,9 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
       bf->d_g.alloc(ctx, 3 * N);
       bf->d_eblk.alloc(ctx, std::max(nxc, 1));
       bf->d_U.alloc(ctx, (size_t)n * n);
+      // sub-tile scratch in the widest contraction precision the controller supports
+      const int64_t t_doubles = keep_fp32 != 2 ? bf->t_elems : (bf->t_elems + 1) / 2;
+      bf->d_T.alloc(ctx, (size_t)std::max<int64_t>(t_doubles, 1));
       bf->d_P.alloc(ctx, (size_t)n * n);
       bf->d_F.alloc(ctx, (size_t)n * n);
       bf->d_E.alloc(ctx, 1);
@@ -922,6 +1029,7 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
         bf->d_P32.alloc(ctx, (size_t)n * n);
         bf->d_round.alloc(ctx, 2 * (size_t)finalize_blocks(n));
       }
+      if (keep_fp32 == 1) bf->d_P32d.alloc(ctx, (size_t)n * n);
       HIPX(hipStreamSynchronize(ctx->stream));
     } catch (...) {
       delete bf;
@@ -935,6 +1043,7 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
 int xc_bfogc_destroy(xc_bfogc* bfogc) {
   return guarded([&] {
     if (!bfogc) return;
+    std::lock_guard<std::recursive_mutex> lk(bfogc->ctx->mu);
     require(!bfogc->in_flight, XC_ERR_STATE, "xc_bfogc_destroy: a server build of this controller is in flight");
     (void)hipStreamSynchronize(bfogc->ctx->stream);
     forget_packs_of(bfogc->ctx, bfogc);
@@ -1100,7 +1209,6 @@ int xc_fock_add(xc_bfogc* bf, double thr, const double* v, const double* g, doub
     if (g) HIPX(hipMemcpyAsync(bf->d_g.p, g, sizeof(double) * 3 * N, hipMemcpyHostToDevice, ctx->stream));
     launch_weight_potential(ctx->stream, N, bf->grid->d_w.p, bf->d_v.p, g ? bf->d_g.p : nullptr, bf->d_vs.p,
                             bf->d_gw.p);
-    launch_fill(ctx->stream, bf->d_U.p, (int64_t)n * n, 0.0);
     run_fock<double>(bf, bf->d_t64.p, g ? 1 : 0, thr, ctx->lanes);
     HIPX(hipMemcpyAsync(bf->d_F.p, F, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
     launch_finalize(ctx->stream, bf->d_U.p, bf->d_F.p, n, 1);
@@ -1118,21 +1226,29 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   hipStream_t s = L.main;
   require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
   require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
-  require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+  require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32 || precision == XC_PREC_MIXED, XC_ERR_ARG,
+          "unknown precision");
   const bool gga = functional_is_gga(functional);
+  const bool mixed = precision == XC_PREC_MIXED;
+  const bool lowp = precision != XC_PREC_FP64;  // FP32 contractions (mixed: plus the FP64 refinement)
   require(!gga || bf->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
-  require(precision == XC_PREC_FP64 || bf->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
-  require(precision == XC_PREC_FP32 || bf->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
+  require(!lowp || bf->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
+  require(lowp || bf->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
+  require(!mixed || (bf->d_t64.p && bf->d_P32d.p), XC_ERR_STATE,
+          "mixed precision needs both tile sets (keep_fp32 = 1)");
   const int n = bf->basis->nao;
   const int64_t N = bf->grid->npts;
   const int nxc = (int)((N + kTile - 1) / kTile);
   if (s == ctx->stream) ctx->mark();
-  if (precision == XC_PREC_FP32) launch_to_float(s, dP, bf->d_P32.p, (int64_t)n * n);
+  if (lowp) launch_to_float(s, dP, bf->d_P32.p, (int64_t)n * n);
+  if (mixed) launch_to_double(s, bf->d_P32.p, bf->d_P32d.p, (int64_t)n * n);
   if (s == ctx->stream) ctx->mark();
-  if (precision == XC_PREC_FP64)
+  if (!lowp) {
     run_density<double>(bf, bf->d_t64.p, dP, gga ? 1 : 0, L);
-  else
+  } else {
     run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0, L);
+    if (mixed) run_density_refine(bf, gga ? 1 : 0, L);
+  }
   if (s == ctx->stream) ctx->mark();
   if (bf->bs == kTile) {
     launch_functional(s, functional, ctx->prm, N, kTile, nullptr, nullptr, bf->d_blocks.p, bf->d_part.p,
@@ -1143,16 +1259,20 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
                       nullptr, bf->grid->d_w.p, nullptr, nullptr, nullptr, bf->d_vs.p, bf->d_gw.p, bf->d_eblk.p);
   }
   if (s == ctx->stream) ctx->mark();
-  launch_fill(s, bf->d_U.p, (int64_t)n * n, 0.0);
-  if (s == ctx->stream) ctx->mark();
-  if (precision == XC_PREC_FP64)
-    run_fock<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr, L);
+  if (!lowp)
+    run_fock_tiles<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr, L);
   else
-    run_fock<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr, L);
+    run_fock_tiles<float>(bf, bf->d_t32.p, gga ? 1 : 0, thr, L);
+  if (s == ctx->stream) ctx->mark();
+  if (!lowp)
+    run_fock_reduce<double>(bf, L);
+  else
+    run_fock_reduce<float>(bf, L);
   if (s == ctx->stream) ctx->mark();
   // FP32 contractions see fl32(P): E_xc[P] = E_xc[fl32(P)] + tr(F (P - fl32(P))) to first order,
-  // the trace formed by the finalize blocks and added with the energy sum
-  const bool fp32 = precision == XC_PREC_FP32;
+  // the trace formed by the finalize blocks and added with the energy sum (the mixed form's FP64
+  // refinement also contracts fl32(P), so the same term applies to every block)
+  const bool fp32 = lowp;
   launch_finalize(s, bf->d_U.p, dF, n, 0, fp32 ? dP : nullptr, fp32 ? bf->d_P32.p : nullptr, bf->d_round.p);
   launch_sum(s, bf->d_eblk.p, nxc, d_energy, 0, fp32 ? bf->d_round.p : nullptr, fp32 ? finalize_blocks(n) : 0);
   if (s == ctx->stream) ctx->mark();
@@ -1183,6 +1303,8 @@ static void enqueue_build_pol(xc_bfogc* bf, int functional, int precision, doubl
   hipStream_t s = L.main;
   require(!bf->in_flight, XC_ERR_STATE, "a server build of this controller is in flight");
   require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
+  require(precision != XC_PREC_MIXED, XC_ERR_ARG,
+          "mixed precision is available for restricted single-system builds only");
   require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
   const bool gga = functional_is_gga(functional);
   require(!gga || bf->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
@@ -1213,7 +1335,6 @@ static void enqueue_build_pol(xc_bfogc* bf, int functional, int precision, doubl
   const double* gw[2] = {bf->d_gw.p, bf->d_gw_b.p};
   double* Fs[2] = {dFa, dFb};
   for (int sp = 0; sp < 2; ++sp) {
-    launch_fill(s, bf->d_U.p, (int64_t)n * n, 0.0);
     if (precision == XC_PREC_FP64)
       run_fock<double>(bf, bf->d_t64.p, gga ? 1 : 0, thr, L, vs[sp], gw[sp]);
     else
@@ -1235,6 +1356,8 @@ static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int func
                          const double* dP_act, const double* const* dP_env, double* dF) {
   xc_ctx* ctx = act->ctx;
   require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
+  require(precision != XC_PREC_MIXED, XC_ERR_ARG,
+          "mixed precision is available for restricted single-system builds only");
   require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
   require(nenv >= 0 && (nenv == 0 || (env && dP_env)), XC_ERR_ARG, "non-additive build: null environment list");
   const bool gga = functional_is_gga(functional);
@@ -1254,6 +1377,7 @@ static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int func
     require(b->grid == act->grid, XC_ERR_ARG,
             "non-additive build: all subsystem controllers must be defined on the same (supersystem) grid");
     require(b->bs == act->bs, XC_ERR_ARG, "non-additive build: controllers differ in block size");
+    require_idle(b, "non-additive build");
     require(!gga || b->deriv >= 1, XC_ERR_STATE, "GGA functional needs AO gradients (highest derivative 1)");
     require(!fp32 || b->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
     require(fp32 || b->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
@@ -1294,12 +1418,15 @@ static void enqueue_nadd(xc_bfogc* act, int nenv, xc_bfogc* const* env, int func
     launch_sum(s, b->d_eblk.p, nxc, ctx->d_energy, 1);
   }
   ctx->mark();
-  launch_fill(s, act->d_U.p, (int64_t)n * n, 0.0);
+  if (fp32)
+    run_fock_tiles<float>(act, act->d_t32.p, gga ? 1 : 0, thr, ctx->lanes);
+  else
+    run_fock_tiles<double>(act, act->d_t64.p, gga ? 1 : 0, thr, ctx->lanes);
   ctx->mark();
   if (fp32)
-    run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr, ctx->lanes);
+    run_fock_reduce<float>(act, ctx->lanes);
   else
-    run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr, ctx->lanes);
+    run_fock_reduce<double>(act, ctx->lanes);
   ctx->mark();
   launch_finalize(s, act->d_U.p, dF, n, 0);
   ctx->mark();
@@ -1315,6 +1442,8 @@ static void enqueue_nadd_pol(xc_bfogc* act, int nenv, xc_bfogc* const* env, int 
                              const double* const* dPb_env, double* dFa, double* dFb) {
   xc_ctx* ctx = act->ctx;
   require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
+  require(precision != XC_PREC_MIXED, XC_ERR_ARG,
+          "mixed precision is available for restricted single-system builds only");
   require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
   require(nenv >= 0 && (nenv == 0 || (env && dPa_env && dPb_env)), XC_ERR_ARG,
           "non-additive build: null environment list");
@@ -1391,7 +1520,6 @@ static void enqueue_nadd_pol(xc_bfogc* act, int nenv, xc_bfogc* const* env, int 
   const double* gw[2] = {act->d_gw.p, act->d_gw_b.p};
   double* Fs[2] = {dFa, dFb};
   for (int sp = 0; sp < 2; ++sp) {
-    launch_fill(s, act->d_U.p, (int64_t)n * n, 0.0);
     if (fp32)
       run_fock<float>(act, act->d_t32.p, g, thr, ctx->lanes, vs[sp], gw[sp]);
     else
@@ -1411,6 +1539,7 @@ int xc_nadd_fock_build_unrestricted(xc_bfogc* active, int nenv, xc_bfogc* const*
             "xc_nadd_fock_build_unrestricted: null environment list");
     xc_ctx* ctx = active->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    require_idle(active, nenv, env, "xc_nadd_fock_build_unrestricted");
     std::vector<xc_bfogc*> sub(1, active);
     for (int i = 0; i < nenv; ++i) {
       require(env[i] && Pa_env[i] && Pb_env[i], XC_ERR_ARG,
@@ -1447,13 +1576,16 @@ int xc_nadd_fock_build_unrestricted(xc_bfogc* active, int nenv, xc_bfogc* const*
 int xc_bfogc_update_density(xc_bfogc* bf, int precision, const double* dP, int with_gradient) {
   return guarded([&] {
     require(bf && dP, XC_ERR_ARG, "xc_bfogc_update_density: null argument");
-    require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+    require(precision != XC_PREC_MIXED, XC_ERR_ARG,
+          "mixed precision is available for restricted single-system builds only");
+  require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
     require(!with_gradient || bf->deriv >= 1, XC_ERR_STATE,
             "xc_bfogc_update_density: gradient requested but AO gradients not evaluated");
     require(precision == XC_PREC_FP64 || bf->d_t32.p, XC_ERR_STATE, "FP32 contractions need keep_fp32 tiles");
     require(precision == XC_PREC_FP32 || bf->d_t64.p, XC_ERR_STATE, "FP64 contractions need FP64 tiles");
     xc_ctx* ctx = bf->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    require_idle(bf, "xc_bfogc_update_density");
     const int n = bf->basis->nao;
     const int64_t N = bf->grid->npts;
     if (!bf->d_rho_c.p) {
@@ -1482,7 +1614,9 @@ int xc_nadd_fock_build_cached(xc_bfogc* act, int nenv, xc_bfogc* const* env, int
   return guarded([&] {
     require(act && dF, XC_ERR_ARG, "xc_nadd_fock_build_cached: null argument");
     require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
-    require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+    require(precision != XC_PREC_MIXED, XC_ERR_ARG,
+          "mixed precision is available for restricted single-system builds only");
+  require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
     require(nenv >= 0 && (nenv == 0 || env), XC_ERR_ARG, "non-additive build: null environment list");
     xc_ctx* ctx = act->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -1502,6 +1636,7 @@ int xc_nadd_fock_build_cached(xc_bfogc* act, int nenv, xc_bfogc* const* env, int
       require(b->ctx == ctx, XC_ERR_ARG, "non-additive build: controllers live on different contexts");
       require(b->grid == act->grid, XC_ERR_ARG,
               "non-additive build: all subsystem controllers must be defined on the same (supersystem) grid");
+      require_idle(b, "xc_nadd_fock_build_cached");
       require(b->rho_c_valid, XC_ERR_STATE,
               "xc_nadd_fock_build_cached: a subsystem has no cached density (call xc_bfogc_update_density)");
       require(!gga || b->rho_c_grad, XC_ERR_STATE,
@@ -1549,12 +1684,15 @@ int xc_nadd_fock_build_cached(xc_bfogc* act, int nenv, xc_bfogc* const* env, int
       launch_subtract_scalars(s, L, ctx->d_energy);
     }
     ctx->mark();
-    launch_fill(s, act->d_U.p, (int64_t)n * n, 0.0);
+    if (precision == XC_PREC_FP32)
+      run_fock_tiles<float>(act, act->d_t32.p, gga ? 1 : 0, thr, ctx->lanes);
+    else
+      run_fock_tiles<double>(act, act->d_t64.p, gga ? 1 : 0, thr, ctx->lanes);
     ctx->mark();
     if (precision == XC_PREC_FP32)
-      run_fock<float>(act, act->d_t32.p, gga ? 1 : 0, thr, ctx->lanes);
+      run_fock_reduce<float>(act, ctx->lanes);
     else
-      run_fock<double>(act, act->d_t64.p, gga ? 1 : 0, thr, ctx->lanes);
+      run_fock_reduce<double>(act, ctx->lanes);
     ctx->mark();
     launch_finalize(s, act->d_U.p, dF, n, 0);
     ctx->mark();
@@ -1574,6 +1712,7 @@ int xc_nadd_fock_build_device(xc_bfogc* active, int nenv, xc_bfogc* const* env, 
     require(active && dP_active && dF, XC_ERR_ARG, "xc_nadd_fock_build_device: null argument");
     xc_ctx* ctx = active->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    require_idle(active, nenv, env, "xc_nadd_fock_build_device");
     enqueue_nadd(active, nenv, env, functional, precision, thr, dP_active, dP_env, dF);
     if (energy) {
       HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1590,6 +1729,7 @@ int xc_nadd_fock_build(xc_bfogc* active, int nenv, xc_bfogc* const* env, int fun
     require(nenv >= 0 && (nenv == 0 || (env && P_env)), XC_ERR_ARG, "xc_nadd_fock_build: null environment list");
     xc_ctx* ctx = active->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    require_idle(active, nenv, env, "xc_nadd_fock_build");
     const int n = active->basis->nao;
     HIPX(hipMemcpyAsync(active->d_P.p, P_active, sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
     std::vector<const double*> dP(nenv);
@@ -1631,7 +1771,9 @@ int xc_fock_build_packed(int nsys, xc_bfogc* const* bfs, int functional, int pre
     require(ctx != nullptr, XC_ERR_ARG, "xc_fock_build_packed: null controller");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG, "unknown functional");
-    require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
+    require(precision != XC_PREC_MIXED, XC_ERR_ARG,
+          "mixed precision is available for restricted single-system builds only");
+  require(precision == XC_PREC_FP64 || precision == XC_PREC_FP32, XC_ERR_ARG, "unknown precision");
     const bool gga = functional_is_gga(functional);
     const bool fp32 = precision == XC_PREC_FP32;
     std::vector<DevBlock> blocks;
@@ -1639,6 +1781,7 @@ int xc_fock_build_packed(int nsys, xc_bfogc* const* bfs, int functional, int pre
     std::vector<WorkItem> dens[kDensMaxS + 1];
     std::vector<FockItem> fock[kFockNumS];
     int64_t maxnn = 0;
+    int maxn = 0;
     for (int i = 0; i < nsys; ++i) {
       xc_bfogc* bf = bfs[i];
       require(bf && dP[i] && dF[i], XC_ERR_ARG, "xc_fock_build_packed: null controller or matrix");
@@ -1677,6 +1820,9 @@ int xc_fock_build_packed(int nsys, xc_bfogc* const* bfs, int functional, int pre
       ps.vs = bf->d_vs.p;
       ps.gw = bf->d_gw.p;
       ps.U = bf->d_U.p;
+      ps.T = bf->d_T.p;
+      ps.ent = bf->d_rowent.p;
+      ps.row_ptr = bf->d_rowptr.p;
       ps.F = dF[i];
       ps.eblk = bf->d_eblk.p;
       ps.npts = bf->grid->npts;
@@ -1684,6 +1830,7 @@ int xc_fock_build_packed(int nsys, xc_bfogc* const* bfs, int functional, int pre
       ps.blk0 = blk0;
       ps.nxc = (int)((bf->grid->npts + kTile - 1) / kTile);
       maxnn = std::max(maxnn, (int64_t)n * n);
+      maxn = std::max(maxn, n);
     }
     // a composition seen once before runs as a captured graph; the first build is eager
     std::vector<uint64_t> key;
@@ -1705,15 +1852,19 @@ int xc_fock_build_packed(int nsys, xc_bfogc* const* bfs, int functional, int pre
       if (!ctx->pack) ctx->pack = new PackWork();
       pwp = ctx->pack;
     } else if (hit == ctx->pack_graphs.end()) {
-      if (ctx->pack_graphs.size() >= 16) {  // bounded cache: forget the oldest-keyed composition
-        delete ctx->pack_graphs.begin()->second;
-        ctx->pack_graphs.erase(ctx->pack_graphs.begin());
+      if (ctx->pack_graphs.size() >= 16) {  // bounded cache: forget the least recently used composition
+        auto lru = ctx->pack_graphs.begin();
+        for (auto it = ctx->pack_graphs.begin(); it != ctx->pack_graphs.end(); ++it)
+          if (it->second->last_use < lru->second->last_use) lru = it;
+        delete lru->second;
+        ctx->pack_graphs.erase(lru);
       }
-      ctx->pack_graphs.emplace(key, new PackWork());
+      ctx->pack_graphs.emplace(key, new PackWork()).first->second->last_use = ++ctx->pack_calls;
       if (!ctx->pack) ctx->pack = new PackWork();
       pwp = ctx->pack;
     } else {
       pwp = hit->second;
+      pwp->last_use = ++ctx->pack_calls;
       capture = pwp->exec == nullptr;
     }
     PackWork& pw = *pwp;
@@ -1748,15 +1899,18 @@ int xc_fock_build_packed(int nsys, xc_bfogc* const* bfs, int functional, int pre
                                        nullptr, 0, g, nullptr, 0, dsys);
         });
         launch_functional_packed(s, functional, ctx->prm, (int)blocks.size(), pw.blocks.p, dsys);
-        launch_pack_clear(s, dsys, nsys, maxnn);
         over_lanes(L, kFockNumS, [&](int c) { return (int)fock[c].size(); }, [&](int c, hipStream_t st) {
           if (fp32)
             launch_fock_sub<float>(st, c, pw.fock[c].p, (int)fock[c].size(), pw.blocks.p, nullptr, nullptr,
-                                   nullptr, nullptr, g, thr, nullptr, 0, 0, dsys);
+                                   nullptr, nullptr, g, thr, nullptr, 0, dsys);
           else
             launch_fock_sub<double>(st, c, pw.fock[c].p, (int)fock[c].size(), pw.blocks.p, nullptr, nullptr,
-                                    nullptr, nullptr, g, thr, nullptr, 0, 0, dsys);
+                                    nullptr, nullptr, g, thr, nullptr, 0, dsys);
         });
+        if (fp32)
+          launch_fock_reduce<float>(s, nullptr, nullptr, nullptr, nullptr, nullptr, maxn, dsys, nsys);
+        else
+          launch_fock_reduce<double>(s, nullptr, nullptr, nullptr, nullptr, nullptr, maxn, dsys, nsys);
         launch_pack_finalize(s, dsys, nsys, fp32 ? 1 : 0);
         launch_pack_sum(s, dsys, nsys, fp32 ? 1 : 0);
       };
@@ -2371,6 +2525,7 @@ int xc_server_wait(xc_server* srv, int64_t ticket) {
 int xc_server_stats(const xc_server* srv, int* busy_slots, int64_t* submitted, int64_t* graph_launches) {
   return guarded([&] {
     require(srv, XC_ERR_ARG, "xc_server_stats: null argument");
+    std::lock_guard<std::recursive_mutex> lk(srv->ctx->mu);
     if (busy_slots) {
       int b = 0;
       for (const auto& sl : srv->slots) b += sl.busy ? 1 : 0;

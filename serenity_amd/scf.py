@@ -452,7 +452,8 @@ class ScfTask:
                     wall_s=round(time.perf_counter() - t0, 3))
 
     def run(self, mode="FP64", max_cycles=100, P0=None):
-        """mode "FP64" or "MIXED"; returns energies, cycle counts and timings."""
+        """mode "FP64" or "MIXED" (iterations with the mixed-precision builds, then an FP64 finish from
+        their converged density); returns energies, cycle counts and timings."""
         if self.unrestricted:
             if mode not in ("FP64", "FP32"):
                 raise xc.XCError(1, "ScfTask: unrestricted runs take the precision FP64 or FP32")
@@ -468,10 +469,12 @@ class ScfTask:
             out.update(cycles=r["cycles"], converged=r["converged"], energy_small_grid=r["energy"])
             out["energy"], out["P_final"] = self.final_evaluation(r["P"], "FP64")
         elif mode == "MIXED":
-            r32 = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "FP32")), P0=P0,
+            # iterations with FP32 contractions (the dense blocks' density refined in FP64), the
+            # energy of their converged density with the same builds, then the FP64 finish
+            r32 = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "MIXED")), P0=P0,
                           max_cycles=max_cycles, log=self.say, **MIXED_THRESHOLDS)
             out["cycles_fp32"] = r32["cycles"]
-            out["energy_before_fp64_finish"] = self.final_energy(r32["P"], "FP32")
+            out["energy_before_fp64_finish"] = self.final_energy(r32["P"], "MIXED")
             out["P_fp32"] = r32["P"]
             r = perform(self.ints, self.nocc, gpu_xc_build(self._potential("small", "FP64")), P0=r32["P"],
                         max_cycles=max_cycles, log=self.say)

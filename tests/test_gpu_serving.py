@@ -86,7 +86,27 @@ def test_server_busy_and_errors(ctx):
     assert srv.submit(bf2, "PBE", "FP64", s["P"]) is None  # no free slot
     with pytest.raises(xc.XCError, match="in flight"):
         xc.FuncPotential(bf, "PBE").getMatrix(s["P"])
+    # every path that writes the controller's work buffers refuses it, before any copy or launch,
+    # whether it is the active subsystem or an environment one
+    n = s["basis"].nao
+    dP = ctx.alloc(8 * n * n)
+    ctx.upload(dP, np.asfortranarray(s["P"]))
+    for act, env in ((bf, bf2), (bf2, bf)):
+        nadd = xc.NAddFuncPotential(act, [env], "PBE")
+        with pytest.raises(xc.XCError, match="in flight"):
+            nadd.getMatrix(s["P"], [s["P"]])
+        with pytest.raises(xc.XCError, match="in flight"):
+            nadd.getMatrixUnrestricted(0.5 * s["P"], 0.5 * s["P"], [0.5 * s["P"]], [0.5 * s["P"]])
+    with pytest.raises(xc.XCError, match="in flight"):
+        bf.update_density(dP)
+    dF = ctx.alloc(8 * n * n)
+    with pytest.raises(xc.XCError, match="in flight"):
+        xc.NAddFuncPotential(bf, [bf2], "PBE").build_cached(dF)
+    with pytest.raises(xc.XCError, match="in flight"):
+        xc.NAddFuncPotential(bf2, [bf], "PBE").build_device(dP, [dP], dF)
     F, E = srv.wait(t)
+    ctx.free(dP)
+    ctx.free(dF)
     Fs = xc.FuncPotential(bf2, "PBE").getMatrix(s["P"])
     assert np.abs(F - Fs).max() < 1e-12
     with pytest.raises(xc.XCError, match="ticket"):
