@@ -1,22 +1,25 @@
 """XC Fock-build benchmark on one MI355X.
 
-Workload: C20H42 (all-anti, seeded jitter) / def2-SVP (490 AOs) / PBE on the
-default SSF accuracy-4 grid, with the synthetic density P = 2 C_occ C_occ^T
-(81 occupied orbitals, C_occ = S^-1/2 Q).  A step is one XC Fock build
-P -> (F, E_xc) with the grid, basis and AO tiles resident in HBM and P already
-on the device.  Both contraction precisions are timed on the same P: the
-FP32-MFMA path (the headline value) and the all-FP64 path; dE_xc and max|dF|
-of each against the other and against the CPU oracle are reported.
+Workload (default): (H2O)64 on a 4 x 4 x 4 lattice (seeded jitter and
+orientations) / def2-TZVP (2752 AOs) / PBE on the default SSF accuracy-4 grid,
+with the synthetic density P = 2 C_occ C_occ^T (320 occupied orbitals,
+C_occ = S^-1/2 Q).  A step is one XC Fock build P -> (F, E_xc) with the grid,
+basis and AO tiles resident in HBM and P already on the device.  Both
+contraction precisions are timed on the same P: the FP32-MFMA path (the
+headline value) and the all-FP64 path; dE_xc and max|dF| of each against the
+other and against the CPU oracle are reported.  The same build is also timed
+through the serving path (host P in, host F out, replayed graph).
 
-Beside the headline line the run reports (in the same JSON object) the
-embedding sweep of C6H6 + 24 H2O (non-additive PBE + TF potentials) and a
-serving stream of 2000 Poisson-arriving builds at a fixed QPS (p50/p99 latency,
-builds/s; setup of the resident systems is not timed), and the full restricted
-SCF of (H2O)8/def2-SVP/LDA with RI-J (SMALL-grid iterations, DEFAULT-grid final
-energy) in FP64 and in the mixed form (FP32 iterations + FP64 finish), beside the
-same SCF driven by the CPU oracle's XC build.
+Beside the headline line the run reports (in the same JSON object) the same
+timings for C20H42/def2-SVP/PBE, the embedding sweep of C6H6 + 24 H2O
+(non-additive PBE + TF potentials), packed builds of small systems, a serving
+stream of 2000 Poisson-arriving builds at a fixed QPS (p50/p99 latency,
+builds/s), and the full restricted SCFs with RI-J (SMALL-grid iterations,
+DEFAULT-grid final energy) of (H2O)8/def2-SVP/LDA and (H2O)64/def2-TZVP/PBE in
+FP64 and in the mixed form (FP32 iterations + FP64 finish), beside the same SCFs
+driven by the CPU oracle's XC build.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-fde] [--no-serving] [--no-scf]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--config 3] [--no-fde] [--no-serving] [--no-scf]
 """
 import argparse
 import json
@@ -24,22 +27,28 @@ import os
 import sys
 import time
 
+# OpenMP placement of the CPU baseline (read by the OpenMP runtimes when they load)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-PEAK_TFLOPS = {"FP64": 78.6, "FP32": 157.3}  # MI355X dense matrix peaks
+PEAK_TFLOPS = {"FP64": 78.6, "FP32": 157.3, "MIXED": 157.3}  # MI355X dense matrix peaks (MIXED: its FP32 part)
 PEAK_HBM_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "xc_fock_build_c20h42_svp_pbe_hbm_traffic.md")
+TRAFFIC_PROFILE = {2: os.path.join(HERE, "profiles", "xc_fock_build_c20h42_svp_pbe_hbm_traffic.md"),
+                   3: os.path.join(HERE, "profiles", "xc_fock_build_h2o64_tzvp_pbe_hbm_traffic.md")}
 
 
-def measured_traffic():
+def measured_traffic(config):
     """HBM bytes per build of each contraction stage, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
     passes summarised in the profile (lines "traffic <stage> <precision> <bytes>"), or {}."""
     out = {}
-    if os.path.exists(TRAFFIC_PROFILE):
-        for line in open(TRAFFIC_PROFILE):
+    path = TRAFFIC_PROFILE.get(config)
+    if path and os.path.exists(path):
+        for line in open(path):
             p = line.split()
             if len(p) == 4 and p[0] == "traffic":
                 out[(p[1], p[2])] = float(p[3])
@@ -68,7 +77,18 @@ def time_path(ctx, pot, dP, dF, steps, warmup):
     return (t1 - t0) * 1e3 / steps, {k: v / steps for k, v in stages.items()}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sysd, min_seconds):
+    """Full oracle builds of the workload until min_seconds have passed (at least one)."""
     from oracle import oracle as O
     b, g, P, func = sysd["basis"], sysd["grid"], sysd["P"], sysd["cfg"]["functional"]
     reps, t0 = 0, time.perf_counter()
@@ -80,6 +100,55 @@ def cpu_baseline(sysd, min_seconds):
         if el >= min_seconds or reps >= 50:
             break
     return dict(seconds=el, reps=reps, F=F, E=E, threads=O.max_threads())
+
+
+def served_build(ctx, bf, functional, P, reps=10):
+    """The same build through the serving path: host P in, host F and E_xc out, one slot, the
+    controller's build replayed from its captured graph (first submission captures it)."""
+    from serenity_amd import xc
+    out = {}
+    srv = xc.Server(ctx, nslots=1, use_graphs=True)
+    try:
+        for prec in ("FP64", "FP32", "MIXED"):
+            srv.wait(srv.submit(bf, functional, prec, P))
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                F, E = srv.wait(srv.submit(bf, functional, prec, P))
+                ts.append(time.perf_counter() - t0)
+            ms = 1e3 * float(np.median(ts))
+            out[prec.lower()] = dict(ms_per_build_median=round(ms, 4), builds_per_s=round(1e3 / ms, 2), E=E)
+    finally:
+        srv._release()
+    return out
+
+
+def time_config(ctx, config, steps, warmup):
+    """FP64 and FP32 builds of another benchmark configuration on its synthetic P."""
+    from serenity_amd import xc, workloads as W
+    sysd = W.build_system(config)
+    b, g, P, cfg = sysd["basis"], sysd["grid"], sysd["P"], sysd["cfg"]
+    N, n = g.getNGridPoints(), b.nao
+    bf = xc.BasisFunctionOnGridController(ctx, b, g, 128, 1e-9, 1, keep_fp32=True)
+    dP, dF = ctx.alloc(8 * n * n), ctx.alloc(8 * n * n)
+    ctx.upload(dP, np.asfortranarray(P).ravel(order="F"))
+    res = {}
+    for prec in ("FP64", "FP32"):
+        pot = xc.FuncPotential(bf, cfg["functional"], prec)
+        ms, stages = time_path(ctx, pot, dP, dF, steps, warmup)
+        E = pot.build_device(dP, dF, sync=True)
+        F = np.zeros(n * n)
+        ctx.download(F, dF)
+        res[prec] = (ms, stages, E, F)
+    ctx.free(dP)
+    ctx.free(dF)
+    bf._release()
+    (m64, s64, e64, f64), (m32, s32, e32, f32) = res["FP64"], res["FP32"]
+    return dict(workload=cfg["name"], npoints=N, nao=n,
+                fp64=dict(ms_per_build=round(m64, 5), grid_pts_per_s=N / (m64 * 1e-3), stages_ms=s64),
+                fp32_mfma=dict(ms_per_build=round(m32, 5), grid_pts_per_s=N / (m32 * 1e-3), stages_ms=s32),
+                speedup_fp32_over_fp64=m64 / m32, dE_xc_fp32_vs_fp64=abs(e32 - e64),
+                max_dF_fp32_vs_fp64=float(np.abs(f32 - f64).max()))
 
 
 def fde_sweep(ctx, steps, warmup, check_oracle):
@@ -279,17 +348,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--config", type=int, default=3, choices=(1, 2, 3))
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fde", action="store_true", help="skip the embedding (non-additive potential) sweep")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-scf", action="store_true", help="skip the full SCFs of configs 1 and 3")
     ap.add_argument("--no-packed", action="store_true", help="skip the packed-build comparison of small systems")
-    ap.add_argument("--scf3", action="store_true",
-                    help="also run the full SCF of (H2O)64/def2-TZVP/PBE (FP64 and mixed; a few minutes)")
-    ap.add_argument("--scf3-oracle", action="store_true",
-                    help="with --scf3: also the SCF driven by the CPU oracle's XC build (a few more minutes)")
+    ap.add_argument("--no-extra-config", action="store_true",
+                    help="skip the timing of the second configuration (C20H42/def2-SVP/PBE beside config 3)")
+    ap.add_argument("--no-served", action="store_true", help="skip the build timed through the serving path")
+    ap.add_argument("--no-scf3", action="store_true",
+                    help="skip the full SCF of (H2O)64/def2-TZVP/PBE (FP64 and mixed; about a minute)")
+    ap.add_argument("--no-scf3-oracle", action="store_true",
+                    help="skip the (H2O)64 SCF driven by the CPU oracle's XC build (about 90 s on 16 threads)")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--serving-qps", type=float, default=400.0)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -329,7 +401,7 @@ def main():
     dP, dF = ctx.alloc(8 * n * n), ctx.alloc(8 * n * n)
     ctx.upload(dP, np.asfortranarray(P).ravel(order="F"))
     results = {}
-    for prec in ("FP64", "FP32"):
+    for prec in ("FP64", "FP32", "MIXED"):
         pot = xc.FuncPotential(bf, cfg["functional"], prec)
         ms, stages = time_path(ctx, pot, dP, dF, args.steps, args.warmup)
         E = pot.build_device(dP, dF, sync=True)
@@ -344,7 +416,7 @@ def main():
     # algorithmic work: 2 sum_b B_b n_b^2 flops per contraction (phi.P and phi^T.G)
     flops_per_kernel = 2.0 * sum_Bnb2
 
-    traffic = measured_traffic() if args.config == 2 else {}
+    traffic = measured_traffic(args.config)
 
     def roofline(prec):
         st = results[prec]["stages"]
@@ -356,16 +428,23 @@ def main():
                     hbm_gbs=round(tb / (st[kern] * 1e-3) / 1e9, 1) if tb else None,
                     flops_per_launch=flops_per_kernel, avg_launch_ms=round(st[kern], 5))
 
-    r64, r32 = results["FP64"], results["FP32"]
+    r64, r32, rmx = results["FP64"], results["FP32"], results["MIXED"]
     out_extra = dict(
+        dE_xc_mixed_vs_fp64=abs(rmx["E"] - r64["E"]),
+        max_dF_mixed_vs_fp64=float(np.abs(rmx["F"] - r64["F"]).max()),
         dE_xc_fp32_vs_fp64=abs(r32["E"] - r64["E"]),
         max_dF_fp32_vs_fp64=float(np.abs(r32["F"] - r64["F"]).max()),
         E_xc_fp64=r64["E"],
         fp64=dict(ms_per_build=round(r64["ms"], 5), grid_pts_per_s=N / (r64["ms"] * 1e-3),
                   builds_per_s=1e3 / r64["ms"], stages_ms=r64["stages"], roofline=roofline("FP64")),
         fp32_mfma=dict(ms_per_build=round(r32["ms"], 5), grid_pts_per_s=N / (r32["ms"] * 1e-3),
-                       builds_per_s=1e3 / r32["ms"], stages_ms=r32["stages"]),
+                       builds_per_s=1e3 / r32["ms"], stages_ms=r32["stages"], roofline=roofline("FP32")),
+        mixed=dict(ms_per_build=round(rmx["ms"], 5), grid_pts_per_s=N / (rmx["ms"] * 1e-3),
+                   builds_per_s=1e3 / rmx["ms"], stages_ms=rmx["stages"],
+                   form="FP32-MFMA contractions; the density of the blocks whose largest density exceeds 2 bohr^-3 "
+                        "contracted again in FP64; FP64 functional, reductions and F accumulation"),
         speedup_fp32_over_fp64=r64["ms"] / r32["ms"],
+        speedup_mixed_over_fp64=r64["ms"] / rmx["ms"],
         ao_setup_s=round(t_ao, 3),
     )
 
@@ -375,15 +454,39 @@ def main():
         cb = cpu_baseline(sysd, args.cpu_seconds)
         pts_s = N * cb["reps"] / cb["seconds"]
         cpu = dict(value=pts_s, unit="grid pts/s", cores=cb["threads"], kind="port",
-                   sample="%d full XC Fock builds of the same workload (%.1f s, OpenMP, FP64, built-in loops, "
-                          "no BLAS)" % (cb["reps"], cb["seconds"]),
-                   builds_per_s=cb["reps"] / cb["seconds"])
+                   sample="%d full XC Fock builds of the same workload (%.1f s; CPU restatement of Serenity's "
+                          "per-block path, OpenMP schedule(dynamic), FP64, register-blocked built-in loops, no BLAS)"
+                          % (cb["reps"], cb["seconds"]),
+                   builds_per_s=cb["reps"] / cb["seconds"], cpu_model=cpu_model(),
+                   omp=dict(OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS"),
+                            OMP_PROC_BIND=os.environ.get("OMP_PROC_BIND"),
+                            OMP_PLACES=os.environ.get("OMP_PLACES")))
         out_extra["dE_xc_fp64_vs_oracle"] = abs(r64["E"] - cb["E"])
         out_extra["max_dF_fp64_vs_oracle"] = float(np.abs(r64["F"] - cb["F"]).max())
         out_extra["dE_xc_fp32_vs_oracle"] = abs(r32["E"] - cb["E"])
         out_extra["max_dF_fp32_vs_oracle"] = float(np.abs(r32["F"] - cb["F"]).max())
+        out_extra["dE_xc_mixed_vs_oracle"] = abs(rmx["E"] - cb["E"])
         log("oracle: %.3f builds/s on %d threads; FP64 |dE| %.2e max|dF| %.2e" %
             (cpu["builds_per_s"], cb["threads"], out_extra["dE_xc_fp64_vs_oracle"], out_extra["max_dF_fp64_vs_oracle"]))
+
+    if not args.no_served:
+        sv1 = served_build(ctx, bf, cfg["functional"], P)
+        for k in ("fp64", "fp32", "mixed"):
+            sv1[k]["dE_vs_device_build"] = abs(sv1[k].pop("E") - results[k.upper()]["E"])
+        sv1["note"] = ("one slot; each build copies P host->device and F device->host (%.1f MB each way) "
+                       "around the replayed graph" % (8.0 * n * n / 1e6))
+        log("served build: FP64 %.3f ms, FP32 %.3f ms, mixed %.3f ms (median, host P/F)" %
+            (sv1["fp64"]["ms_per_build_median"], sv1["fp32"]["ms_per_build_median"],
+             sv1["mixed"]["ms_per_build_median"]))
+        out_extra["served_build"] = sv1
+
+    if not args.no_extra_config:
+        other = 2 if args.config == 3 else 3
+        ex = time_config(ctx, other, args.steps, args.warmup)
+        log("%s: FP64 %.3f ms, FP32 %.3f ms (%.2fx), dE %.2e" %
+            (ex["workload"], ex["fp64"]["ms_per_build"], ex["fp32_mfma"]["ms_per_build"],
+             ex["speedup_fp32_over_fp64"], ex["dE_xc_fp32_vs_fp64"]))
+        out_extra["config%d" % other] = ex
 
     if not args.no_fde:
         fde = fde_sweep(ctx, max(1, args.steps // 4), 1, not args.no_cpu_baseline)
@@ -416,8 +519,9 @@ def main():
              "" if "cpu_oracle" not in sc else "; CPU oracle SCF %.2f s, |dE| %.2e" %
              (sc["cpu_oracle"]["wall_s"], sc["fp64"]["dE_vs_cpu_oracle"])))
         out_extra["scf"] = sc
-    if args.scf3 and not args.no_scf:
-        sc3 = scf_leg(ctx, args.scf3_oracle, config=3, cycle_log=lambda *a: log("  scf3", *a))
+    if not args.no_scf3 and not args.no_scf:
+        sc3 = scf_leg(ctx, not (args.no_scf3_oracle or args.no_cpu_baseline), config=3,
+                      cycle_log=lambda *a: log("  scf3", *a))
         log("SCF (H2O)64/TZVP/PBE: FP64 E %.10f (%d cycles, %.1f s); mixed before finish dE %.2e, after %.2e "
             "(%d+%d cycles, %.1f s)" %
             (sc3["fp64"]["energy"], sc3["fp64"]["cycles"], sc3["fp64"]["wall_s"],
@@ -453,13 +557,13 @@ def main():
         out_extra["serving"] = sv
 
     line = dict(
-        metric="XC Fock-build grid pts/s (FP32-MFMA contractions, FP64 functional; FP64 path and dE stated)",
-        value=N / (r32["ms"] * 1e-3),
+        metric="XC Fock-build grid pts/s, FP32-MFMA vs FP64 at stated dE; p50/p99 latency @QPS",
+        value=N / (rmx["ms"] * 1e-3),
         unit="grid pts/s",
         n_gpus=1,
         steps=args.steps,
         warmup=args.warmup,
-        ms_per_step=round(r32["ms"], 5),
+        ms_per_step=round(rmx["ms"], 5),
         higher_is_better=True,
         scaling="weak",
         vs_baseline=None,
@@ -468,7 +572,7 @@ def main():
         config=dict(workload=cfg["name"] + ", SSF/Ahlrichs/Lebedev accuracy-4 grid, seeded geometry, "
                     "synthetic P (P = 2 C C^T, C = S^-1/2 Q)", npoints=N, nao=n, nblocks=nblocks,
                     mean_nb=round(sum_nb / nblocks, 2), seed=W.SEED, parallelism="single GPU"),
-        roofline=roofline("FP32"),
+        roofline=roofline("MIXED"),
         cpu_baseline=cpu,
     )
     line.update(out_extra)

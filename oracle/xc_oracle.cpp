@@ -551,6 +551,79 @@ FParams params_from(const double* p) {
   return P;
 }
 
+// The two dense block contractions, register-blocked.  Every output element is
+// summed in the same order as the plain triple loop (k ascending for BP, p
+// ascending for T), so the blocking changes speed only, not results.
+//
+// BP[j][p] += sum_k phi[sig[k]][p] Ps[k][j]   (phi AO-major with B points per AO, BP j-major)
+void phi_times_p(const double* phi, const std::vector<int>& sig, int B, const double* Ps, int m, double* BP) {
+  constexpr int JB = 4, PB = 8;
+  int j0 = 0;
+  for (; j0 + JB <= m; j0 += JB) {
+    int p0 = 0;
+    for (; p0 + PB <= B; p0 += PB) {
+      double acc[JB][PB];
+      for (int u = 0; u < JB; ++u)
+        for (int q = 0; q < PB; ++q) acc[u][q] = BP[(size_t)(j0 + u) * B + p0 + q];
+      for (int k = 0; k < m; ++k) {
+        const double* ph = phi + (size_t)sig[k] * B + p0;
+        for (int u = 0; u < JB; ++u) {
+          const double pk = Ps[k + (size_t)m * (j0 + u)];
+          for (int q = 0; q < PB; ++q) acc[u][q] += ph[q] * pk;
+        }
+      }
+      for (int u = 0; u < JB; ++u)
+        for (int q = 0; q < PB; ++q) BP[(size_t)(j0 + u) * B + p0 + q] = acc[u][q];
+    }
+    for (int u = 0; u < JB; ++u)
+      for (int k = 0; k < m; ++k) {
+        const double pk = Ps[k + (size_t)m * (j0 + u)];
+        const double* ph = phi + (size_t)sig[k] * B;
+        double* bp = BP + (size_t)(j0 + u) * B;
+        for (int p = p0; p < B; ++p) bp[p] += ph[p] * pk;
+      }
+  }
+  for (int j = j0; j < m; ++j)
+    for (int k = 0; k < m; ++k) {
+      const double pk = Ps[k + (size_t)m * j];
+      const double* ph = phi + (size_t)sig[k] * B;
+      double* bp = BP + (size_t)j * B;
+      for (int p = 0; p < B; ++p) bp[p] += ph[p] * pk;
+    }
+}
+
+// T[i][j] = sum_p phi[sig[i]][p] G[j][p]   (G j-major with B points per column; T i-major m x m)
+void phi_t_times_g(const double* phi, const std::vector<int>& sig, int B, const double* G, int m, double* T) {
+  constexpr int IB = 4, JB = 8;
+  std::vector<double> Gt((size_t)B * m);  // point-major copy of G
+  for (int j = 0; j < m; ++j)
+    for (int p = 0; p < B; ++p) Gt[(size_t)p * m + j] = G[(size_t)j * B + p];
+  for (int i0 = 0; i0 < m; i0 += IB) {
+    const int ni = std::min(IB, m - i0);
+    for (int j0 = 0; j0 < m; j0 += JB) {
+      const int nj = std::min(JB, m - j0);
+      double acc[IB][JB] = {};
+      if (ni == IB && nj == JB) {
+        for (int p = 0; p < B; ++p) {
+          const double* g = &Gt[(size_t)p * m + j0];
+          for (int u = 0; u < IB; ++u) {
+            const double f = phi[(size_t)sig[i0 + u] * B + p];
+            for (int v = 0; v < JB; ++v) acc[u][v] += f * g[v];
+          }
+        }
+      } else {
+        for (int p = 0; p < B; ++p)
+          for (int u = 0; u < ni; ++u) {
+            const double f = phi[(size_t)sig[i0 + u] * B + p];
+            for (int v = 0; v < nj; ++v) acc[u][v] += f * Gt[(size_t)p * m + j0 + v];
+          }
+      }
+      for (int u = 0; u < ni; ++u)
+        for (int v = 0; v < nj; ++v) T[(size_t)(i0 + u) * m + j0 + v] = acc[u][v];
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -610,13 +683,7 @@ int oracle_density(int nshell, const int* l, const int* pure, const int* nprim, 
     std::vector<double> Ps((size_t)m * m), BP((size_t)B * m, 0.0);
     for (int j = 0; j < m; ++j)
       for (int i = 0; i < m; ++i) Ps[i + (size_t)m * j] = P[sig[i] + (size_t)n * sig[j]];
-    for (int j = 0; j < m; ++j)
-      for (int k = 0; k < m; ++k) {
-        const double pkj = Ps[k + (size_t)m * j];
-        const double* phik = &ao.val[(size_t)sig[k] * B];
-        double* bpj = &BP[(size_t)j * B];
-        for (int p = 0; p < B; ++p) bpj[p] += phik[p] * pkj;
-      }
+    phi_times_p(ao.val.data(), sig, B, Ps.data(), m, BP.data());
     std::vector<double> GX, GY, GZ;  // grad phi . Psub for the Hessian
     if (hess) {
       GX.assign((size_t)B * m, 0.0);
@@ -774,12 +841,11 @@ int oracle_fock_add(int nshell, const int* l, const int* pure, const int* nprim,
         G[(size_t)j * B + p] = gv;
       }
     // T = phi^T G (m x m); F[sig,sig] += T + T^T
+    std::vector<double> T((size_t)m * m);
+    phi_t_times_g(aoB.val.data(), sig, B, G.data(), m, T.data());
     for (int j = 0; j < m; ++j)
       for (int i = 0; i < m; ++i) {
-        const double* pi = &aoB.val[(size_t)sig[i] * B];
-        const double* gj = &G[(size_t)j * B];
-        double t = 0.0;
-        for (int p = 0; p < B; ++p) t += pi[p] * gj[p];
+        const double t = T[(size_t)i * m + j];
         Fm[sig[i] + (size_t)n * sig[j]] += t;
         Fm[sig[j] + (size_t)n * sig[i]] += t;
       }

@@ -81,11 +81,18 @@ class ResidentSystems:
         self.lru = collections.OrderedDict()  # key -> (bfogc, footprint bytes)
         self.bytes = 0
         self.created = self.evicted = 0
+        self.last_created = False  # whether the last get() had to build the controller
+
+    def fits(self, key):
+        """Whether a controller for key could be admitted without evicting another."""
+        return key in self.lru or self.bytes < self.budget
 
     def get(self, key, busy):
+        self.last_created = False
         if key in self.lru:
             self.lru.move_to_end(key)
             return self.lru[key][0]
+        self.last_created = True
         d = self.host.get(key)
         before = self.ctx.memory_in_use()
         bf = xc.BasisFunctionOnGridController(self.ctx, d["basis"], d["grid"], 128, 1e-9, 1, keep_fp32=self.tiles)
@@ -122,7 +129,10 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
     say("host setup of %d systems: %.1f s" % (len(keys), t_host))
     t0 = time.perf_counter()
     if prewarm:
+        # admit systems in key order while they fit the budget (nothing is evicted before the stream)
         for k in keys:
+            if not res.fits(k):
+                break
             res.get(k, ())
         ctx.synchronize()
     t_dev = time.perf_counter() - t0
@@ -130,7 +140,7 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
     srv = xc.Server(ctx, nslots=nslots, use_graphs=use_graphs)
     # capture every resident system's build once (first-call graph capture is setup, not serving)
     if prewarm:
-        for k in keys:
+        for k in list(res.lru):
             d = host.get(k)
             for f in {f for _, kk, f in stream if kk == k}:
                 try:
@@ -146,6 +156,7 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
     busy = set()
     energies = {}
     failed = {}     # request -> error message: a failing request ends alone, the others go on
+    cold = set()    # requests whose system had to be (re)built on the device when they were served
     nopack = set()  # requests of a pack that failed: served one by one
     packbuf = {}  # key -> (dP, dF) device matrices of systems built in packs
     npacked = npacks = 0
@@ -163,6 +174,8 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
                 ctx.upload(dP, np.asfortranarray(d["P"]))
                 packbuf[key] = (dP, dF)
             pots.append(xc.FuncPotential(res.get(key, keep), func, precision))
+            if res.last_created:
+                cold.add(j)
             dPs.append(packbuf[key][0])
             dFs.append(packbuf[key][1])
         E = xc.fock_build_packed(pots, dPs, dFs)
@@ -210,6 +223,8 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
             if key in busy:
                 continue
             bf = res.get(key, busy)
+            if res.last_created:
+                cold.add(j)
             try:
                 t = srv.submit(bf, func, precision, host.get(key)["P"])
             except xc.XCError as e:
@@ -234,7 +249,13 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
                 nxt += 1
     wall = time.perf_counter() - t0
     ok = np.array([j not in failed for j in range(n)])
-    lat = ((done_at - arrive) * 1e3)[ok]
+    lat_all = (done_at - arrive) * 1e3
+    lat = lat_all[ok]
+    is_cold = np.array([j in cold for j in range(n)])
+    lat_cold, lat_warm = lat_all[ok & is_cold], lat_all[ok & ~is_cold]
+
+    def pct(x, q):
+        return round(float(np.percentile(x, q)), 3) if len(x) else None
     st = srv.stats()
     srv._release()
     for dP, dF in packbuf.values():
@@ -246,25 +267,35 @@ def run_stream(ctx, stream, nslots=4, budget_gb=200.0, prewarm=True, precision="
                 p50_ms=round(float(np.percentile(lat, 50)), 3), p99_ms=round(float(np.percentile(lat, 99)), 3),
                 mean_ms=round(float(lat.mean()), 3), max_ms=round(float(lat.max()), 3),
                 builds_per_s=round(n / wall, 2), wall_s=round(wall, 3), distinct_systems=len(keys),
-                resident_gb=round(res.bytes / 1e9, 2), created=res.created, evicted=res.evicted,
+                budget_gb=budget_gb, resident_gb=round(res.bytes / 1e9, 2), created=res.created, evicted=res.evicted,
+                cold_requests=int(is_cold[ok].sum()), cold_p50_ms=pct(lat_cold, 50), cold_p99_ms=pct(lat_cold, 99),
+                warm_p50_ms=pct(lat_warm, 50), warm_p99_ms=pct(lat_warm, 99),
                 graph_launches=st["graph_launches"], host_setup_s=round(t_host, 1), device_setup_s=round(t_dev, 1),
                 pack_max_nao=pack_nao, packed_requests=npacked, packed_launches=npacks, failed=len(failed),
                 _failed=failed,
                 _energies=energies, _host=host)
 
 
-def fp64_check(ctx, host, energies, sample, seed=W.SEED):
-    """Max |dE_xc| of the served FP32 builds against FP64 builds on the GPU, over a seeded sample."""
+def fp64_check(ctx, host, energies, sample=None, seed=W.SEED):
+    """Max |dE_xc| of the served FP32 builds against FP64 builds on the GPU: over every distinct
+    (system, functional) served (sample None), or over a seeded sample of them."""
     keys = sorted(energies)
-    rng = np.random.default_rng(seed + 11)
-    pick = [keys[j] for j in rng.choice(len(keys), size=min(sample, len(keys)), replace=False)]
+    if sample is None or sample >= len(keys):
+        pick = keys
+    else:
+        rng = np.random.default_rng(seed + 11)
+        pick = [keys[j] for j in rng.choice(len(keys), size=sample, replace=False)]
     worst = 0.0
+    by_system = collections.defaultdict(list)
     for key, func in pick:
+        by_system[key].append(func)
+    for key, funcs in by_system.items():
         d = host.get(key)
         bf = xc.BasisFunctionOnGridController(ctx, d["basis"], d["grid"], 128, 1e-9, 1)
-        pot = xc.FuncPotential(bf, func, "FP64")
-        pot.getMatrix(d["P"])
-        worst = max(worst, abs(pot.getEnergy() - energies[(key, func)]))
+        for func in funcs:
+            pot = xc.FuncPotential(bf, func, "FP64")
+            pot.getMatrix(d["P"])
+            worst = max(worst, abs(pot.getEnergy() - energies[(key, func)]))
         bf._release()
     return worst, len(pick)
 
