@@ -7,9 +7,20 @@
  *
  *   serenity_amd::BasisFunctionOnGridController   BasisFunctionOnGridController.h:105-131
  *   serenity_amd::DensityOnGridCalculator         DensityOnGridCalculator.h:45-134
+ *   serenity_amd::DensityMatrixController         DensityMatrixController.h (setDensityMatrix,
+ *                                                 getDensityMatrix, notifying its dependents)
+ *   serenity_amd::DensityMatrixDensityOnGridController
+ *                                                 DensityMatrixDensityOnGridController.h:45-120
  *   serenity_amd::ScalarOperatorToMatrixAdder     ScalarOperatorToMatrixAdder.h:48-196
  *   serenity_amd::FunctionalLibrary               FunctionalLibrary.h:51-64
+ *   serenity_amd::SystemController                the parts of SystemController.h the XC path reads
+ *                                                 (geometry, basis, settings.dft.functional)
  *   serenity_amd::FuncPotential                   FuncPotential.h:48-145
+ *
+ * The reference's forms are there (FuncPotential(system, dMat, grid, functional) with
+ * getMatrix(), getEnergy(P) and notify(); FunctionalLibrary::calcData(type, functional,
+ * densityOnGridController, order)); the explicit forms that take P as an argument are kept beside
+ * them for callers without a density-matrix controller.
  *
  * Matrices are column-major (Eigen's default): element (i, j) of an n x n
  * matrix at data[i + n * j].  Grid points are 3 x N column-major.  Every
@@ -52,7 +63,35 @@ struct Matrix {
 };
 
 enum class Functional { LDA = XC_FUNC_LDA, PBE = XC_FUNC_PBE, TF = XC_FUNC_TF, PBE_LIBXC = XC_FUNC_PBE_LIBXC };
-enum class Precision { FP64 = XC_PREC_FP64, FP32 = XC_PREC_FP32 };
+enum class Precision { FP64 = XC_PREC_FP64, FP32 = XC_PREC_FP32, MIXED = XC_PREC_MIXED };
+/* PartialDerivatives.h:931: derivatives with respect to the gradient invariant sigma = |grad rho|^2,
+ * to the gradient components, or the potential alone. */
+enum class FUNCTIONAL_DATA_TYPE { GRADIENT_INVARIANTS = 0, GRADIENTS = 1, POTENTIAL = 2 };
+
+/* ObjectSensitiveClass / NotifyingClass: lazily evaluated objects register with what they depend on
+ * and are told when it changes. */
+class ObjectSensitiveClass {
+ public:
+  virtual ~ObjectSensitiveClass() = default;
+  virtual void notify() = 0;
+};
+
+class NotifyingClass {
+ public:
+  void addSensitiveObject(std::weak_ptr<ObjectSensitiveClass> o) { _objects.push_back(std::move(o)); }
+  void notifyObjects() {
+    std::vector<std::weak_ptr<ObjectSensitiveClass>> alive;
+    for (auto& w : _objects)
+      if (auto o = w.lock()) {
+        o->notify();
+        alive.push_back(w);
+      }
+    _objects.swap(alive);
+  }
+
+ private:
+  std::vector<std::weak_ptr<ObjectSensitiveClass>> _objects;
+};
 
 /* One HIP device and its stream. */
 class Context {
@@ -241,22 +280,106 @@ class DensityOnGridCalculator {
   std::vector<int> _nonNegligible;
 };
 
-/* FunctionalData of calcData(GRADIENTS, ...): epuv, dFdRho, dFdGradRho (3 x N), energy. */
+/* DensityMatrixController (restricted): holds the density matrix and tells its dependents when it
+ * is replaced (setDensityMatrix -> notifyObjects). */
+class DensityMatrixController : public NotifyingClass {
+ public:
+  explicit DensityMatrixController(Matrix P) : _P(std::move(P)) {}
+  const Matrix& getDensityMatrix() const { return _P; }
+  void setDensityMatrix(Matrix P) {
+    _P = std::move(P);
+    notifyObjects();
+  }
+
+ private:
+  Matrix _P;
+};
+
+/* DensityMatrixDensityOnGridController: rho and grad rho of the controller's density matrix on the
+ * grid, computed on first use and kept until the density matrix changes. */
+class DensityMatrixDensityOnGridController : public ObjectSensitiveClass,
+                                             public std::enable_shared_from_this<DensityMatrixDensityOnGridController> {
+ public:
+  static std::shared_ptr<DensityMatrixDensityOnGridController> produce(
+      std::shared_ptr<DensityOnGridCalculator> calc, std::shared_ptr<DensityMatrixController> dMat) {
+    auto d = std::shared_ptr<DensityMatrixDensityOnGridController>(
+        new DensityMatrixDensityOnGridController(std::move(calc), dMat));
+    dMat->addSensitiveObject(d);
+    return d;
+  }
+  const std::vector<double>& getDensityOnGrid() {
+    update();
+    return _rho;
+  }
+  const std::vector<double>& getDensityGradientOnGrid() {
+    update();
+    return _grad;
+  }
+  std::shared_ptr<BasisFunctionOnGridController> getBasisFunctionOnGridController() const {
+    return _calc->getBasisFunctionOnGridController();
+  }
+  void notify() override { _valid = false; }
+
+ private:
+  DensityMatrixDensityOnGridController(std::shared_ptr<DensityOnGridCalculator> calc,
+                                       std::shared_ptr<DensityMatrixController> dMat)
+      : _calc(std::move(calc)), _dMat(std::move(dMat)) {}
+  void update() {
+    if (_valid) return;
+    _calc->calcDensityAndGradientOnGrid(_dMat->getDensityMatrix(), _rho, _grad);
+    _valid = true;
+  }
+  std::shared_ptr<DensityOnGridCalculator> _calc;
+  std::shared_ptr<DensityMatrixController> _dMat;
+  std::vector<double> _rho, _grad;
+  bool _valid = false;
+};
+
+/* FunctionalData of calcData: epuv, dFdRho, and dFdGradRho (3 x N, GRADIENTS) or dFdSigma (N,
+ * GRADIENT_INVARIANTS), energy. */
 struct FunctionalData {
-  std::vector<double> epuv, dFdRho, dFdGradRho;
+  std::vector<double> epuv, dFdRho, dFdGradRho, dFdSigma;
   double energy = 0.0;
 };
 
 class FunctionalLibrary {
  public:
   FunctionalLibrary(std::shared_ptr<Context> ctx, unsigned int maxBlockSize) : _ctx(std::move(ctx)), _bs(maxBlockSize) {}
+  /* calcData(type, functional, densityOnGridController, order) (FunctionalLibrary.h:61-63): the
+   * functional on the controller's (cached) density, with the weights of its grid.  order 1. */
+  FunctionalData calcData(FUNCTIONAL_DATA_TYPE type, Functional f,
+                          std::shared_ptr<DensityMatrixDensityOnGridController> densityOnGridController,
+                          unsigned int order = 1) {
+    if (order != 1)
+      throw SerenityError(XC_ERR_ARG, "FunctionalLibrary::calcData: derivatives of order 1 only");
+    if (type == FUNCTIONAL_DATA_TYPE::POTENTIAL)
+      throw SerenityError(XC_ERR_ARG, "FunctionalLibrary::calcData: POTENTIAL data are not available");
+    const auto& rho = densityOnGridController->getDensityOnGrid();
+    const auto& grad = densityOnGridController->getDensityGradientOnGrid();
+    const auto& w = densityOnGridController->getBasisFunctionOnGridController()->getGridController()->getWeights();
+    FunctionalData d = calcData(f, rho, &grad, w);
+    if (type == FUNCTIONAL_DATA_TYPE::GRADIENT_INVARIANTS && !d.dFdGradRho.empty()) {
+      // d eps / d grad rho = 2 (d eps / d sigma) grad rho
+      const size_t N = rho.size();
+      d.dFdSigma.assign(N, 0.0);
+      for (size_t p = 0; p < N; ++p) {
+        const double gx = grad[p], gy = grad[N + p], gz = grad[2 * N + p];
+        const double s = gx * gx + gy * gy + gz * gz;
+        if (s > 0.0)
+          d.dFdSigma[p] = (d.dFdGradRho[p] * gx + d.dFdGradRho[N + p] * gy + d.dFdGradRho[2 * N + p] * gz) / (2.0 * s);
+      }
+      d.dFdGradRho.clear();
+    }
+    return d;
+  }
+  /* The explicit form on given arrays (GRADIENTS data). */
   FunctionalData calcData(Functional f, const std::vector<double>& rho, const std::vector<double>* grad,
                           const std::vector<double>& weights) {
     FunctionalData d;
     const int64_t N = (int64_t)rho.size();
     d.epuv.resize(N);
     d.dFdRho.resize(N);
-    const bool gga = f == Functional::PBE;
+    const bool gga = f == Functional::PBE || f == Functional::PBE_LIBXC;
     if (gga) d.dFdGradRho.resize(3 * N);
     check(xc_functional(_ctx->get(), (int)f, N, (int)_bs, rho.data(), grad ? grad->data() : nullptr, weights.data(),
                         d.epuv.data(), d.dFdRho.data(), gga ? d.dFdGradRho.data() : nullptr, &d.energy));
@@ -266,6 +389,27 @@ class FunctionalLibrary {
  private:
   std::shared_ptr<Context> _ctx;
   unsigned int _bs;
+};
+
+/* The parts of Serenity's SystemController the XC path reads: the context, the geometry (atomic
+ * numbers, coordinates in bohr), the basis and the functional of the system's DFT settings. */
+class SystemController {
+ public:
+  SystemController(std::shared_ptr<Context> ctx, std::vector<int> Z, std::vector<double> xyz,
+                   std::shared_ptr<BasisController> basis, Functional functional = Functional::PBE)
+      : _ctx(std::move(ctx)), _Z(std::move(Z)), _xyz(std::move(xyz)), _basis(std::move(basis)), _f(functional) {}
+  std::shared_ptr<Context> getContext() const { return _ctx; }
+  const std::vector<int>& getAtomicNumbers() const { return _Z; }
+  const std::vector<double>& getCoordinates() const { return _xyz; }
+  std::shared_ptr<BasisController> getBasisController() const { return _basis; }
+  Functional getFunctional() const { return _f; }
+
+ private:
+  std::shared_ptr<Context> _ctx;
+  std::vector<int> _Z;
+  std::vector<double> _xyz;
+  std::shared_ptr<BasisController> _basis;
+  Functional _f;
 };
 
 /* ScalarOperatorToMatrixAdder(bfogc, blockAveThreshold): F += integral of v (and g). */
@@ -293,12 +437,47 @@ class ScalarOperatorToMatrixAdder {
   double _thr;
 };
 
-/* FuncPotential: getMatrix(P) returns V_xc (FuncPotential.cpp:73-111), getEnergy the E_xc of that build. */
-class FuncPotential {
+/* FuncPotential (FuncPotential.h:48-145).  The reference's form: FuncPotential(system, dMat, grid,
+ * functional), created with std::make_shared; getMatrix() builds V_xc of the controller's current
+ * density matrix on first use and keeps it until the density matrix changes (notify), getEnergy(P)
+ * returns the E_xc of that build (P is not read, as in the reference).  The explicit form
+ * FuncPotential(bfogc, functional, precision) has getMatrix(P) and getEnergy(). */
+class FuncPotential : public ObjectSensitiveClass, public std::enable_shared_from_this<FuncPotential> {
  public:
+  FuncPotential(std::shared_ptr<SystemController> system, std::shared_ptr<DensityMatrixController> dMat,
+                std::shared_ptr<GridController> grid, Functional functional, Precision p = Precision::FP64,
+                double blockAveThreshold = 1e-11)
+      : _system(std::move(system)), _dMat(std::move(dMat)), _f(functional), _p(p), _thr(blockAveThreshold) {
+    _bf = std::make_shared<BasisFunctionOnGridController>(_system->getBasisController(), std::move(grid), 128, 1e-9,
+                                                          1, p != Precision::FP64);
+  }
   FuncPotential(std::shared_ptr<BasisFunctionOnGridController> bf, Functional f, Precision p = Precision::FP64,
                 double blockAveThreshold = 1e-11)
       : _bf(std::move(bf)), _f(f), _p(p), _thr(blockAveThreshold) {}
+  const Matrix& getMatrix() {
+    if (!_dMat) throw SerenityError(XC_ERR_STATE, "FuncPotential: constructed without a density matrix controller");
+    if (!_registered) {
+      std::shared_ptr<FuncPotential> self;
+      try {
+        self = shared_from_this();
+      } catch (const std::bad_weak_ptr&) {
+        throw SerenityError(XC_ERR_STATE, "FuncPotential: create it with std::make_shared to use getMatrix()");
+      }
+      _dMat->addSensitiveObject(self);
+      _registered = true;
+    }
+    if (!_cacheValid) {
+      _F = getMatrix(_dMat->getDensityMatrix());
+      _cacheValid = true;
+    }
+    return _F;
+  }
+  double getEnergy(const Matrix& /*P*/) {
+    getMatrix();
+    return _energy;
+  }
+  void notify() override { _cacheValid = false; }
+  Functional getFunctional() const { return _f; }
   Matrix getMatrix(const Matrix& P) {
     const int n = (int)_bf->getNBasisFunctions();
     if (P.rows != n || P.cols != n)
@@ -325,12 +504,16 @@ class FuncPotential {
   }
 
  private:
+  std::shared_ptr<SystemController> _system;
+  std::shared_ptr<DensityMatrixController> _dMat;
   std::shared_ptr<BasisFunctionOnGridController> _bf;
   Functional _f;
   Precision _p;
   double _thr;
   double _energy = 0.0;
   bool _built = false;
+  bool _registered = false, _cacheValid = false;
+  Matrix _F;
 };
 
 /* NAddFuncPotential (NAddFuncPotential.cpp:192-301), restricted: the non-additive potential of the active

@@ -117,6 +117,52 @@ int main(int argc, char** argv) {
         threw = true;
       }
       write(out, "threw_on_bad_P", {threw ? 1.0 : 0.0});
+    } else if (cs == "funcpot_ref") {
+      // the reference's form: FuncPotential(system, dMat, grid, functional), getMatrix(), notify
+      std::ifstream pf(argv[2]);
+      std::string token, basis_path;
+      while (pf >> token)
+        if (token == "basis_path") pf >> basis_path;
+      auto Z = ints(in["Z"]);
+      auto b = BasisController::fromFile(ctx, basis_path, Z, in["xyz"], true);
+      auto g = GridController::molecular(ctx, Z, in["xyz"], XC_GRID_SSF, (int)in["accuracy"][0]);
+      const int n = (int)b->getNBasisFunctions();
+      Matrix P(n, n);
+      P.data = in["P"];
+      const auto f = (Functional)(int)in["functional"][0];
+      auto system = std::make_shared<SystemController>(ctx, Z, in["xyz"], b, f);
+      auto dMat = std::make_shared<DensityMatrixController>(P);
+      auto pot = std::make_shared<FuncPotential>(system, dMat, g, f);
+      const Matrix& F1 = pot->getMatrix();
+      write(out, "F1", F1.data);
+      write(out, "E1", {pot->getEnergy(P)});
+      // a new density matrix reaches the potential through notify
+      Matrix P2 = P;
+      for (double& x : P2.data) x *= 0.9;
+      dMat->setDensityMatrix(P2);
+      write(out, "F2", pot->getMatrix().data);
+      write(out, "E2", {pot->getEnergy(P2)});
+      // FunctionalLibrary::calcData on a density-on-grid controller of the same density matrix
+      auto bf = std::make_shared<BasisFunctionOnGridController>(b, g, 128, 1e-9, 1);
+      auto calc = std::make_shared<DensityOnGridCalculator>(bf, 1e-11);
+      auto dog = DensityMatrixDensityOnGridController::produce(calc, dMat);
+      FunctionalLibrary lib(ctx, 128);
+      auto d = lib.calcData(FUNCTIONAL_DATA_TYPE::GRADIENTS, f, dog);
+      write(out, "calc_energy", {d.energy});
+      write(out, "calc_dFdRho", d.dFdRho);
+      auto d2 = lib.calcData(FUNCTIONAL_DATA_TYPE::GRADIENT_INVARIANTS, f, dog);
+      write(out, "calc_dFdSigma", d2.dFdSigma);
+      write(out, "rho", dog->getDensityOnGrid());
+      write(out, "grad", dog->getDensityGradientOnGrid());
+      // the explicit form without shared ownership cannot register for notify
+      bool threw = false;
+      try {
+        FuncPotential loose(system, dMat, g, f);
+        loose.getMatrix();
+      } catch (const SerenityError&) {
+        threw = true;
+      }
+      write(out, "threw_without_make_shared", {threw ? 1.0 : 0.0});
     } else if (cs == "nadd") {
       // active and environment subsystems (sections prefixed act_ / env_) on one grid
       auto subsystem = [&](const std::string& pre) {
@@ -146,6 +192,14 @@ int main(int argc, char** argv) {
         threw = true;
       }
       write(out, "threw_on_missing_env", {threw ? 1.0 : 0.0});
+      // NAddFuncPotential<UNRESTRICTED>: the closed-shell dimer with both spins at P / 2
+      Matrix Pa2 = Pa, Pe2 = Pe;
+      for (double& x : Pa2.data) x *= 0.5;
+      for (double& x : Pe2.data) x *= 0.5;
+      auto FaFb = pot.getMatrix(Pa2, Pa2, {Pe2}, {Pe2});
+      write(out, "F_alpha", FaFb.first.data);
+      write(out, "F_beta", FaFb.second.data);
+      write(out, "energy_unrestricted", {pot.getEnergy()});
     } else {
       std::cerr << "unknown case " << cs << std::endl;
       return 2;

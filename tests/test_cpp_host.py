@@ -117,8 +117,39 @@ def test_cpp_funcpot_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+def test_cpp_reference_shaped_funcpotential(tmp_path):
+    """FuncPotential(system, dMat, grid, functional) with getMatrix(), getEnergy(P) and notify through a
+    DensityMatrixController (FuncPotential.h:61-109), and FunctionalLibrary::calcData(type, functional,
+    densityOnGridController) (FunctionalLibrary.h:61-63), against the oracle."""
+    Z, X = W.alkane(2)
+    b = xc.BasisController.from_file("DEF2-SVP", Z, X)
+    g = xc.molecular_grid(Z, X, accuracy=3)
+    P = W.synthetic_density(b.overlap(), W.n_electrons(Z) // 2)
+    res = _run("funcpot_ref", dict(Z=Z, xyz=X, accuracy=[3], functional=[1], P=P.ravel(order="F")), tmp_path,
+               extra="basis_path %s\n" % os.path.join(xc.BASIS_DIR, "DEF2-SVP"))
+    n = b.nao
+    for tag, Pm in (("1", P), ("2", 0.9 * P)):
+        Fo, Eo, _, _ = O.fock_build(b, g.points, g.weights, Pm, "PBE")
+        F = res["F" + tag].reshape(n, n, order="F")
+        assert np.abs(F - Fo).max() < 1e-9 and abs(res["E" + tag][0] - Eo) < 1e-9, tag
+    # calcData on the controller's density (the density matrix is now 0.9 P)
+    d = O.density(b, g.points, 0.9 * P)
+    _, vo, _, Eo = O.functional("PBE", d["rho"], d["grad"], g.weights)
+    assert abs(res["calc_energy"][0] - Eo) < 1e-9
+    assert np.abs(res["calc_dFdRho"] - vo).max() < 1e-9 * max(1.0, np.abs(vo).max())
+    # dF/dsigma times 2 grad rho is dF/dgrad rho
+    rho, grad = res["rho"], res["grad"].reshape(3, -1)
+    assert np.abs(rho - d["rho"]).max() < 1e-10
+    _, _, go, _ = O.functional("PBE", rho, grad, g.weights)
+    sig = res["calc_dFdSigma"]
+    assert np.abs(2.0 * sig[None, :] * grad - go).max() < 1e-8 * max(1.0, np.abs(go).max())
+    assert res["threw_without_make_shared"][0] == 1.0
+
+
+@pytest.mark.gpu
 def test_cpp_nadd_known_answers(tmp_path):
-    """NAddFuncPotential through the C++ class: NAddFuncPotential_test.cpp:58-82 (LDA, 1e-5), oracle to 1e-12."""
+    """NAddFuncPotential through the C++ class: NAddFuncPotential_test.cpp:58-82 (LDA, 1e-5), oracle to 1e-12,
+    and its unrestricted overload on the same dimer (:87-116)."""
     s = common.h2_fde_system()
     (ba, Pa), (be, Pe), g = s["act"], s["env"], s["grid"]
     sec = dict(points=g.points, weights=g.weights, functional=[0], P_act=Pa.ravel(order="F"),
@@ -133,3 +164,11 @@ def test_cpp_nadd_known_answers(tmp_path):
     Fo, Eo = O.nadd_fock_build(ba, [be], g.points, g.weights, Pa, [Pe], "LDA")
     assert np.abs(F - Fo).max() < 1e-12 and abs(res["energy"][0] - Eo) < 1e-12
     assert res["threw_on_missing_env"][0] == 1.0
+    # the unrestricted overload (NAddFuncPotential_test.cpp:87-116): both spins at P / 2 give the
+    # restricted matrix for each spin and the same energy
+    for key in ("F_alpha", "F_beta"):
+        Fs = res[key].reshape(ba.nao, ba.nao, order="F")
+        for e in kat["entries"]:
+            assert abs(Fs[e["row"], e["col"]] - e["expected"]) < kat["tolerance"], (key, e)
+        assert np.abs(Fs - F).max() < 1e-12
+    assert abs(res["energy_unrestricted"][0] - res["energy"][0]) < 1e-12
