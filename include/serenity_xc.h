@@ -233,6 +233,21 @@ int xc_density(xc_bfogc* bfogc, const double* P, double* rho, double* grad, int*
 int xc_functional(xc_ctx* ctx, int functional, int64_t npoints, int block_size, const double* rho,
                   const double* grad, const double* weights, double* eps, double* v, double* g, double* energy);
 
+/* No counterpart in the reference: xc_functional on device arrays that allocates
+ * nothing, for callers that cannot call hipMalloc (code under HIP graph capture,
+ * serving loops and harnesses that forbid allocation).  d_rho (N), d_grad (3 x N
+ * SoA; may be NULL for LDA-type functionals), d_weights (N) in; d_eps, d_v (N)
+ * and d_g (3 x N, may be NULL; zero-filled for LDA-type functionals) out, all
+ * device buffers of the context's device; energy (host, may be NULL: then the
+ * call only enqueues) = sum_p w_p eps_p.  Results equal xc_functional's.  The
+ * per-block energies live in one static device workspace per device of at most
+ * 131072 blocks (1 MiB), so at most 131072 * block_size points (XC_ERR_ARG
+ * beyond); calls that overlap on one device share that workspace and race, so
+ * serialise them (calls on one context are). */
+int xc_functional_device_noalloc(xc_ctx* ctx, int functional, int64_t npoints, int block_size, const double* d_rho,
+                                 const double* d_grad, const double* d_weights, double* d_eps, double* d_v,
+                                 double* d_g, double* energy);
+
 /* ScalarOperatorToMatrixAdder::addScalarOperatorToMatrix(F, v[, g])
  * (ScalarOperatorToMatrixAdder.cpp:51-113, addBlock :179-303): F += integral
  * of the scalar potential v (and gradient potential g, 3 x N SoA, or NULL).

@@ -94,6 +94,7 @@ def lib():
     _sig("xc_fock_build_unrestricted_device", C.c_int, vp, C.c_int, C.c_int, C.c_double, vp, vp, vp, vp, dp)
     _sig("xc_density", C.c_int, vp, dp, dp, dp, ip)
     _sig("xc_functional", C.c_int, vp, C.c_int, i64, C.c_int, dp, dp, dp, dp, dp, dp, dp)
+    _sig("xc_functional_device_noalloc", C.c_int, vp, C.c_int, i64, C.c_int, vp, vp, vp, vp, vp, vp, dp)
     _sig("xc_fock_add", C.c_int, vp, C.c_double, dp, dp, dp)
     _sig("xc_fock_build", C.c_int, vp, C.c_int, C.c_int, C.c_double, dp, dp, dp)
     _sig("xc_fock_build_device", C.c_int, vp, C.c_int, C.c_int, C.c_double, vp, vp, dp)

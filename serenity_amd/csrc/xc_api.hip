@@ -85,6 +85,35 @@ int guarded(F&& f) {
 
 }  // namespace
 
+// Workspace of xc_functional_device_noalloc: the per-block energies and their sum live in this
+// file-scope device array instead of a per-call allocation.  Each device holds its own copy;
+// host code never uses the array's name as an address, only the pointer that
+// hipGetSymbolAddress returns for the current device (kept per device in g_noalloc_ptr).
+constexpr int kNoallocBlocks = 131072;  // 1 MiB of per-block energies + the sum
+static __device__ __align__(256) double scratch[kNoallocBlocks + 1];
+
+namespace {
+std::mutex g_noalloc_mu;
+std::map<int, double*> g_noalloc_ptr;  // device -> address of that device's scratch
+
+// Called once per device with that device current (context creation).
+void noalloc_setup_current_device(int device) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr.count(device)) return;
+  void* p = nullptr;
+  HIPX(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  require(p != nullptr, XC_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+  g_noalloc_ptr[device] = static_cast<double*>(p);
+}
+
+double* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  auto it = g_noalloc_ptr.find(device);
+  require(it != g_noalloc_ptr.end(), XC_ERR_STATE, "no-allocation workspace not set up on this device");
+  return it->second;
+}
+}  // namespace
+
 // Execution lanes of a build: its stream plus kAux helper streams.  The launches
 // of one stage (one per work class) are independent, so they are spread over
 // the lanes after a fork and joined before the next stage.
@@ -466,6 +495,7 @@ int xc_ctx_create(int device, xc_ctx** out) {
     auto* c = new xc_ctx();
     c->device = device;
     HIPX(hipSetDevice(device));
+    noalloc_setup_current_device(device);
     HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->lanes.create(c->stream, true);
     c->d_energy = (double*)c->alloc(sizeof(double));
@@ -1192,6 +1222,38 @@ int xc_functional(xc_ctx* ctx, int functional, int64_t npts, int block_size, con
     HIPX(hipMemcpyAsync(&E, d_E.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPX(hipStreamSynchronize(ctx->stream));
     if (energy) *energy = E;
+  });
+}
+
+int xc_functional_device_noalloc(xc_ctx* ctx, int functional, int64_t npts, int block_size, const double* d_rho,
+                                 const double* d_grad, const double* d_weights, double* d_eps, double* d_v,
+                                 double* d_g, double* energy) {
+  return guarded([&] {
+    require(ctx && d_rho && d_weights && npts > 0, XC_ERR_ARG, "xc_functional_device_noalloc: bad arguments");
+    require(functional >= 0 && functional < kNumFunctionals, XC_ERR_ARG,
+            "xc_functional_device_noalloc: unknown functional");
+    require(block_size >= 1 && block_size <= kTile, XC_ERR_ARG,
+            "xc_functional_device_noalloc: block size must be in [1,128]");
+    const bool gga = functional_is_gga(functional);
+    require(!gga || d_grad, XC_ERR_ARG, "xc_functional_device_noalloc: GGA functional needs the density gradient");
+    const int64_t nxc = (npts + block_size - 1) / block_size;
+    require(nxc <= kNoallocBlocks, XC_ERR_ARG,
+            "xc_functional_device_noalloc: " + std::to_string(nxc) + " blocks of " + std::to_string(block_size) +
+                " points exceed the workspace of at most " + std::to_string(kNoallocBlocks) + " blocks");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipSetDevice(ctx->device));
+    double* scr = noalloc_scratch(ctx->device);
+    // every per-block slot [0, nxc) is written by the functional kernel and the sum slot by the
+    // sum kernel, so nothing left from an earlier call is read
+    launch_functional(ctx->stream, functional, ctx->prm, npts, block_size, d_rho, gga ? d_grad : nullptr, nullptr,
+                      nullptr, d_weights, d_eps, d_v, gga ? d_g : nullptr, nullptr, nullptr, scr);
+    if (!gga && d_g) launch_fill(ctx->stream, d_g, 3 * npts, 0.0);
+    launch_sum(ctx->stream, scr, nxc, scr + kNoallocBlocks);
+    HIPX(hipGetLastError());
+    if (energy) {
+      HIPX(hipMemcpyAsync(energy, scr + kNoallocBlocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIPX(hipStreamSynchronize(ctx->stream));
+    }
   });
 }
 
