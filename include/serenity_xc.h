@@ -106,10 +106,13 @@ int xc_molecular_grid(int natoms, const int* Z, const double* coords_bohr, int g
  * weights, in place (HilbertRTreeSorting::sort, src/grid/HilbertRTreeSorting.cpp
  * :27-216; equal keys keep their input order). */
 int xc_hilbert_sort(int64_t npoints, double* points, double* weights);
-/* xc_molecular_grid with the fuzzy-cell partition (the O(N natoms^2) part) on
- * the context's device; the candidates, the weight cut and the Hilbert order are
- * those of the host build, so the result is the same grid.  No counterpart in the
- * reference, whose grid is built on the CPU (GridFactory.cpp:51-300). */
+/* xc_molecular_grid built on the context's device: the host makes each
+ * element's reference atom grid once; the candidates, the fuzzy-cell partition
+ * (the O(N natoms^2) part), the weight cut and the Hilbert order run on the
+ * device with the host build's arithmetic (shared functions, a stable sort on
+ * the same keys), so the result is the same grid.  No counterpart in the
+ * reference, whose grid is built on the CPU (GridFactory.cpp:51-300,
+ * HilbertRTreeSorting.cpp:27-216). */
 int xc_molecular_grid_device(xc_ctx* ctx, int natoms, const int* Z, const double* coords, int grid_type,
                              int radial_type, int accuracy, int smoothing, double weight_threshold, int sort,
                              int64_t* npoints, double** points, double** weights, int64_t* npoints_before_cut);

@@ -10,6 +10,7 @@
 #include <stdexcept>
 
 #include "host.h"
+#include "hilbert.h"
 #include "partition.h"
 
 namespace sxc {
@@ -254,14 +255,19 @@ void lebedev_rule(int npoints, std::vector<double>& x, std::vector<double>& y, s
   throw std::runtime_error("lebedev: no rule with " + std::to_string(npoints) + " points");
 }
 
-void hilbert_sort(std::vector<double>& points, std::vector<double>& weights) {
-  const int64_t n = (int64_t)weights.size();
-  if (n < 2) return;
+int64_t hilbert_length(int64_t n) {
   int64_t length = 2, nvert = 8;
   while (n * 8 > nvert) {
     length *= 2;
     nvert *= 8;
   }
+  return length;
+}
+
+void hilbert_sort(std::vector<double>& points, std::vector<double>& weights) {
+  const int64_t n = (int64_t)weights.size();
+  if (n < 2) return;
+  const int64_t length = hilbert_length(n);
   double mn[3], spread[3];
   for (int d = 0; d < 3; ++d) {
     double lo = std::numeric_limits<double>::infinity(), hi = -lo;
@@ -272,34 +278,9 @@ void hilbert_sort(std::vector<double>& points, std::vector<double>& weights) {
     mn[d] = lo;
     spread[d] = double(length) / (hi - lo);
   }
-  // Child-cube numbering and the transition table between refinement levels.
-  static const int trans[8][8] = {{0, 7, 6, 1, 2, 5, 4, 3}, {0, 3, 4, 6, 7, 5, 2, 1}, {0, 3, 4, 6, 7, 5, 2, 1},
-                                  {2, 3, 0, 1, 6, 7, 4, 5}, {2, 3, 0, 1, 6, 7, 4, 5}, {6, 5, 2, 1, 0, 3, 4, 7},
-                                  {6, 5, 2, 1, 0, 3, 4, 7}, {4, 3, 2, 5, 6, 1, 0, 7}};
-  static const int val[2][2][2] = {{{5, 6}, {4, 7}}, {{2, 1}, {3, 0}}};
   std::vector<int64_t> key(n);
 #pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < n; ++i) {
-    int64_t pt[3];
-    for (int d = 0; d < 3; ++d) pt[d] = (int64_t)(int)((points[3 * i + d] - mn[d]) * spread[d]);
-    int64_t l = length / 2;
-    int gx = pt[0] > l, gy = pt[1] > l, gz = pt[2] > l;
-    int v = val[gx][gy][gz];
-    int64_t idx = v;
-    while (l > 1) {
-      idx *= 8;
-      if (gx) pt[0] -= l;
-      if (gy) pt[1] -= l;
-      if (gz) pt[2] -= l;
-      l /= 2;
-      gx = pt[0] > l;
-      gy = pt[1] > l;
-      gz = pt[2] > l;
-      v = trans[v][val[gx][gy][gz]];
-      idx += v;
-    }
-    key[i] = idx;
-  }
+  for (int64_t i = 0; i < n; ++i) key[i] = hilbert_key(&points[3 * i], mn, spread, length);
   // Descending key order; equal keys keep their input order (the order a
   // single-threaded run of the reference's selection/merge sort produces).
   std::vector<int64_t> order(n);
@@ -317,7 +298,8 @@ void hilbert_sort(std::vector<double>& points, std::vector<double>& weights) {
   weights.swap(w2);
 }
 
-GridProblem grid_problem(const std::vector<int>& Z, const std::vector<double>& xyz, const GridSettings& s) {
+GridProblem grid_problem(const std::vector<int>& Z, const std::vector<double>& xyz, const GridSettings& s,
+                         bool with_candidates) {
   const int nAtoms = (int)Z.size();
   if (nAtoms == 0) throw std::runtime_error("grid: empty geometry");
   if ((int)xyz.size() != 3 * nAtoms) throw std::runtime_error("grid: coordinate array size mismatch");
@@ -361,7 +343,18 @@ GridProblem grid_problem(const std::vector<int>& Z, const std::vector<double>& x
   for (int k = 0; k < nAtoms; ++k)
     for (int l = 0; l < nAtoms; ++l)
       if (l != k) P.min_dist[k] = std::min(P.min_dist[k], dist[l + (size_t)nAtoms * k]);
+  std::vector<int64_t> elem_off(kMaxZ + 1, -1);
   for (int k = 0; k < nAtoms; ++k) {
+    if (elem_off[Z[k]] < 0) {
+      elem_off[Z[k]] = (int64_t)P.ref_w.size();
+      P.ref_xyz.insert(P.ref_xyz.end(), refPts[Z[k]].begin(), refPts[Z[k]].end());
+      P.ref_w.insert(P.ref_w.end(), refW[Z[k]].begin(), refW[Z[k]].end());
+    }
+    P.ref_off.push_back(elem_off[Z[k]]);
+  }
+  P.cand_off.assign(nAtoms + 1, 0);
+  for (int k = 0; k < nAtoms; ++k) P.cand_off[k + 1] = P.cand_off[k] + (int64_t)refW[Z[k]].size();
+  for (int k = 0; k < nAtoms && with_candidates; ++k) {
     const auto& rp = refPts[Z[k]];
     const auto& rw = refW[Z[k]];
     for (size_t i = 0; i < rw.size(); ++i) {

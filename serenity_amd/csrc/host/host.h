@@ -82,8 +82,17 @@ struct GridProblem {
   std::vector<double> xyz, dist, apair, min_dist;
   std::vector<double> cand_xyz, cand_w;
   std::vector<int> cand_atom;
+  // the same candidates as reference grids: ref_xyz / ref_w hold each element's atom grid once;
+  // atom k's candidates are ref points ref_off[k] .. ref_off[k] + (cand_off[k + 1] - cand_off[k])
+  // translated to the atom, candidate index cand_off[k] + i
+  std::vector<double> ref_xyz, ref_w;
+  std::vector<int64_t> ref_off, cand_off;
 };
-GridProblem grid_problem(const std::vector<int>& Z, const std::vector<double>& xyz_bohr, const GridSettings& s);
+// with_candidates = false leaves cand_xyz / cand_w / cand_atom empty (the device builds them).
+GridProblem grid_problem(const std::vector<int>& Z, const std::vector<double>& xyz_bohr, const GridSettings& s,
+                         bool with_candidates = true);
+// Hilbert-curve box of n points: the cube edge length (a power of 2, at least 2) the keys use.
+int64_t hilbert_length(int64_t n);
 // Partitioned weight of every candidate (-1: dropped by the SSF screening), on the host.
 void partition_weights_host(const GridProblem& P, std::vector<double>& w);
 // Weight cut (w > threshold, candidates in order) and optional Hilbert ordering.

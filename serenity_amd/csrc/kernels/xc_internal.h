@@ -252,6 +252,24 @@ void launch_partition(hipStream_t s, int type, unsigned smoothing, int natoms, c
                       const double* apair, const double* min_dist, int64_t ncand, const double* cand_xyz,
                       const double* cand_w, const int* cand_atom, double* w);
 
+// The device grid build (kernels/grid_kernels.hip): partition weights of every candidate generated
+// from the reference grids (keep[c] = w[c] > thr), the cut (exclusive scan of keep into pos, the
+// kept points in candidate order), the bounding box (mn, spread) and the Hilbert order.
+void launch_partition_gen(hipStream_t s, int type, unsigned smoothing, int natoms, const double* xyz,
+                          const double* dist, const double* apair, const double* min_dist, int64_t ncand,
+                          const double* ref_xyz, const double* ref_w, const int64_t* ref_off,
+                          const int64_t* cand_off, double thr, double* w, int* keep);
+size_t grid_scan_temp_bytes(int64_t n);
+void launch_cut(hipStream_t s, int natoms, const double* xyz, int64_t ncand, const double* ref_xyz,
+                const int64_t* ref_off, const int64_t* cand_off, const double* w, const int* keep, int* pos,
+                void* temp, size_t temp_bytes, double* out_xyz, double* out_w);
+int bbox_part_count();
+void launch_bbox(hipStream_t s, const double* pts, int64_t n, int64_t length, double* part, double* box);
+size_t grid_sort_temp_bytes(int64_t n);
+void launch_hilbert_order(hipStream_t s, const double* pts, const double* w, int64_t n, const double* box,
+                          int64_t length, int64_t* key, int64_t* key_sorted, int64_t* idx, int64_t* order,
+                          void* temp, size_t temp_bytes, double* out_pts, double* out_w);
+
 void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
                              double* vs, double* gw);
 void launch_fill(hipStream_t s, double* p, int64_t n, double value);

@@ -633,37 +633,77 @@ int xc_molecular_grid_device(xc_ctx* ctx, int natoms, const int* Z, const double
     s.smoothing = (unsigned)smoothing;
     s.weight_threshold = weight_threshold;
     s.sort = sort != 0;
-    const GridProblem P =
-        grid_problem(std::vector<int>(Z, Z + natoms), std::vector<double>(coords, coords + 3 * natoms), s);
-    const int64_t nc = (int64_t)P.cand_w.size();
-    std::vector<double> w((size_t)nc);
-    {
-      std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-      HIPX(hipSetDevice(ctx->device));
-      DBuf<double> d_xyz, d_dist, d_apair, d_min, d_cxyz, d_cw, d_w;
-      DBuf<int> d_catom;
-      d_xyz.upload(ctx, P.xyz.data(), P.xyz.size());
-      d_dist.upload(ctx, P.dist.data(), P.dist.size());
-      d_apair.upload(ctx, P.apair.data(), P.apair.size());
-      d_min.upload(ctx, P.min_dist.data(), P.min_dist.size());
-      d_cxyz.upload(ctx, P.cand_xyz.data(), P.cand_xyz.size());
-      d_cw.upload(ctx, P.cand_w.data(), P.cand_w.size());
-      d_catom.upload(ctx, P.cand_atom.data(), P.cand_atom.size());
-      d_w.alloc(ctx, (size_t)std::max<int64_t>(nc, 1));
-      launch_partition(ctx->stream, P.type, P.smoothing, P.natoms, d_xyz.p, d_dist.p, d_apair.p, d_min.p, nc,
-                       d_cxyz.p, d_cw.p, d_catom.p, d_w.p);
-      HIPX(hipGetLastError());
-      if (nc) HIPX(hipMemcpyAsync(w.data(), d_w.p, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
-      HIPX(hipStreamSynchronize(ctx->stream));
+    // everything after the per-element reference grids runs on the device: candidates, partition,
+    // weight cut, Hilbert order
+    const GridProblem P = grid_problem(std::vector<int>(Z, Z + natoms),
+                                       std::vector<double>(coords, coords + 3 * natoms), s, false);
+    const int64_t nc = P.cand_off.back();
+    require(nc < (int64_t)1 << 31, XC_ERR_ARG, "xc_molecular_grid_device: too many candidate points");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPX(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DBuf<double> d_xyz, d_dist, d_apair, d_min, d_ref_xyz, d_ref_w, d_w, d_pts, d_pw, d_part, d_box, d_opts, d_ow;
+    DBuf<int64_t> d_ref_off, d_cand_off, d_key, d_key2, d_idx, d_order;
+    DBuf<int> d_keep, d_pos;
+    DBuf<unsigned char> d_temp;
+    d_xyz.upload(ctx, P.xyz.data(), P.xyz.size());
+    d_dist.upload(ctx, P.dist.data(), P.dist.size());
+    d_apair.upload(ctx, P.apair.data(), P.apair.size());
+    d_min.upload(ctx, P.min_dist.data(), P.min_dist.size());
+    d_ref_xyz.upload(ctx, P.ref_xyz.data(), P.ref_xyz.size());
+    d_ref_w.upload(ctx, P.ref_w.data(), P.ref_w.size());
+    d_ref_off.upload(ctx, P.ref_off.data(), P.ref_off.size());
+    d_cand_off.upload(ctx, P.cand_off.data(), P.cand_off.size());
+    const size_t ncs = (size_t)std::max<int64_t>(nc, 1);
+    d_w.alloc(ctx, ncs);
+    d_keep.alloc(ctx, ncs);
+    d_pos.alloc(ctx, ncs);
+    d_pts.alloc(ctx, 3 * ncs);
+    d_pw.alloc(ctx, ncs);
+    launch_partition_gen(st, P.type, P.smoothing, P.natoms, d_xyz.p, d_dist.p, d_apair.p, d_min.p, nc, d_ref_xyz.p,
+                         d_ref_w.p, d_ref_off.p, d_cand_off.p, weight_threshold, d_w.p, d_keep.p);
+    const size_t scan_bytes = grid_scan_temp_bytes(nc);
+    d_temp.alloc(ctx, std::max<size_t>(scan_bytes, 1));
+    launch_cut(st, P.natoms, d_xyz.p, nc, d_ref_xyz.p, d_ref_off.p, d_cand_off.p, d_w.p, d_keep.p, d_pos.p, d_temp.p,
+               scan_bytes, d_pts.p, d_pw.p);
+    HIPX(hipGetLastError());
+    int last[2] = {0, 0};
+    if (nc) {
+      HIPX(hipMemcpyAsync(&last[0], d_pos.p + nc - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPX(hipMemcpyAsync(&last[1], d_keep.p + nc - 1, sizeof(int), hipMemcpyDeviceToHost, st));
     }
-    MolecularGrid g = finish_grid(P, w, s);
-    const int64_t n = (int64_t)g.weights.size();
+    HIPX(hipStreamSynchronize(st));
+    const int64_t n = (int64_t)last[0] + last[1];
+    const double* res_pts = d_pts.p;
+    const double* res_w = d_pw.p;
+    if (s.sort && n >= 2) {
+      const int64_t length = hilbert_length(n);
+      d_part.alloc(ctx, 6 * (size_t)bbox_part_count());
+      d_box.alloc(ctx, 6);
+      launch_bbox(st, d_pts.p, n, length, d_part.p, d_box.p);
+      const size_t sort_bytes = grid_sort_temp_bytes(n);
+      if (sort_bytes > d_temp.n) d_temp.alloc(ctx, sort_bytes);
+      d_key.alloc(ctx, n);
+      d_key2.alloc(ctx, n);
+      d_idx.alloc(ctx, n);
+      d_order.alloc(ctx, n);
+      d_opts.alloc(ctx, 3 * (size_t)n);
+      d_ow.alloc(ctx, n);
+      launch_hilbert_order(st, d_pts.p, d_pw.p, n, d_box.p, length, d_key.p, d_key2.p, d_idx.p, d_order.p, d_temp.p,
+                           sort_bytes, d_opts.p, d_ow.p);
+      HIPX(hipGetLastError());
+      res_pts = d_opts.p;
+      res_w = d_ow.p;
+    }
     *npoints = n;
     *points = (double*)std::malloc(sizeof(double) * 3 * std::max<int64_t>(n, 1));
     *weights = (double*)std::malloc(sizeof(double) * std::max<int64_t>(n, 1));
-    std::memcpy(*points, g.points.data(), sizeof(double) * 3 * n);
-    std::memcpy(*weights, g.weights.data(), sizeof(double) * n);
-    if (npoints_before_cut) *npoints_before_cut = g.n_before_cut;
+    if (n) {
+      HIPX(hipMemcpyAsync(*points, res_pts, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+      HIPX(hipMemcpyAsync(*weights, res_w, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    }
+    HIPX(hipStreamSynchronize(st));
+    if (npoints_before_cut) *npoints_before_cut = nc;
   });
 }
 
