@@ -531,11 +531,13 @@ def main():
 
     if not args.no_serving:
         from serenity_amd import serving as SV
+        bf._release()  # the headline workload's controller: the serving legs get the device to themselves
         stream = SV.make_stream(args.serving_requests, args.serving_qps)
         sv = SV.run_stream(ctx, stream, nslots=4, log=log)
         energies, host = sv.pop("_energies"), sv.pop("_host")
         sv.pop("_failed")
-        sv["max_dE_xc_fp32_vs_fp64"], sv["fp64_sample"] = SV.fp64_check(ctx, host, energies, 8)
+        # every distinct (system, functional) served, against FP64 builds of the same density
+        sv["max_dE_xc_fp32_vs_fp64"], sv["fp64_checked"] = SV.fp64_check(ctx, host, energies)
         sv["workload"] = ("Poisson stream: (H2O)_n n~U{1..48} / CkH2k+2 k~U{1..30}, def2-SVP/TZVP, LDA/PBE, "
                           "FP32-MFMA builds on resident FP32 tiles")
         log("serving: %d requests at %.0f QPS: p50 %.3f ms, p99 %.3f ms, %.1f builds/s, max dE vs FP64 %.2e" %
@@ -554,6 +556,32 @@ def main():
         sv["packed"] = {k: v for k, v in svp.items() if k in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "builds_per_s",
                                                                "pack_max_nao", "packed_requests", "packed_launches",
                                                                "max_dE_packed_vs_slots")}
+        # offered load sweep (packing on): where p99 turns up
+        keep = ("qps_offered", "p50_ms", "p99_ms", "builds_per_s", "packed_requests")
+        sweep = []
+        for qps in (200.0, 800.0, 1600.0):
+            r = SV.run_stream(ctx, SV.make_stream(args.serving_requests, qps), nslots=4, log=log,
+                              pack_nao=SV.PACK_NAO, host=host)
+            for k in ("_energies", "_host", "_failed"):
+                r.pop(k)
+            sweep.append({k: r[k] for k in keep})
+            log("serving at %.0f QPS: p50 %.3f ms, p99 %.3f ms, %.1f builds/s" %
+                (r["qps_offered"], r["p50_ms"], r["p99_ms"], r["builds_per_s"]))
+        sweep.append({k: svp[k] for k in keep})
+        sv["qps_sweep"] = sorted(sweep, key=lambda x: x["qps_offered"])
+        # under memory pressure: a 64 GB budget against the ~210 GB working set, so systems are evicted
+        # and built again (grid on the GPU, AO tiles) inside the timed stream; offered at 30 QPS, below
+        # what cold builds sustain, so that the latencies are build times rather than queueing
+        pr = SV.run_stream(ctx, SV.make_stream(300, 30.0), nslots=4, budget_gb=64.0, log=log,
+                           pack_nao=SV.PACK_NAO, host=host)
+        for k in ("_energies", "_host", "_failed"):
+            pr.pop(k)
+        sv["pressure_64gb"] = {k: pr[k] for k in ("qps_offered", "requests", "p50_ms", "p99_ms", "builds_per_s", "created", "evicted",
+                                                   "cold_requests", "cold_p50_ms", "cold_p99_ms", "warm_p50_ms",
+                                                   "warm_p99_ms", "resident_gb")}
+        log("serving under a 64 GB budget: p50 %.3f ms, p99 %.3f ms, %d evictions, %d cold requests "
+            "(p50 %s ms, p99 %s ms)" % (pr["p50_ms"], pr["p99_ms"], pr["evicted"], pr["cold_requests"],
+                                        pr["cold_p50_ms"], pr["cold_p99_ms"]))
         out_extra["serving"] = sv
 
     line = dict(

@@ -180,3 +180,28 @@ def test_serving_failed_request_is_isolated(ctx):
     for k, e in e1.items():
         assert abs(e - e0[k]) < 1e-10, k
     assert len(e1) >= len(e0) - 1
+
+
+def test_serving_under_memory_pressure(ctx):
+    """A device budget far below the working set: systems are evicted and admitted again inside the
+    stream (their grids built on the GPU when first admitted, inside the latency of that request),
+    and every request's E_xc equals the same build done alone."""
+    from serenity_amd import serving as SV
+    stream = SV.make_stream(60, 1.0e6)
+    host = SV.HostSystems(ctx=ctx)
+    out = SV.run_stream(ctx, stream, nslots=2, budget_gb=0.3, host=host, pack_nao=SV.PACK_NAO)
+    energies = out.pop("_energies")
+    say("pressure: %d requests, %d controllers built, %d evicted, %d cold (p50 %s ms) / warm p50 %s ms" %
+        (out["requests"], out["created"], out["evicted"], out["cold_requests"], out["cold_p50_ms"],
+         out["warm_p50_ms"]))
+    assert out["failed"] == 0 and out["evicted"] > 0 and out["cold_requests"] > 0
+    worst = 0.0
+    for (key, func), e in energies.items():
+        d = host.get(key)
+        bf = xc.BasisFunctionOnGridController(ctx, d["basis"], host.grid(key), 128, 1e-9, 1, keep_fp32="only")
+        pot = xc.FuncPotential(bf, func, "FP32")
+        pot.getMatrix(d["P"])
+        worst = max(worst, abs(pot.getEnergy() - e))
+        bf._release()
+    say("pressure: max |dE| against the builds alone %.2e over %d (system, functional)" % (worst, len(energies)))
+    assert worst < 1e-10
