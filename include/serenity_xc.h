@@ -35,12 +35,13 @@ enum {
 };
 
 /* Functionals available on the device.  LDA = Slater + VWN5, PBE = PBE
- * exchange + PBE correlation (PW92 LDA part), TF = Thomas-Fermi kinetic.
- * Serenity: functional_definitions.dat:44,47,80.  XC_FUNC_PBE carries the
+ * exchange + PBE correlation (PW92 LDA part), TF = Thomas-Fermi kinetic,
+ * BP86 = Becke 88 exchange + Perdew 86 correlation (on PZ81).
+ * Serenity: functional_definitions.dat:44,47,48,80.  XC_FUNC_PBE carries the
  * constants of XCFun (Serenity's default evaluator); XC_FUNC_PBE_LIBXC those of
  * LibXC (customFunc.impl = LIBXC), which differ in the exchange mu. */
 enum { XC_FUNC_LDA = 0, XC_FUNC_PBE = 1, XC_FUNC_TF = 2, XC_FUNC_SLATER = 3, XC_FUNC_VWN5 = 4,
-       XC_FUNC_PBEX = 5, XC_FUNC_PBEC = 6, XC_FUNC_PBE_LIBXC = 7 };
+       XC_FUNC_PBEX = 5, XC_FUNC_PBEC = 6, XC_FUNC_PBE_LIBXC = 7, XC_FUNC_BP86 = 8 };
 
 /* Precision of the two dense contractions (phi.P and phi^T.(w v phi)).  The
  * functional, the E_xc reduction and the F accumulation are FP64 in every mode.

@@ -62,7 +62,14 @@ struct Matrix {
   void setZero() { std::fill(data.begin(), data.end(), 0.0); }
 };
 
-enum class Functional { LDA = XC_FUNC_LDA, PBE = XC_FUNC_PBE, TF = XC_FUNC_TF, PBE_LIBXC = XC_FUNC_PBE_LIBXC };
+enum class Functional {
+  LDA = XC_FUNC_LDA,
+  PBE = XC_FUNC_PBE,
+  TF = XC_FUNC_TF,
+  PBE_LIBXC = XC_FUNC_PBE_LIBXC,
+  BP86 = XC_FUNC_BP86
+};
+inline bool isGGA(Functional f) { return f == Functional::PBE || f == Functional::PBE_LIBXC || f == Functional::BP86; }
 enum class Precision { FP64 = XC_PREC_FP64, FP32 = XC_PREC_FP32, MIXED = XC_PREC_MIXED };
 /* PartialDerivatives.h:931: derivatives with respect to the gradient invariant sigma = |grad rho|^2,
  * to the gradient components, or the potential alone. */
@@ -379,7 +386,7 @@ class FunctionalLibrary {
     const int64_t N = (int64_t)rho.size();
     d.epuv.resize(N);
     d.dFdRho.resize(N);
-    const bool gga = f == Functional::PBE || f == Functional::PBE_LIBXC;
+    const bool gga = isGGA(f);
     if (gga) d.dFdGradRho.resize(3 * N);
     check(xc_functional(_ctx->get(), (int)f, N, (int)_bs, rho.data(), grad ? grad->data() : nullptr, weights.data(),
                         d.epuv.data(), d.dFdRho.data(), gga ? d.dFdGradRho.data() : nullptr, &d.energy));

@@ -13,8 +13,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libxc_oracle.so")
-FUNCTIONALS = {"LDA": 0, "PBE": 1, "TF": 2, "SLATER": 3, "VWN5": 4, "PBEX": 5, "PBEC": 6, "PBE_LIBXC": 7}
-GGA = {"PBE", "PBEX", "PBEC", "PBE_LIBXC"}
+FUNCTIONALS = {"LDA": 0, "PBE": 1, "TF": 2, "SLATER": 3, "VWN5": 4, "PBEX": 5, "PBEC": 6, "PBE_LIBXC": 7, "BP86": 8}
+GGA = {"PBE", "PBEX", "PBEC", "PBE_LIBXC", "BP86"}
 
 dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)

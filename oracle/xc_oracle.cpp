@@ -305,6 +305,15 @@ struct FParams {
   double pw_A = 0.0310907, pw_a1 = 0.21370, pw_b1 = 7.5957, pw_b2 = 3.5876, pw_b3 = 1.6382, pw_b4 = 0.49294;
   double kappa = 0.804, mu = 0.066725 * M_PI * M_PI / 3.0, beta = 0.06672455060314922;
   double mu_libxc = 0.2195149727645171;
+  // BP86 (Becke 1988 exchange, Perdew 1986 correlation on Perdew-Zunger 1981): the published
+  // constants; the prefactor of the P86 Phi is f~ (9 pi)^(1/6), which the reference's BP86 Fock
+  // matrix of H2/def2-TZVP (FuncPotential_test.cpp:148-232) reproduces with the original PZ81
+  // C and D (1.745 f~ or the continuity-corrected C, D do not)
+  double b88_beta = 0.0042;
+  double pzU[7] = {-0.1423, 1.0529, 0.3334, 0.0311, -0.048, 0.0020, -0.0116};     // gamma b1 b2 A B C D
+  double pzP[7] = {-0.0843, 1.3981, 0.2611, 0.01555, -0.0269, 0.0007, -0.0048};
+  double p86_alpha = 0.023266, p86_beta = 7.389e-6, p86_gamma = 8.723, p86_delta = 0.472, p86_c0 = 0.001667,
+         p86_c1 = 0.002568, p86_ft = 0.11;
 };
 
 // e, de/drho, de/dsigma
@@ -383,7 +392,63 @@ EV tf(double r) {
   return {cf * r * r23, (5.0 / 3.0) * cf * r23, 0.0};
 }
 
-bool is_gga(int f) { return f == 1 || f == 5 || f == 6 || f == 7; }
+// Becke 1988 exchange, closed shell (rho_s = rho / 2, |grad rho_s| = |grad rho| / 2): Slater plus
+// two spins of -beta rho_s^(4/3) x^2 / Dn, Dn = 1 + 6 beta x asinh x, x = |grad rho_s| / rho_s^(4/3),
+// i.e. -(beta / 2) sigma rho_s^(-4/3) / Dn; with q = x Dn'(x) / Dn its derivatives are
+// d/dsigma = -(beta / 2) rho_s^(-4/3) (1 - q / 2) / Dn, d/drho = (beta / 3) sigma rho_s^(-7/3) (1 - q) / Dn
+EV b88(double r, double sig, const FParams& P) {
+  const EV lda = slater(r);
+  const double b = P.b88_beta;
+  const double ns = 0.5 * r, ns43 = ns * std::cbrt(ns);
+  const double x = 0.5 * std::sqrt(sig) / ns43;
+  const double Dn = 1.0 + 6.0 * b * x * std::asinh(x);
+  const double q = x * 6.0 * b * (std::asinh(x) + x / std::sqrt(1.0 + x * x)) / Dn;
+  const double term = -0.5 * b * sig / ns43 / Dn;
+  return {lda.e + term, lda.dr + (b * sig / 3.0) / (ns43 * ns) / Dn * (1.0 - q),
+          -0.5 * b / ns43 / Dn * (1.0 - 0.5 * q)};
+}
+
+// Perdew-Zunger 1981 correlation per particle and its r_s derivative (c = gamma b1 b2 A B C D)
+void pz81(double rs, const double* c, double& eps, double& deps) {
+  if (rs >= 1.0) {
+    const double s = std::sqrt(rs), den = 1.0 + c[1] * s + c[2] * rs;
+    eps = c[0] / den;
+    deps = -c[0] * (0.5 * c[1] / s + c[2]) / (den * den);
+  } else {
+    const double l = std::log(rs);
+    eps = c[3] * l + c[4] + c[5] * rs * l + c[6] * rs;
+    deps = c[3] / rs + c[5] * (l + 1.0) + c[6];
+  }
+}
+
+EV pz81c(double r, const FParams& P) {
+  const double rs = std::cbrt(3.0 / (4.0 * M_PI * r));
+  double eps, de;
+  pz81(rs, P.pzU, eps, de);
+  return {r * eps, eps - rs / 3.0 * de, 0.0};
+}
+
+// Perdew 1986 gradient correction, closed shell: e = exp(-Phi) C(n) sigma / n^(4/3),
+// Phi = f~ (9 pi)^(1/6) C(inf) / C(n) sqrt(sigma) / n^(7/6), C(inf) = c0 + c1
+EV p86(double r, double sig, const FParams& P) {
+  const double rs = std::cbrt(3.0 / (4.0 * M_PI * r));
+  const double num = P.p86_c1 + P.p86_alpha * rs + P.p86_beta * rs * rs;
+  const double den = 1.0 + P.p86_gamma * rs + P.p86_delta * rs * rs + 1.0e4 * P.p86_beta * rs * rs * rs;
+  const double Cn = P.p86_c0 + num / den;
+  const double dCn_drs = ((P.p86_alpha + 2.0 * P.p86_beta * rs) * den -
+                          num * (P.p86_gamma + 2.0 * P.p86_delta * rs + 3.0e4 * P.p86_beta * rs * rs)) /
+                         (den * den);
+  const double dCn = dCn_drs * (-rs / (3.0 * r));
+  const double k = P.p86_ft * std::pow(9.0 * M_PI, 1.0 / 6.0) * (P.p86_c0 + P.p86_c1);
+  const double r43 = r * std::cbrt(r);
+  const double Phi = k * std::sqrt(sig) / (Cn * std::pow(r, 7.0 / 6.0));
+  const double ex = std::exp(-Phi);
+  const double e = ex * Cn * sig / r43;
+  const double dPhi_dr = Phi * (-dCn / Cn - 7.0 / (6.0 * r));
+  return {e, e * (-dPhi_dr + dCn / Cn - 4.0 / (3.0 * r)), ex * Cn / r43 * (1.0 - 0.5 * Phi)};
+}
+
+bool is_gga(int f) { return f == 1 || f == 5 || f == 6 || f == 7 || f == 8; }
 
 EV eval(int f, double r, double sig, const FParams& P) {
   EV a{0, 0, 0}, b{0, 0, 0};
@@ -400,6 +465,11 @@ EV eval(int f, double r, double sig, const FParams& P) {
       L.mu = P.mu_libxc;
       a = pbex(r, sig, L);
       b = pbec(r, sig, P);
+    } break;
+    case 8: {
+      a = b88(r, sig, P);
+      const EV c = pz81c(r, P), d = p86(r, sig, P);
+      b = {c.e + d.e, c.dr + d.dr, c.ds + d.ds};
     } break;
     default: break;
   }
@@ -454,6 +524,7 @@ A5 aexp(const A5& a) { return un(a, std::exp(a.v), std::exp(a.v)); }
 A5 asqrt(const A5& a) { return un(a, std::sqrt(a.v), 0.5 / std::sqrt(a.v)); }
 A5 aatan(const A5& a) { return un(a, std::atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
 A5 apow(const A5& a, double p) { return un(a, std::pow(a.v, p), a.v > 0 ? p * std::pow(a.v, p - 1.0) : 0.0); }
+A5 aasinh(const A5& a) { return un(a, std::asinh(a.v), 1.0 / std::sqrt(1.0 + a.v * a.v)); }
 
 A5 vwn_g(const A5& x, double A, double b, double c, double x0) {
   // VWN fit in x = sqrt(r_s)
@@ -471,7 +542,7 @@ const double kF2 = 4.0 / (9.0 * (std::cbrt(2.0) - 1.0));  // f''(0)
 A5 fzeta(const A5& z) {
   return (apow(1.0 + z, 4.0 / 3.0) + apow(1.0 - z, 4.0 / 3.0) - 2.0) / (2.0 * std::cbrt(2.0) - 2.0);
 }
-// kind: 0 Slater, 1 TF, 2 PBE exchange with mu_xcfun, 3 PBE exchange with mu_libxc
+// kind: 0 Slater, 1 TF, 2 PBE exchange with mu_xcfun, 3 PBE exchange with mu_libxc, 4 Becke 88
 A5 x_pol(int kind, const A5& ra, const A5& rb, const A5& saa, const A5& sbb, bool aon, bool bon, const FParams& P) {
   A5 e = cst(0.0);
   for (int s = 0; s < 2; ++s) {
@@ -482,6 +553,16 @@ A5 x_pol(int kind, const A5& ra, const A5& rb, const A5& saa, const A5& sbb, boo
       e = e + (-1.5 * std::cbrt(3.0 / (4.0 * M_PI))) * apow(r, 4.0 / 3.0);
     } else if (kind == 1) {
       e = e + 0.5 * (0.3 * std::pow(3.0 * M_PI * M_PI, 2.0 / 3.0)) * apow(2.0 * r, 5.0 / 3.0);
+    } else if (kind == 4) {
+      // -C rho_s^(4/3) - beta rho_s^(4/3) x^2 / (1 + 6 beta x asinh x), x^2 = sigma_ss / rho_s^(8/3)
+      const A5 r43 = apow(r, 4.0 / 3.0);
+      const A5 u = sg / (r43 * r43);
+      A5 xas = u - u * u / 6.0;  // x asinh x for tiny x
+      if (u.v >= 1e-12) {
+        const A5 x = asqrt(u);
+        xas = x * aasinh(x);
+      }
+      e = e + (-1.5 * std::cbrt(3.0 / (4.0 * M_PI))) * r43 - P.b88_beta * (sg / r43) / (1.0 + 6.0 * P.b88_beta * xas);
     } else {
       const double mu = kind == 2 ? P.mu : P.mu_libxc;
       const A5 r2 = 2.0 * r, s4 = 4.0 * sg;
@@ -522,6 +603,25 @@ A5 pbec_pol(const A5& ra, const A5& rb, const A5& sig, const FParams& P) {
   const A5 H = gam * phi3 * aln(1.0 + (P.beta / gam) * t2 * (1.0 + A * t2) / (1.0 + A * t2 + A * A * t2 * t2));
   return n * (ec + H);
 }
+A5 pz_g(const A5& rs, const double* c) {
+  if (rs.v >= 1.0) return c[0] / (1.0 + c[1] * asqrt(rs) + c[2] * rs);
+  const A5 l = aln(rs);
+  return c[3] * l + c[4] + c[5] * rs * l + c[6] * rs;
+}
+// PZ81 with the spin interpolation f(zeta), and the P86 correction of the total density with
+// d(zeta) = 2^(1/3) sqrt(((1+zeta)/2)^(5/3) + ((1-zeta)/2)^(5/3))
+A5 bp86c_pol(const A5& ra, const A5& rb, const A5& sig, const FParams& P) {
+  const A5 n = ra + rb, z = (ra - rb) / n;
+  const A5 rs = apow(3.0 / (4.0 * M_PI * n), 1.0 / 3.0);
+  const A5 eU = pz_g(rs, P.pzU), eP = pz_g(rs, P.pzP);
+  const A5 dz = std::cbrt(2.0) * asqrt(apow(0.5 * (1.0 + z), 5.0 / 3.0) + apow(0.5 * (1.0 - z), 5.0 / 3.0));
+  const A5 Cn = P.p86_c0 + (P.p86_c1 + P.p86_alpha * rs + P.p86_beta * rs * rs) /
+                               (1.0 + P.p86_gamma * rs + P.p86_delta * rs * rs + 1.0e4 * P.p86_beta * rs * rs * rs);
+  const double k = P.p86_ft * std::pow(9.0 * M_PI, 1.0 / 6.0) * (P.p86_c0 + P.p86_c1);
+  const A5 g = sig.v > 0.0 ? asqrt(sig) : 0.0 * sig;
+  const A5 Phi = k * g / (Cn * apow(n, 7.0 / 6.0));
+  return n * (eU + fzeta(z) * (eP - eU)) + aexp(-1.0 * Phi) * Cn * sig / (dz * apow(n, 4.0 / 3.0));
+}
 A5 eval_pol(int f, const A5& ra, const A5& rb, const A5& saa, const A5& sab, const A5& sbb, bool aon, bool bon,
             const FParams& P) {
   const A5 sig = saa + 2.0 * sab + sbb;
@@ -535,6 +635,7 @@ A5 eval_pol(int f, const A5& ra, const A5& rb, const A5& saa, const A5& sab, con
     case 5: return x_pol(2, ra, rb, saa, sbb, aon, bon, P);
     case 6: return pbec_pol(rac, rbc, sig, P);
     case 7: return x_pol(3, ra, rb, saa, sbb, aon, bon, P) + pbec_pol(rac, rbc, sig, P);
+    case 8: return x_pol(4, ra, rb, saa, sbb, aon, bon, P) + bp86c_pol(rac, rbc, sig, P);
     default: return cst(0.0);
   }
 }

@@ -11,8 +11,8 @@ LIB_PATH = os.path.join(HERE, "libserenity_xc.so")
 
 XC_OK = 0
 XC_BUSY = 6
-FUNCTIONALS = {"LDA": 0, "PBE": 1, "TF": 2, "SLATER": 3, "VWN5": 4, "PBEX": 5, "PBEC": 6, "PBE_LIBXC": 7}
-GGA = {"PBE", "PBEX", "PBEC", "PBE_LIBXC"}
+FUNCTIONALS = {"LDA": 0, "PBE": 1, "TF": 2, "SLATER": 3, "VWN5": 4, "PBEX": 5, "PBEC": 6, "PBE_LIBXC": 7, "BP86": 8}
+GGA = {"PBE", "PBEX", "PBEC", "PBE_LIBXC", "BP86"}
 PRECISIONS = {"FP64": 0, "FP32": 1, "MIXED": 2}
 STAGES = ["convert", "density", "functional", "fock", "reduce", "finalize"]
 GRID_TYPES = {"BECKE": 0, "VORONOI": 1, "SSF": 2}

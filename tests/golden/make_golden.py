@@ -13,8 +13,8 @@ nothing reads the reference at test time.  What is extracted:
 * ``kat_fock.json``     -- scalar and gradient potentials on VERY_SMALL and the
   expected Fock matrix, block size 3 (src/data/grid/ScalarOperatorToMatrixAdder
   _test.cpp:40-147, tol 1e-8).
-* ``kat_funcpot_h2.json`` -- the 12x12 LDA Fock matrix of H2/def2-TZVP
-  (src/potentials/FuncPotential_test.cpp:59-100, tol 1e-6).
+* ``kat_funcpot_h2.json`` -- the 12x12 LDA and BP86 Fock matrices of H2/def2-TZVP
+  (src/potentials/FuncPotential_test.cpp:59-100, 148-232, tol 1e-6).
 * ``fixtures.npz``      -- density matrices (HDF5 datasets read with h5dump)
   and stored energies of the H2/def2-TZVP, He/def2-TZVP (PBE) and
   water/def2-TZVP (PBE) test systems, with their geometries.
@@ -111,10 +111,12 @@ def kat_fock():
 
 def kat_funcpot():
     text = read("src/potentials/FuncPotential_test.cpp")
-    body = text.split("TEST_F(FuncPotentialTest, H2_FockMatrix_LDA)")[1].split("TEST_F(")[0]
-    vals = re.search(r"Fref <<(.+?);", body, re.S).group(1)
-    F = [float(x) for x in vals.replace("\n", " ").split(",")]
-    return dict(tolerance=1e-6, functional="LDA", F=np.array(F).reshape(12, 12).tolist())
+
+    def fref(test):
+        body = text.split("TEST_F(FuncPotentialTest, %s)" % test)[1].split("TEST_F(")[0]
+        vals = re.search(r"Fref <<(.+?);", body, re.S).group(1)
+        return np.array([float(x) for x in vals.replace("\n", " ").split(",")]).reshape(12, 12).tolist()
+    return dict(tolerance=1e-6, functional="LDA", F=fref("H2_FockMatrix_LDA"), F_BP86=fref("H2_FockMatrix_GGA"))
 
 
 def kat_nadd():
@@ -176,6 +178,16 @@ def fixtures():
         out[tag + "_xyz_angstrom"] = pos
         out[tag + "_energy_ids"] = np.array(sorted(e), dtype=np.int32)
         out[tag + "_energy_values"] = np.array([e[k] for k in sorted(e)])
+    # BP86: H2 / 6-31G* (Cartesian), RI-J, Becke grid; density and Fock matrices
+    d = "TestSystem_H2_6_31Gs_BP86"
+    out["h2bp86_P"] = h5_dataset(os.path.join(base, d, d + ".dmat.res.h5"), "densityMatrix")
+    out["h2bp86_F"] = h5_dataset(os.path.join(base, d, d + ".FockMatrix.res.h5"), "FockMatrix")
+    sym, pos = xyz(os.path.join(base, d, d + ".xyz"))
+    e = energies(os.path.join(base, d, d + ".energies.res"))
+    out["h2bp86_Z"] = np.array([{"H": 1}[s] for s in sym], dtype=np.int32)
+    out["h2bp86_xyz_angstrom"] = pos
+    out["h2bp86_energy_ids"] = np.array(sorted(e), dtype=np.int32)
+    out["h2bp86_energy_values"] = np.array([e[k] for k in sorted(e)])
     # open shell: triplet O2 / 6-31G / PBE (LibXC), unrestricted density and Fock matrices
     d = "TestSystem_O2_TRIP_6_31G_PBE_TDA"
     for spin in ("alpha", "beta"):

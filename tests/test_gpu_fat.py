@@ -1,7 +1,9 @@
 """Freeze-and-thaw (serenity_amd.fde.FreezeAndThawTask) on the GPU builds.
 
-The reference's freeze-and-thaw test energies (FreezeAndThawTask_test.cpp:73-125) use BP86, whose
-XCFun constants are not pinned here, so they are not reproduced: parity unpinned for them.  The
+The reference's freeze-and-thaw test energies (FreezeAndThawTask_test.cpp:73-125, BP86 with TF on
+two H2/6-31G* subsystems) are the active subsystem's entry of Serenity's FDE energy bookkeeping with
+the embedding settings of its stored test systems; they are not reproduced: parity unpinned for them
+(BP86 itself is pinned by the reference's Fock matrix, FuncPotential_test.cpp:148-232).  The
 freeze-and-thaw cycle is pinned instead by what the mathematics fixes:
   * one subsystem: the cycle is an ordinary KS-SCF on the same grid and J engine;
   * two non-polar subsystems far apart: the non-additive terms vanish and the subsystems do not
@@ -9,9 +11,9 @@ freeze-and-thaw cycle is pinned instead by what the mathematics fixes:
     grid plus the classical interaction of their isolated densities and nuclei, all with the same
     supersystem density-fitted J (whose fitted densities carry no exact charge, so that interaction is
     not zero);
-  * water dimer: the supersystem energy of the converged densities recomputed with the oracle's
-    XC and non-additive builds equals the GPU energy, and each subsystem's density commutes with
-    its embedded Fock matrix built by the oracle (the fixed point is the oracle's too).
+  * water dimer (PBE and BP86): the supersystem energy of the converged densities recomputed with
+    the oracle's XC and non-additive builds equals the GPU energy, and each subsystem's density
+    commutes with its embedded Fock matrix built by the oracle (the fixed point is the oracle's too).
 """
 import numpy as np
 import pytest
@@ -105,10 +107,11 @@ def test_fat_far_apart_subsystems_add(ctx):
     task.close()
 
 
-def test_fat_water_dimer_against_oracle(ctx):
+@pytest.mark.parametrize("functional", ["PBE", "BP86"])
+def test_fat_water_dimer_against_oracle(ctx, functional):
     Z, X = W.water_chain(2)
     subs = [(Z[:3], X[:3]), (Z[3:], X[3:])]
-    task = fde.FreezeAndThawTask(ctx, subs, log=lambda *a: say(*a))
+    task = fde.FreezeAndThawTask(ctx, subs, functional=functional, log=lambda *a: say(*a))
     r = task.run()
     assert r["converged"]
     g = task.grid
@@ -119,11 +122,11 @@ def test_fat_water_dimer_against_oracle(ctx):
     worst_comm = 0.0
     for i, (b, P) in enumerate(zip(task.bases, task.P)):
         e += float(np.vdot(P, task.h[i]))
-        Fo, Eo, _, _ = O.fock_build(b, g.points, g.weights, P, "PBE")
+        Fo, Eo, _, _ = O.fock_build(b, g.points, g.weights, P, functional)
         e += Eo
         env_b = [task.bases[j] for j in range(2) if j != i]
         env_P = [task.P[j] for j in range(2) if j != i]
-        Fx, Ex = O.nadd_fock_build(b, env_b, g.points, g.weights, P, env_P, "PBE")
+        Fx, Ex = O.nadd_fock_build(b, env_b, g.points, g.weights, P, env_P, functional)
         Ft, Et = O.nadd_fock_build(b, env_b, g.points, g.weights, P, env_P, "TF")
         if i == 0:
             e += Ex + Et
@@ -132,8 +135,8 @@ def test_fat_water_dimer_against_oracle(ctx):
         S = task.S[i]
         comm = F @ P @ S - S @ P @ F
         worst_comm = max(worst_comm, float(np.abs(comm).max()))
-    say("water dimer FaT: E %.12f (oracle recomputation %.12f, |dE| %.2e), %d cycles, %d SCF cycles; "
-        "max |FPS - SPF| (oracle F) %.2e" % (r["energy"], e, abs(e - r["energy"]), r["cycles"], r["scf_cycles"],
+    say("water dimer FaT (%s): E %.12f (oracle recomputation %.12f, |dE| %.2e), %d cycles, %d SCF cycles; "
+        "max |FPS - SPF| (oracle F) %.2e" % (functional, r["energy"], e, abs(e - r["energy"]), r["cycles"], r["scf_cycles"],
                                               worst_comm))
     assert abs(e - r["energy"]) < 1e-9
     assert worst_comm < 1e-5

@@ -89,7 +89,7 @@ def test_fock_known_answers(ctx):
     assert np.abs(F2 - ref2).max() < 1e-15
 
 
-@pytest.mark.parametrize("name", ["LDA", "PBE", "TF", "SLATER", "VWN5", "PBEX", "PBEC"])
+@pytest.mark.parametrize("name", ["LDA", "PBE", "TF", "SLATER", "VWN5", "PBEX", "PBEC", "BP86"])
 def test_functional_matches_oracle(ctx, name):
     rng = np.random.default_rng(7)
     N = 128 * 37 + 5
@@ -216,6 +216,26 @@ def test_h2_lda_fock_matrix(ctx):
     F = xc.FuncPotential(bf, "LDA").getMatrix(fx["h2_P"])
     Fref = np.array(common.load_json("kat_funcpot_h2.json")["F"])
     assert np.abs(F - Fref).max() < 1e-8
+
+
+def test_h2_bp86_fock_matrix(ctx):
+    """FuncPotential_test.cpp:148-232 (BP86) through the HIP path, FP64 and FP32 contractions."""
+    fx = common.fixtures()
+    Z, X = fx["h2_Z"], fx["h2_xyz_angstrom"] * xc.ANGSTROM_TO_BOHR
+    b = xc.BasisController.from_file("DEF2-TZVP", Z, X, spherical=False)
+    g = xc.molecular_grid(Z, X, grid_type="BECKE", accuracy=4, weight_threshold=1e-10, smoothing=3)
+    bf = xc.BasisFunctionOnGridController(ctx, b, g, 128, 1e-9, 1, keep_fp32=True)
+    Fref = np.array(common.load_json("kat_funcpot_h2.json")["F_BP86"])
+    Fo, Eo, _, _ = O.fock_build(b, g.points, g.weights, fx["h2_P"], "BP86")
+    pot = xc.FuncPotential(bf, "BP86")
+    F = pot.getMatrix(fx["h2_P"])
+    say("H2 BP86: max|F - Fref| %.2e, vs oracle %.2e, |dE| %.2e" %
+        (np.abs(F - Fref).max(), np.abs(F - Fo).max(), abs(pot.getEnergy() - Eo)))
+    assert np.abs(F - Fref).max() < 1e-7
+    assert np.abs(F - Fo).max() < 1e-12 and abs(pot.getEnergy() - Eo) < 1e-12
+    F32 = xc.FuncPotential(bf, "BP86", "FP32").getMatrix(fx["h2_P"])
+    assert np.abs(F32 - Fref).max() < 1e-6
+    bf._release()
 
 
 def test_pbe_energy_fixtures(ctx):

@@ -37,7 +37,7 @@ def test_o2_triplet_matches_oracle_and_stored_fock(ctx):
     bf._release()
 
 
-@pytest.mark.parametrize("functional", ["PBE", "LDA"])
+@pytest.mark.parametrize("functional", ["PBE", "LDA", "BP86"])
 def test_open_shell_alkane_matches_oracle(ctx, functional):
     """C20H42/def2-SVP with 81 alpha and 80 beta synthetic orbitals: FP64 vs oracle, FP32 vs FP64."""
     d = W.build_system(2)

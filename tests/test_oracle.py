@@ -74,6 +74,17 @@ def test_h2_lda_fock_matrix():
     assert abs(rho @ grid.weights - 2.0) < 1e-6
 
 
+def test_h2_bp86_fock_matrix():
+    """FuncPotential_test.cpp:148-232: the stored 12x12 BP86 Fock matrix of H2/def2-TZVP (tolerance
+    1e-6).  It fixes the P86 prefactor f~ (9 pi)^(1/6) with the original PZ81 constants; the oracle's
+    closed forms (hand-derived derivatives) agree to 1e-7."""
+    basis, grid, P = h2_system()
+    F, E, _, _ = O.fock_build(basis, grid.points, grid.weights, P, "BP86")
+    Fref = np.array(common.load_json("kat_funcpot_h2.json")["F_BP86"])
+    assert np.abs(F - Fref).max() < 1e-7
+    assert np.array_equal(F, F.T)
+
+
 @pytest.mark.parametrize("tag,exc,tol", [("he", -1.0490748073, 1e-6), ("water", -9.2488924049, 5e-6)])
 def test_pbe_exchange_correlation_energy(tag, exc, tol):
     """Stored E_xc (energy id 202) of TestSystem_He/Water_Def2-TZVP_DFT.  The stored value was
@@ -117,7 +128,7 @@ def _fd_check(name, rho, sigma):
         assert np.all(np.abs(g[0] / (2 * gx) - dsig) <= tol), (name, g[0] / (2 * gx), dsig)
 
 
-@pytest.mark.parametrize("name", ["LDA", "PBE", "TF", "SLATER", "VWN5", "PBEX", "PBEC"])
+@pytest.mark.parametrize("name", ["LDA", "PBE", "TF", "SLATER", "VWN5", "PBEX", "PBEC", "BP86"])
 @pytest.mark.parametrize("sigma", [0.0, 1e-3, 0.3])
 def test_functional_derivatives_finite_difference(name, sigma):
     rho = np.array([1e-4, 3e-3, 0.05, 0.7, 12.0])

@@ -33,7 +33,7 @@ def _pts(seed, n=40):
     return ra, rb, ga, gb
 
 
-@pytest.mark.parametrize("name", ["LDA", "PBE", "TF", "SLATER", "VWN5", "PBEX", "PBEC", "PBE_LIBXC"])
+@pytest.mark.parametrize("name", ["LDA", "PBE", "TF", "SLATER", "VWN5", "PBEX", "PBEC", "PBE_LIBXC", "BP86"])
 def test_polarised_derivatives_by_finite_differences(name):
     ra, rb, ga, gb = _pts(7)
     w = np.ones_like(ra)
@@ -68,7 +68,7 @@ def test_polarised_derivatives_by_finite_differences(name):
     assert worst < 1e-5, worst  # rounding of the differences; a wrong term is O(1)
 
 
-@pytest.mark.parametrize("name", ["LDA", "PBE", "TF", "PBE_LIBXC"])
+@pytest.mark.parametrize("name", ["LDA", "PBE", "TF", "PBE_LIBXC", "BP86"])
 def test_unpolarised_case_equals_restricted(name):
     ra, _, ga, _ = _pts(3)
     w = np.ones_like(ra)
