@@ -24,19 +24,23 @@ namespace {
 
 /* Work items of one block are consecutive in a list.  Workgroups of a launch are
  * dealt to the 8 XCDs round-robin (workgroup w runs on XCD w mod 8, from some
- * starting XCD), each with its own L2.  Deal whole blocks to 8 queues of balanced
- * length and interleave them, so that one XCD runs a block's items one after
- * another and its re-reads of the block's tiles hit that XCD's L2. */
+ * starting XCD), each with its own L2.  Cut the list into 8 contiguous ranges of
+ * whole blocks (about a fair share of items each; the blocks are in grid order,
+ * so a range is a region of the molecule) and interleave them: one XCD runs a
+ * block's items one after another, its re-reads of the block's tiles hit its L2,
+ * and the density-matrix rows its region reads stay in that L2 too (density stage
+ * at (H2O)64/def2-TZVP: 4.46 -> 4.36 ms FP32 against dealing blocks to the
+ * shortest queue). */
 template <typename Item>
 void xcd_order(std::vector<Item>& v) {
   constexpr int kXcd = 8;
   std::vector<std::vector<Item>> q(kXcd);
+  const size_t target = (v.size() + kXcd - 1) / kXcd;
+  size_t x = 0;
   for (size_t i = 0; i < v.size();) {
     size_t j = i;
     while (j < v.size() && v[j].block == v[i].block) ++j;
-    size_t x = 0;
-    for (size_t y = 1; y < q.size(); ++y)
-      if (q[y].size() < q[x].size()) x = y;
+    if (q[x].size() >= target && x + 1 < q.size()) ++x;
     q[x].insert(q[x].end(), v.begin() + i, v.begin() + j);
     i = j;
   }
