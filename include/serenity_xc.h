@@ -79,7 +79,9 @@ int xc_ctx_destroy(xc_ctx* ctx);
 int xc_ctx_synchronize(xc_ctx* ctx);
 /* The hipStream_t every kernel of this context is launched on. */
 void* xc_ctx_stream(xc_ctx* ctx);
-/* Device bytes currently held by handles of this context. */
+/* Device bytes currently held by handles of this context (in 2 MiB steps for blocks of
+ * 1 MiB and more).  Released blocks stay with the context for reuse and return to the
+ * device when an allocation fails and when the context is destroyed. */
 int xc_ctx_memory(xc_ctx* ctx, int64_t* bytes_in_use);
 
 /* ---- host-side helpers (no device work) ------------------------------- */

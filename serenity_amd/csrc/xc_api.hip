@@ -12,6 +12,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/serenity_xc.h"
@@ -195,19 +196,68 @@ struct xc_ctx {
     if (profile) HIPX(hipEventRecord(next_event(), stream));
   }
 
+  // Device blocks are kept for reuse when released: a serving loop builds and evicts controllers
+  // again and again, and hipMalloc / hipFree of their multi-GB buffers cost milliseconds each and
+  // now and then stall.  Blocks are sized in 2 MiB steps (256 bytes below 1 MiB); an allocation
+  // takes the smallest cached block that fits within 1/8 + 2 MiB of its size, else a new one.
+  // The cache is given back to the device when a hipMalloc fails and when the context ends.
+  // `bytes` counts the blocks in use.
+  std::multimap<size_t, void*> cache;            // block size -> released block
+  std::unordered_map<void*, size_t> block_size;  // every block of this context -> size
+  size_t cached = 0;
+  static size_t block_round(size_t b) {
+    constexpr size_t kBig = size_t(1) << 20, kStep = size_t(2) << 20;
+    return b >= kBig ? (b + kStep - 1) / kStep * kStep : (b + 255) / 256 * 256;
+  }
+  void trim() {
+    (void)hipDeviceSynchronize();
+    for (auto& kv : cache) {
+      (void)hipFree(kv.second);
+      block_size.erase(kv.second);
+    }
+    cache.clear();
+    cached = 0;
+  }
   void* alloc(size_t bytes_) {
     if (bytes_ == 0) return nullptr;
-    void* p = nullptr;
     HIPX(hipSetDevice(device));
-    hipError_t e = hipMalloc(&p, bytes_);
-    if (e != hipSuccess) throw XcError(XC_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes_) + " bytes failed");
-    bytes += (int64_t)bytes_;
+    const size_t need = block_round(bytes_);
+    auto it = cache.lower_bound(need);
+    if (it != cache.end() && it->first <= need + need / 8 + (size_t(2) << 20)) {
+      void* p = it->second;
+      const size_t sz = it->first;
+      cache.erase(it);
+      cached -= sz;
+      bytes += (int64_t)sz;
+      return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess && cached) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, need);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw XcError(XC_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes_) + " bytes failed");
+    }
+    block_size[p] = need;
+    bytes += (int64_t)need;
     return p;
   }
-  void release(void* p, size_t bytes_) {
+  void release(void* p, size_t) {
     if (!p) return;
-    (void)hipFree(p);
-    bytes -= (int64_t)bytes_;
+    auto it = block_size.find(p);
+    if (it == block_size.end()) {
+      (void)hipFree(p);
+      return;
+    }
+    // enqueued work may still use the block: wait for the device, as hipFree does
+    (void)hipDeviceSynchronize();
+    bytes -= (int64_t)it->second;
+    cache.emplace(it->second, p);
+    cached += it->second;
   }
 };
 
@@ -522,6 +572,7 @@ int xc_ctx_destroy(xc_ctx* ctx) {
     ctx->pack_graphs.clear();
     ctx->lanes.destroy();
     (void)hipStreamDestroy(ctx->stream);
+    ctx->trim();
     delete ctx;
   });
 }
@@ -1038,9 +1089,10 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
           }
         }
       }
-      // 3. admission by footprint
+      // 3. admission by footprint (the context's cached blocks count as free)
       size_t free_b = 0, total_b = 0;
       HIPX(hipMemGetInfo(&free_b, &total_b));
+      free_b += ctx->cached;
       const double need =
           (double)tile_off * ((keep_fp32 != 2 ? sizeof(double) : 0) + (keep_fp32 ? sizeof(float) : 0)) +
           (double)bf->t_elems * (keep_fp32 != 2 ? sizeof(double) : sizeof(float));
