@@ -874,7 +874,8 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
 // U[j + n i], j >= i.  Each lane takes an aligned group of four columns (256 per pass of the
 // wavefront): one 16-byte load of their AO indices and one of their stored values (a block's AO
 // list is padded to a multiple of 32 with -1 and a sub-tile row holds 16 values, so an aligned
-// group never leaves either; the columns of the group before the segment start are masked).  Rows longer than the LDS window are done in
+// group never leaves either; the columns of the group before the segment start are masked).
+// The loads of a wavefront's next entry are issued before the current entry's LDS additions.  Rows longer than the LDS window are done in
 // several passes over the entries.  The LDS additions of different wavefronts meet in an order
 // that varies from run to run, so F varies at the rounding level.
 template <typename T>
@@ -903,29 +904,58 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
     const int wl = min(window, n - i - w0);
     for (int k = tid; k < wl; k += 64 * kReduceWaves) racc[k] = 0.0;
     __syncthreads();
-    for (int e = e0 + w; e < e1; e += kReduceWaves) {
-      const RowEnt E = ent[e];
-      const int* idx = ao_idx + E.idx_off;
-      const T* trow = Tscr + E.rowbase;
-      const int I = E.c0 >> 4;
-      for (int c = (E.c0 & ~3) + 4 * lane; c < E.c1; c += 256) {
-        const int4 gi = *reinterpret_cast<const int4*>(idx + c);
-        const T* src = trow + (int64_t)((c >> 4) - I) * 256 + (c & 15);
-        double v[4];
-        if constexpr (sizeof(T) == 4) {
-          const float4 f = *reinterpret_cast<const float4*>(src);
-          v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
-        } else {
-          const double2 f0 = *reinterpret_cast<const double2*>(src);
-          const double2 f1 = *reinterpret_cast<const double2*>(src + 2);
-          v[0] = f0.x; v[1] = f0.y; v[2] = f1.x; v[3] = f1.y;
-        }
-        const int gg[4] = {gi.x, gi.y, gi.z, gi.w};
+    auto fetch = [&](const RowEnt& E, int c, int4& gi, double (&v)[4]) {
+      gi = *reinterpret_cast<const int4*>(ao_idx + E.idx_off + c);
+      const T* src = Tscr + E.rowbase + (int64_t)((c >> 4) - (E.c0 >> 4)) * 256 + (c & 15);
+      if constexpr (sizeof(T) == 4) {
+        const float4 f = *reinterpret_cast<const float4*>(src);
+        v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+      } else {
+        const double2 f0 = *reinterpret_cast<const double2*>(src);
+        const double2 f1 = *reinterpret_cast<const double2*>(src + 2);
+        v[0] = f0.x; v[1] = f0.y; v[2] = f1.x; v[3] = f1.y;
+      }
+    };
+    auto add = [&](const RowEnt& E, int c, const int4& gi, const double (&v)[4]) {
+      const int gg[4] = {gi.x, gi.y, gi.z, gi.w};
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int g = gg[u] - i - w0;
-          if (c + u >= E.c0 && c + u < E.c1 && g >= 0 && g < wl) atomicAdd(racc + g, v[u]);
+      for (int u = 0; u < 4; ++u) {
+        const int g = gg[u] - i - w0;
+        if (c + u >= E.c0 && c + u < E.c1 && g >= 0 && g < wl) atomicAdd(racc + g, v[u]);
+      }
+    };
+    int e = e0 + w;
+    if (e < e1) {
+      RowEnt E = ent[e];
+      int c = (E.c0 & ~3) + 4 * lane;
+      int4 gi = make_int4(-1, -1, -1, -1);
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      if (c < E.c1) fetch(E, c, gi, v);
+      for (;;) {
+        const int en = e + kReduceWaves;
+        RowEnt En = E;
+        int cn = 0;
+        int4 gn = make_int4(-1, -1, -1, -1);
+        double vn[4] = {0.0, 0.0, 0.0, 0.0};
+        if (en < e1) {  // wavefront-uniform: the next entry's loads before this entry's additions
+          En = ent[en];
+          cn = (En.c0 & ~3) + 4 * lane;
+          if (cn < En.c1) fetch(En, cn, gn, vn);
         }
+        if (c < E.c1) add(E, c, gi, v);
+        for (int c2 = c + 256; c2 < E.c1; c2 += 256) {
+          int4 g2;
+          double v2[4];
+          fetch(E, c2, g2, v2);
+          add(E, c2, g2, v2);
+        }
+        if (en >= e1) break;
+        e = en;
+        E = En;
+        c = cn;
+        gi = gn;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = vn[u];
       }
     }
     __syncthreads();
