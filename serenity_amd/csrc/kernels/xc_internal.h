@@ -20,6 +20,37 @@
 
 namespace sxc {
 
+// Global-memory view of a pointer.  A pointer the compiler cannot trace to a kernel argument (one
+// taken from a packed build's per-system table, or selected between such a pointer and an
+// argument) is generic to it, and accesses through a generic pointer become flat instructions:
+// those count against the LDS counter as well as the vector-memory one, so every wait for an LDS
+// read then also waits for all global loads in flight.  Accesses through gmem<T>* are global ones.
+template <typename T>
+using gmem = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ gmem<T>* to_global(T* p) {
+  return (gmem<T>*)p;
+}
+// Vector types for 16-byte global loads through gmem pointers (HIP's float4 and the like are
+// classes whose copy constructors take generic references).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+// A struct read through a gmem pointer (structs, too, copy only from generic references), in
+// 8-byte words.
+template <typename S>
+__device__ __forceinline__ S load_global(const gmem<const S>* p) {
+  static_assert(sizeof(S) % 8 == 0 && alignof(S) >= 8, "8-byte words");
+  S out;
+  const gmem<const int64_t>* src = reinterpret_cast<const gmem<const int64_t>*>(p);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(S) / 8); ++k) {
+    const int64_t w = src[k];
+    __builtin_memcpy(reinterpret_cast<char*>(&out) + 8 * k, &w, 8);
+  }
+  return out;
+}
+
 constexpr int kTile = 128;      // points per tile row (max block size)
 constexpr int kAoPad = 32;      // AO padding granularity of a block
 constexpr int kMaxL = 6;        // highest angular momentum the AO kernel evaluates

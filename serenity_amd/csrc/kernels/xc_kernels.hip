@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
+#include <type_traits>
 
 #include "mfma.h"
 #include "xc_internal.h"
@@ -180,8 +181,8 @@ __global__ __launch_bounds__(128) void ao_kernel(const DevBlock* __restrict__ bl
 
 // The four FP32 tile values of a lane's MFMA accumulator rows: points 4 (lane / 16) + e,
 // e = 0..3, of a 16-point row segment -- one aligned 16-byte load.
-__device__ inline void quad_rows(const float* seg, int lane, double f[4]) {
-  const float4 v = *reinterpret_cast<const float4*>(seg + 4 * (lane >> 4));
+__device__ inline void quad_rows(const gmem<const float>* seg, int lane, double f[4]) {
+  const f32x4 v = *reinterpret_cast<const gmem<const f32x4>*>(seg + 4 * (lane >> 4));
   f[0] = v.x;
   f[1] = v.y;
   f[2] = v.z;
@@ -241,8 +242,9 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int r = lane & 15, g = lane >> 4;
   const int npad = bk.npad;
-  const T* tb = tiles + bk.tile_off;
-  const int* idx = ao_idx + bk.idx_off;
+  const gmem<const T>* tb = to_global(tiles + bk.tile_off);
+  const gmem<const int>* idx = to_global(ao_idx + bk.idx_off);
+  const gmem<const T>* Pg = to_global(P);
   const int64_t cstride = (int64_t)npad * kTile;
   // the block's AO index list in LDS (P rows of a chunk are then gathered without a
   // dependent global load); wider blocks read it from global memory
@@ -250,76 +252,103 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
   if (idx_lds)
     for (int i = tid; i < npad; i += 512) sIdx[i] = idx[i];
   __syncthreads();
-  const int* lidx = idx_lds ? sIdx : idx;
 
   // staging: phi task = (point pa, AO quad qa); P tasks = (column cb, AO quad qb), 16
   // consecutive lanes taking 16 columns of one quad (their LDS stores then cover distinct banks)
   const int pa = tid & (kTile - 1), qa = tid >> 7;
   const int qb = (tid >> 4) & 3;
   auto col_of = [](int task) { return (task & 15) + 16 * (task >> 6); };
-  int nu[NB];
-#pragma unroll
-  for (int m = 0; m < NB; ++m) {
-    const int task = tid + 512 * m, cb = col_of(task);
-    nu[m] = (task < 4 * NC && cb < ncols) ? lidx[col0 + cb] : -1;
-  }
-  vec_t rA, rB[NB];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int e = 0; e < KV; ++e) rA[e] = tb[(int64_t)(k0 + KV * qa + e) * kTile + pa];
-#pragma unroll
-    for (int m = 0; m < NB; ++m) {
-#pragma unroll
-      for (int e = 0; e < KV; ++e) {
-        const int mu = lidx[k0 + KV * qb + e];
-        // P is symmetric: read P[nu][mu] so that the 16 lanes of a group (consecutive columns
-        // nu, one mu) read consecutive addresses
-        rB[m][e] = (mu >= 0 && nu[m] >= 0) ? P[nu[m] + (int64_t)n * mu] : T(0);
-      }
-    }
-  };
-  auto store = [&]() {
-    *reinterpret_cast<vec_t*>(&sA[pa][KV * qa]) = rA;
-#pragma unroll
-    for (int m = 0; m < NB; ++m) {
-      const int task = tid + 512 * m;
-      if (task < 4 * NC) *reinterpret_cast<vec_t*>(&sB[col_of(task)][KV * qb]) = rB[m];
-    }
-  };
-
   typename M::acc_t acc[S];
 #pragma unroll
   for (int c = 0; c < S; ++c) acc[c] = M::zero();
-  // one LDS buffer; the global loads of chunk k + 1 are in flight during the MFMAs of chunk k
-  const int nch = npad / KC;
-  load(0);
-  for (int ch = 0; ch < nch; ++ch) {
-    store();
-    __syncthreads();
-    if (ch + 1 < nch) load((ch + 1) * KC);
-    const vec_t av = *reinterpret_cast<const vec_t*>(&sA[16 * w + r][KV * g]);
+  // The chunk loop, with the AO indices read from LDS (ds_read: no wait on the global loads in
+  // flight) or, for blocks wider than the LDS list, from global memory.
+  auto chunks = [&](auto idx_in_lds) {
+    auto ix = [&](int i) -> int {
+      if constexpr (decltype(idx_in_lds)::value)
+        return sIdx[i];
+      else
+        return idx[i];
+    };
+    int nu[NB];
 #pragma unroll
-    for (int s0 = 0; s0 < S; s0 += GS) {
-      vec_t bv[GS];
+    for (int m = 0; m < NB; ++m) {
+      const int task = tid + 512 * m, cb = col_of(task);
+      nu[m] = (task < 4 * NC && cb < ncols) ? ix(col0 + cb) : -1;
+    }
+    vec_t rA, rB[NB];
+    unsigned mu_ok = 0;  // bit e: AO KV qb + e of the chunk is not padding
+    auto load = [&](int k0) {
 #pragma unroll
-      for (int u = 0; u < GS; ++u)
-        if (s0 + u < S) bv[u] = *reinterpret_cast<const vec_t*>(&sB[16 * (s0 + u) + r][KV * g]);
+      for (int e = 0; e < KV; ++e) rA[e] = tb[(int64_t)(k0 + KV * qa + e) * kTile + pa];
+      int mu[KV];
+      mu_ok = 0;
 #pragma unroll
-      for (int t = 0; t < KV; ++t)
+      for (int e = 0; e < KV; ++e) {
+        mu[e] = ix(k0 + KV * qb + e);
+        mu_ok |= (unsigned)(mu[e] >= 0) << e;
+      }
+#pragma unroll
+      for (int m = 0; m < NB; ++m) {
+        if (tid + 512 * m < 4 * NC) {  // whole wavefronts
+          // P is symmetric: read P[nu][mu] so that the 16 lanes of a group (consecutive
+          // columns nu, one mu) read consecutive addresses.  Padding (index -1) reads P[0][0]
+          // and is zeroed at the LDS store by a multiply: a select at the load would let the
+          // compiler move the load into a branch, and a multiply there would wait for it
+          // before the chunk's MFMAs.
+#pragma unroll
+          for (int e = 0; e < KV; ++e) rB[m][e] = Pg[max(nu[m], 0) + (int64_t)n * max(mu[e], 0)];
+        }
+      }
+    };
+    auto store = [&]() {
+      *reinterpret_cast<vec_t*>(&sA[pa][KV * qa]) = rA;
+#pragma unroll
+      for (int m = 0; m < NB; ++m) {
+        const int task = tid + 512 * m;
+        if (task < 4 * NC) {
+          vec_t v = rB[m];
+#pragma unroll
+          for (int e = 0; e < KV; ++e) v[e] *= T(((mu_ok >> e) & 1u) && nu[m] >= 0);
+          *reinterpret_cast<vec_t*>(&sB[col_of(task)][KV * qb]) = v;
+        }
+      }
+    };
+    // one LDS buffer; the global loads of chunk k + 1 are in flight during the MFMAs of chunk k
+    const int nch = npad / KC;
+    load(0);
+    for (int ch = 0; ch < nch; ++ch) {
+      store();
+      __syncthreads();
+      if (ch + 1 < nch) load((ch + 1) * KC);
+      const vec_t av = *reinterpret_cast<const vec_t*>(&sA[16 * w + r][KV * g]);
+#pragma unroll
+      for (int s0 = 0; s0 < S; s0 += GS) {
+        vec_t bv[GS];
 #pragma unroll
         for (int u = 0; u < GS; ++u)
-          if (s0 + u < S) {
-            // FP64 accumulates T transposed (rows: AO columns, columns: points), so that a
-            // lane's entries are one point of four AO columns in the dot phase
-            if constexpr (sizeof(T) == 8)
-              acc[s0 + u] = M::step(bv[u][t], av[t], acc[s0 + u]);
-            else
-              acc[s0 + u] = M::step(av[t], bv[u][t], acc[s0 + u]);
-          }
-      __builtin_amdgcn_sched_barrier(0);
+          if (s0 + u < S) bv[u] = *reinterpret_cast<const vec_t*>(&sB[16 * (s0 + u) + r][KV * g]);
+#pragma unroll
+        for (int t = 0; t < KV; ++t)
+#pragma unroll
+          for (int u = 0; u < GS; ++u)
+            if (s0 + u < S) {
+              // FP64 accumulates T transposed (rows: AO columns, columns: points), so that a
+              // lane's entries are one point of four AO columns in the dot phase
+              if constexpr (sizeof(T) == 8)
+                acc[s0 + u] = M::step(bv[u][t], av[t], acc[s0 + u]);
+              else
+                acc[s0 + u] = M::step(av[t], bv[u][t], acc[s0 + u]);
+            }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      __syncthreads();
     }
-    __syncthreads();
-  }
+  };
+  if (idx_lds)
+    chunks(std::true_type{});
+  else
+    chunks(std::false_type{});
 
   if constexpr (sizeof(T) == 4) {
     // Dots straight from the accumulators: lane (r, g) holds T[row(lane, e)][16 s + r] of its
@@ -331,7 +360,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
 #pragma unroll
     for (int c = 0; c < S; ++c) {
       if (16 * c < ncols) {
-        const T* row = tb + (int64_t)(col0 + 16 * c + r) * kTile + 16 * w;
+        const gmem<const T>* row = tb + (int64_t)(col0 + 16 * c + r) * kTile + 16 * w;
         double f[4];
         quad_rows(row, lane, f);
 #pragma unroll
@@ -368,7 +397,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
       for (int e = 0; e < 4; ++e) {
         const int p = 16 * w + M::row(lane, e);
         if (p < bk.npts) {
-          double* dst = part + (int64_t)wk.pass * 4 * npts + bk.first + p;
+          gmem<double>* dst = to_global(part + (int64_t)wk.pass * 4 * npts + bk.first + p);
           dst[0] = rv[e];
           if (want_grad) {
             dst[npts] = xv[e];
@@ -387,7 +416,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
       if (16 * c < ncols) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const T* row = tb + (int64_t)(col0 + 16 * c + g + 4 * e) * kTile + 16 * w + r;
+          const gmem<const T>* row = tb + (int64_t)(col0 + 16 * c + g + 4 * e) * kTile + 16 * w + r;
           const double tv = acc[c][e];
           rv += tv * row[0];
           if (want_grad) {
@@ -411,7 +440,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
     }
     const int p = 16 * w + r;
     if (g == 0 && p < bk.npts) {
-      double* dst = part + (int64_t)wk.pass * 4 * npts + bk.first + p;
+      gmem<double>* dst = to_global(part + (int64_t)wk.pass * 4 * npts + bk.first + p);
       dst[0] = rv;
       if (want_grad) {
         dst[npts] = xv;
@@ -687,7 +716,7 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   const int npad = bk.npad;
   const bool diag = it.nb == 0;
   const int R = it.na + it.nb;
-  const T* tb = tiles + bk.tile_off;
+  const gmem<const T>* tb = to_global(tiles + bk.tile_off);
   const int64_t cstride = (int64_t)npad * kTile;
   vec_t raw[NTASK][4];
   auto grow_of = [&](int lr) { return lr < it.na ? it.a0 + lr : it.b0 + (lr - it.na); };
@@ -701,12 +730,12 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     for (int m = 0; m < NTASK; ++m) {
       const int task = tid + NT * m;
       if (task < 4 * R) {
-        const T* src = tb + (int64_t)grow_of(row_of(task)) * kTile + p0 + KV * q;
-        raw[m][0] = *reinterpret_cast<const vec_t*>(src);
+        const gmem<const T>* src = tb + (int64_t)grow_of(row_of(task)) * kTile + p0 + KV * q;
+        raw[m][0] = *reinterpret_cast<const gmem<const vec_t>*>(src);
         if (gga) {
-          raw[m][1] = *reinterpret_cast<const vec_t*>(src + cstride);
-          raw[m][2] = *reinterpret_cast<const vec_t*>(src + 2 * cstride);
-          raw[m][3] = *reinterpret_cast<const vec_t*>(src + 3 * cstride);
+          raw[m][1] = *reinterpret_cast<const gmem<const vec_t>*>(src + cstride);
+          raw[m][2] = *reinterpret_cast<const gmem<const vec_t>*>(src + 2 * cstride);
+          raw[m][3] = *reinterpret_cast<const gmem<const vec_t>*>(src + 3 * cstride);
         }
       }
     }
@@ -715,10 +744,12 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   if (tid < kTile) {
     const bool in = tid < bk.npts;
     const int64_t q = bk.first + tid;
-    const double v0 = in ? vs[q] : 0.0;
-    const double v1 = (in && gga) ? gw[q] : 0.0;
-    const double v2 = (in && gga) ? gw[npts + q] : 0.0;
-    const double v3 = (in && gga) ? gw[2 * npts + q] : 0.0;
+    const gmem<const double>* vsg = to_global(vs);
+    const gmem<const double>* gwg = to_global(gw);
+    const double v0 = in ? vsg[q] : 0.0;
+    const double v1 = (in && gga) ? gwg[q] : 0.0;
+    const double v2 = (in && gga) ? gwg[npts + q] : 0.0;
+    const double v3 = (in && gga) ? gwg[2 * npts + q] : 0.0;
     spot[0][tid] = 0.5 * v0;
     spot[1][tid] = v1;
     spot[2][tid] = v2;
@@ -752,12 +783,12 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   // in its sub-tiles, so the reduce needs no per-block test
   if ((red[0] + red[1]) / bk.npts < thr) {
     const int nt0 = npad / 16;
-    T* out0 = Tscr + bk.toff;
+    gmem<T>* out0 = to_global(Tscr + bk.toff);
     for (int t = w; t < it.nsub; t += kFockWaves) {
       int i, j;
       decode(t, i, j);
       const int I = it.a0 / 16 + i, J = (diag ? it.a0 : it.b0) / 16 + j;
-      T* sub = out0 + fock_subtile_index(I, J, nt0) * 256;
+      gmem<T>* sub = out0 + fock_subtile_index(I, J, nt0) * 256;
       for (int e = lane; e < 256; e += 64) sub[e] = T(0);
     }
     return;
@@ -854,7 +885,7 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
 
   // store the sub-tiles (block sub-tile (I, J), I <= J; 16 lanes write 16 consecutive values of a row)
   const int nt = npad / 16;
-  T* tb_out = Tscr + bk.toff;
+  gmem<T>* tb_out = to_global(Tscr + bk.toff);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int t = w + kFockWaves * s;
@@ -862,7 +893,7 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     int i, j;
     decode(t, i, j);
     const int I = it.a0 / 16 + i, J = (diag ? it.a0 : it.b0) / 16 + j;
-    T* sub = tb_out + fock_subtile_index(I, J, nt) * 256;
+    gmem<T>* sub = tb_out + fock_subtile_index(I, J, nt) * 256;
 #pragma unroll
     for (int e = 0; e < 4; ++e) sub[M::row(lane, e) * 16 + r] = acc[s][e];
   }
@@ -899,24 +930,27 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
   if (i >= n) return;  // uniform over the workgroup
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
+  const gmem<const RowEnt>* entg = to_global(ent);
+  const gmem<const T>* Tg = to_global(Tscr);
+  const gmem<const int>* idxg = to_global(ao_idx);
+  const int e0 = to_global(row_ptr)[i], e1 = to_global(row_ptr)[i + 1];
   for (int w0 = 0; w0 < n - i; w0 += window) {
     const int wl = min(window, n - i - w0);
     for (int k = tid; k < wl; k += 64 * kReduceWaves) racc[k] = 0.0;
     __syncthreads();
-    auto fetch = [&](const RowEnt& E, int c, int4& gi, double (&v)[4]) {
-      gi = *reinterpret_cast<const int4*>(ao_idx + E.idx_off + c);
-      const T* src = Tscr + E.rowbase + (int64_t)((c >> 4) - (E.c0 >> 4)) * 256 + (c & 15);
+    auto fetch = [&](const RowEnt& E, int c, i32x4& gi, double (&v)[4]) {
+      gi = *reinterpret_cast<const gmem<const i32x4>*>(idxg + E.idx_off + c);
+      const gmem<const T>* src = Tg + E.rowbase + (int64_t)((c >> 4) - (E.c0 >> 4)) * 256 + (c & 15);
       if constexpr (sizeof(T) == 4) {
-        const float4 f = *reinterpret_cast<const float4*>(src);
+        const f32x4 f = *reinterpret_cast<const gmem<const f32x4>*>(src);
         v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
       } else {
-        const double2 f0 = *reinterpret_cast<const double2*>(src);
-        const double2 f1 = *reinterpret_cast<const double2*>(src + 2);
+        const f64x2 f0 = *reinterpret_cast<const gmem<const f64x2>*>(src);
+        const f64x2 f1 = *reinterpret_cast<const gmem<const f64x2>*>(src + 2);
         v[0] = f0.x; v[1] = f0.y; v[2] = f1.x; v[3] = f1.y;
       }
     };
-    auto add = [&](const RowEnt& E, int c, const int4& gi, const double (&v)[4]) {
+    auto add = [&](const RowEnt& E, int c, const i32x4& gi, const double (&v)[4]) {
       const int gg[4] = {gi.x, gi.y, gi.z, gi.w};
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -926,25 +960,25 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
     };
     int e = e0 + w;
     if (e < e1) {
-      RowEnt E = ent[e];
+      RowEnt E = load_global(entg + e);
       int c = (E.c0 & ~3) + 4 * lane;
-      int4 gi = make_int4(-1, -1, -1, -1);
+      i32x4 gi = {-1, -1, -1, -1};
       double v[4] = {0.0, 0.0, 0.0, 0.0};
       if (c < E.c1) fetch(E, c, gi, v);
       for (;;) {
         const int en = e + kReduceWaves;
         RowEnt En = E;
         int cn = 0;
-        int4 gn = make_int4(-1, -1, -1, -1);
+        i32x4 gn = {-1, -1, -1, -1};
         double vn[4] = {0.0, 0.0, 0.0, 0.0};
         if (en < e1) {  // wavefront-uniform: the next entry's loads before this entry's additions
-          En = ent[en];
+          En = load_global(entg + en);
           cn = (En.c0 & ~3) + 4 * lane;
           if (cn < En.c1) fetch(En, cn, gn, vn);
         }
         if (c < E.c1) add(E, c, gi, v);
         for (int c2 = c + 256; c2 < E.c1; c2 += 256) {
-          int4 g2;
+          i32x4 g2;
           double v2[4];
           fetch(E, c2, g2, v2);
           add(E, c2, g2, v2);
@@ -959,7 +993,7 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
       }
     }
     __syncthreads();
-    for (int j = tid; j < wl; j += 64 * kReduceWaves) U[(int64_t)(i + w0 + j) + (int64_t)n * i] = racc[j];
+    for (int j = tid; j < wl; j += 64 * kReduceWaves) to_global(U)[(int64_t)(i + w0 + j) + (int64_t)n * i] = racc[j];
     __syncthreads();
   }
 }
