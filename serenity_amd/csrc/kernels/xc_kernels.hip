@@ -246,81 +246,73 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
   const gmem<const int>* idx = to_global(ao_idx + bk.idx_off);
   const gmem<const T>* Pg = to_global(P);
   const int64_t cstride = (int64_t)npad * kTile;
-  // the block's AO index list in LDS (P rows of a chunk are then gathered without a
-  // dependent global load); wider blocks read it from global memory
-  const bool idx_lds = npad <= kDensIdxLds;
-  if (idx_lds)
-    for (int i = tid; i < npad; i += 512) sIdx[i] = idx[i];
-  __syncthreads();
-
   // staging: phi task = (point pa, AO quad qa); P tasks = (column cb, AO quad qb), 16
   // consecutive lanes taking 16 columns of one quad (their LDS stores then cover distinct banks)
   const int pa = tid & (kTile - 1), qa = tid >> 7;
   const int qb = (tid >> 4) & 3;
   auto col_of = [](int task) { return (task & 15) + 16 * (task >> 6); };
+  int nu[NB];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    const int task = tid + 512 * m, cb = col_of(task);
+    nu[m] = (task < 4 * NC && cb < ncols) ? idx[col0 + cb] : -1;
+  }
   typename M::acc_t acc[S];
 #pragma unroll
   for (int c = 0; c < S; ++c) acc[c] = M::zero();
-  // The chunk loop, with the AO indices read from LDS (ds_read: no wait on the global loads in
-  // flight) or, for blocks wider than the LDS list, from global memory.
-  auto chunks = [&](auto idx_in_lds) {
-    auto ix = [&](int i) -> int {
-      if constexpr (decltype(idx_in_lds)::value)
-        return sIdx[i];
-      else
-        return idx[i];
-    };
-    int nu[NB];
+  vec_t rA, rB[NB];
+  unsigned mu_ok = 0;  // bit e: AO KV qb + e of the chunk is not padding
+  // chunk at AO k0 of the block, its AO indices at sIdx[j0 ..]
+  auto load = [&](int k0, int j0) {
+#pragma unroll
+    for (int e = 0; e < KV; ++e) rA[e] = tb[(int64_t)(k0 + KV * qa + e) * kTile + pa];
+    int mu[KV];
+    mu_ok = 0;
+#pragma unroll
+    for (int e = 0; e < KV; ++e) {
+      mu[e] = sIdx[j0 + KV * qb + e];
+      mu_ok |= (unsigned)(mu[e] >= 0) << e;
+    }
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
-      const int task = tid + 512 * m, cb = col_of(task);
-      nu[m] = (task < 4 * NC && cb < ncols) ? ix(col0 + cb) : -1;
+      if (tid + 512 * m < 4 * NC) {  // whole wavefronts
+        // P is symmetric: read P[nu][mu] so that the 16 lanes of a group (consecutive
+        // columns nu, one mu) read consecutive addresses.  Padding (index -1) reads P[0][0]
+        // and is zeroed at the LDS store by a multiply: a select at the load would let the
+        // compiler move the load into a branch, and a multiply there would wait for it
+        // before the chunk's MFMAs.
+#pragma unroll
+        for (int e = 0; e < KV; ++e) rB[m][e] = Pg[max(nu[m], 0) + (int64_t)n * max(mu[e], 0)];
+      }
     }
-    vec_t rA, rB[NB];
-    unsigned mu_ok = 0;  // bit e: AO KV qb + e of the chunk is not padding
-    auto load = [&](int k0) {
+  };
+  auto store = [&]() {
+    *reinterpret_cast<vec_t*>(&sA[pa][KV * qa]) = rA;
 #pragma unroll
-      for (int e = 0; e < KV; ++e) rA[e] = tb[(int64_t)(k0 + KV * qa + e) * kTile + pa];
-      int mu[KV];
-      mu_ok = 0;
+    for (int m = 0; m < NB; ++m) {
+      const int task = tid + 512 * m;
+      if (task < 4 * NC) {
+        vec_t v = rB[m];
 #pragma unroll
-      for (int e = 0; e < KV; ++e) {
-        mu[e] = ix(k0 + KV * qb + e);
-        mu_ok |= (unsigned)(mu[e] >= 0) << e;
+        for (int e = 0; e < KV; ++e) v[e] *= T(((mu_ok >> e) & 1u) && nu[m] >= 0);
+        *reinterpret_cast<vec_t*>(&sB[col_of(task)][KV * qb]) = v;
       }
-#pragma unroll
-      for (int m = 0; m < NB; ++m) {
-        if (tid + 512 * m < 4 * NC) {  // whole wavefronts
-          // P is symmetric: read P[nu][mu] so that the 16 lanes of a group (consecutive
-          // columns nu, one mu) read consecutive addresses.  Padding (index -1) reads P[0][0]
-          // and is zeroed at the LDS store by a multiply: a select at the load would let the
-          // compiler move the load into a branch, and a multiply there would wait for it
-          // before the chunk's MFMAs.
-#pragma unroll
-          for (int e = 0; e < KV; ++e) rB[m][e] = Pg[max(nu[m], 0) + (int64_t)n * max(mu[e], 0)];
-        }
-      }
-    };
-    auto store = [&]() {
-      *reinterpret_cast<vec_t*>(&sA[pa][KV * qa]) = rA;
-#pragma unroll
-      for (int m = 0; m < NB; ++m) {
-        const int task = tid + 512 * m;
-        if (task < 4 * NC) {
-          vec_t v = rB[m];
-#pragma unroll
-          for (int e = 0; e < KV; ++e) v[e] *= T(((mu_ok >> e) & 1u) && nu[m] >= 0);
-          *reinterpret_cast<vec_t*>(&sB[col_of(task)][KV * qb]) = v;
-        }
-      }
-    };
-    // one LDS buffer; the global loads of chunk k + 1 are in flight during the MFMAs of chunk k
-    const int nch = npad / KC;
-    load(0);
+    }
+  };
+  // The block's AO index list goes to LDS (the P rows of a chunk are then gathered without a
+  // dependent global load) in windows of kDensIdxLds entries: one window unless the block is
+  // wider.  One LDS buffer; the global loads of chunk k + 1 of a window are in flight during the
+  // MFMAs of chunk k.
+  for (int wb = 0; wb < npad; wb += kDensIdxLds) {
+    const int wl = min(kDensIdxLds, npad - wb);
+    for (int i = tid; i < wl; i += 512) sIdx[i] = idx[wb + i];
+    __syncthreads();
+    const int nch = wl / KC;
+    load(wb, 0);
     for (int ch = 0; ch < nch; ++ch) {
       store();
       __syncthreads();
-      if (ch + 1 < nch) load((ch + 1) * KC);
+      if (ch + 1 < nch) load(wb + (ch + 1) * KC, (ch + 1) * KC);
       const vec_t av = *reinterpret_cast<const vec_t*>(&sA[16 * w + r][KV * g]);
 #pragma unroll
       for (int s0 = 0; s0 < S; s0 += GS) {
@@ -342,13 +334,9 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
             }
         __builtin_amdgcn_sched_barrier(0);
       }
-      __syncthreads();
+      __syncthreads();  // (also orders the next window's index staging after this window's reads)
     }
-  };
-  if (idx_lds)
-    chunks(std::true_type{});
-  else
-    chunks(std::false_type{});
+  }
 
   if constexpr (sizeof(T) == 4) {
     // Dots straight from the accumulators: lane (r, g) holds T[row(lane, e)][16 s + r] of its
@@ -807,21 +795,34 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   auto store = [&](int c) {
     T(*sPhi)[RS] = sPhi2[NBUF == 2 ? (c & 1) : 0];
     T(*sG)[RS] = sG2[NBUF == 2 ? (c & 1) : 0];
-    // the KV points of this thread's tasks are the same for every task
+    // the KV points of this thread's tasks are the same for every task; their potentials are
+    // read as whole vectors, the GGA test taken once (a test per value branches around each
+    // LDS read and waits for it)
     const int p0 = c * KC + KV * q;
     vec_t gv[NTASK];
+    auto pot = [&](int k) {
+      vec_t wk;
 #pragma unroll
-    for (int e = 0; e < KV; ++e) {
-      const T w0 = (T)spot[0][p0 + e];
-      const T w1 = gga ? (T)spot[1][p0 + e] : T(0);
-      const T w2 = gga ? (T)spot[2][p0 + e] : T(0);
-      const T w3 = gga ? (T)spot[3][p0 + e] : T(0);
-#pragma unroll
-      for (int m = 0; m < NTASK; ++m) {
-        T v = w0 * raw[m][0][e];
-        if (gga) v += w1 * raw[m][1][e] + w2 * raw[m][2][e] + w3 * raw[m][3][e];
-        gv[m][e] = v;
+      for (int e = 0; e < KV; e += 2) {
+        const f64x2 d = *reinterpret_cast<const f64x2*>(&spot[k][p0 + e]);
+        wk[e] = (T)d.x;
+        wk[e + 1] = (T)d.y;
       }
+      return wk;
+    };
+    const vec_t w0 = pot(0);
+    if (gga) {
+      const vec_t w1 = pot(1), w2 = pot(2), w3 = pot(3);
+#pragma unroll
+      for (int m = 0; m < NTASK; ++m)
+#pragma unroll
+        for (int e = 0; e < KV; ++e)
+          gv[m][e] = w0[e] * raw[m][0][e] + w1[e] * raw[m][1][e] + w2[e] * raw[m][2][e] + w3[e] * raw[m][3][e];
+    } else {
+#pragma unroll
+      for (int m = 0; m < NTASK; ++m)
+#pragma unroll
+        for (int e = 0; e < KV; ++e) gv[m][e] = w0[e] * raw[m][0][e];
     }
 #pragma unroll
     for (int m = 0; m < NTASK; ++m) {
