@@ -76,8 +76,11 @@ struct DevBlock {
 };
 
 // Fock sub-tiles of a block: the npad / 16 = nt sub-tile rows and columns of its upper
-// triangle, (I, J) with I <= J stored at sub-tile index I nt - I (I - 1) / 2 + (J - I), each as
-// 16 x 16 row-major values, in the block's slice of the scratch (DevBlock::toff).
+// triangle, (I, J) with I <= J, in the block's slice of the scratch (DevBlock::toff), stored by
+// matrix rows: compact row a (in sub-tile row I = a / 16) holds the columns 16 I .. npad - 1 (from
+// its diagonal sub-tile on) contiguously, and the 16 rows of sub-tile row I follow those of the
+// rows above, starting at element 256 fock_subtile_index(I, I, nt).  The reduce then reads each
+// AO row's segment of a block as one contiguous run.
 __host__ __device__ inline int64_t fock_subtile_index(int I, int J, int nt) {
   return (int64_t)I * nt - (int64_t)I * (I - 1) / 2 + (J - I);
 }
@@ -85,12 +88,16 @@ __host__ __device__ inline int64_t fock_subtile_elems(int npad) {
   const int64_t nt = npad / 16;
   return nt * (nt + 1) / 2 * 256;
 }
+// element of compact row a, column 16 (a / 16), in the block's slice
+__host__ __device__ inline int64_t fock_row_start(int a, int npad) {
+  const int I = a / 16;
+  return 256 * fock_subtile_index(I, I, npad / 16) + (int64_t)(a % 16) * (npad - 16 * I);
+}
 
 // One contribution to a row of the Fock matrix: block `block` holds AO `row` at compact position
-// c0; its sub-tile row for that AO starts at element rowbase of the scratch (sub-tile (c0 / 16,
-// c0 / 16), row c0 % 16), and the compact columns c0 .. c1 - 1 (AOs idx[idx_off + c], in
-// increasing global order) carry the values F[row][idx[c]] of the block.  Column c lies in
-// sub-tile c / 16, at rowbase + (c / 16 - c0 / 16) * 256 + c % 16.
+// c0, and the compact columns c0 .. c1 - 1 (AOs idx[idx_off + c], in increasing global order)
+// carry the values F[row][idx[c]] of the block, column c at element rowbase + c of the scratch
+// (rowbase = the block's toff + fock_row_start(c0, npad) - 16 (c0 / 16)).
 struct RowEnt {
   int64_t rowbase;
   int c0, c1, idx_off, block;

@@ -684,9 +684,9 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   constexpr int NBUF = sizeof(T) == 4 ? 2 : 1;
   __shared__ __attribute__((aligned(16))) T sPhi2[NBUF][kFockRows][RS];
   __shared__ __attribute__((aligned(16))) T sG2[NBUF][kFockRows][RS];
-  // w v / 2, w g_x, w g_y, w g_z of each point, one plane per quantity: the four point runs a
-  // wavefront reads at once then start on distinct banks
-  __shared__ __attribute__((aligned(16))) double spot[4][kTile];
+  // w v / 2, w g_x, w g_y, w g_z of each point in the contraction type, one plane per quantity:
+  // the four point runs a wavefront reads at once then start on distinct banks
+  __shared__ __attribute__((aligned(16))) T spot[4][kTile];
   __shared__ double red[2];
 
   const FockItem it = work[blockIdx.x];
@@ -738,10 +738,10 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     const double v1 = (in && gga) ? gwg[q] : 0.0;
     const double v2 = (in && gga) ? gwg[npts + q] : 0.0;
     const double v3 = (in && gga) ? gwg[2 * npts + q] : 0.0;
-    spot[0][tid] = 0.5 * v0;
-    spot[1][tid] = v1;
-    spot[2][tid] = v2;
-    spot[3][tid] = v3;
+    spot[0][tid] = (T)(0.5 * v0);
+    spot[1][tid] = (T)v1;
+    spot[2][tid] = (T)v2;
+    spot[3][tid] = (T)v3;
     double a = fabs(v0) + fabs(v1) + fabs(v2) + fabs(v3);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
@@ -770,14 +770,13 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   // a skipped block (every item of it takes the same decision from the same values) leaves zeros
   // in its sub-tiles, so the reduce needs no per-block test
   if ((red[0] + red[1]) / bk.npts < thr) {
-    const int nt0 = npad / 16;
     gmem<T>* out0 = to_global(Tscr + bk.toff);
     for (int t = w; t < it.nsub; t += kFockWaves) {
       int i, j;
       decode(t, i, j);
       const int I = it.a0 / 16 + i, J = (diag ? it.a0 : it.b0) / 16 + j;
-      gmem<T>* sub = out0 + fock_subtile_index(I, J, nt0) * 256;
-      for (int e = lane; e < 256; e += 64) sub[e] = T(0);
+      gmem<T>* sub = out0 + fock_row_start(16 * I, npad) + 16 * (J - I);
+      for (int e = lane; e < 256; e += 64) sub[(int64_t)(e >> 4) * (npad - 16 * I) + (e & 15)] = T(0);
     }
     return;
   }
@@ -800,16 +799,7 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     // LDS read and waits for it)
     const int p0 = c * KC + KV * q;
     vec_t gv[NTASK];
-    auto pot = [&](int k) {
-      vec_t wk;
-#pragma unroll
-      for (int e = 0; e < KV; e += 2) {
-        const f64x2 d = *reinterpret_cast<const f64x2*>(&spot[k][p0 + e]);
-        wk[e] = (T)d.x;
-        wk[e + 1] = (T)d.y;
-      }
-      return wk;
-    };
+    auto pot = [&](int k) { return *reinterpret_cast<const vec_t*>(&spot[k][p0]); };
     const vec_t w0 = pot(0);
     if (gga) {
       const vec_t w1 = pot(1), w2 = pot(2), w3 = pot(3);
@@ -885,7 +875,6 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   }
 
   // store the sub-tiles (block sub-tile (I, J), I <= J; 16 lanes write 16 consecutive values of a row)
-  const int nt = npad / 16;
   gmem<T>* tb_out = to_global(Tscr + bk.toff);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -894,9 +883,10 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
     int i, j;
     decode(t, i, j);
     const int I = it.a0 / 16 + i, J = (diag ? it.a0 : it.b0) / 16 + j;
-    gmem<T>* sub = tb_out + fock_subtile_index(I, J, nt) * 256;
+    gmem<T>* sub = tb_out + fock_row_start(16 * I, npad) + 16 * (J - I) + r;
+    const int stride = npad - 16 * I;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) sub[M::row(lane, e) * 16 + r] = acc[s][e];
+    for (int e = 0; e < 4; ++e) sub[(int64_t)M::row(lane, e) * stride] = acc[s][e];
   }
 }
 
@@ -904,11 +894,12 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
 // block order) are dealt to the kReduceWaves wavefronts round-robin; a wavefront adds each entry's
 // stored row segment into the workgroup's LDS copy of the row, and the row is written once to
 // U[j + n i], j >= i.  Each lane takes an aligned group of four columns (256 per pass of the
-// wavefront): one 16-byte load of their AO indices and one of their stored values (a block's AO
-// list is padded to a multiple of 32 with -1 and a sub-tile row holds 16 values, so an aligned
-// group never leaves either; the columns of the group before the segment start are masked).
-// The loads of a wavefront's next entry are issued before the current entry's LDS additions.  Rows longer than the LDS window are done in
-// several passes over the entries.  The LDS additions of different wavefronts meet in an order
+// wavefront): one 16-byte load of their AO indices and one of their stored values, the lanes of a
+// wavefront reading one contiguous run of the block's stored row (a block's AO list is padded to
+// a multiple of 32 with -1 and its stored row a runs from column 16 (a / 16) to the padded end,
+// so an aligned group never leaves either; the columns of the group before the segment start are
+// masked).  The loads of a wavefront's next entry are issued before the current entry's LDS
+// additions.  Rows longer than the LDS window are done in several passes over the entries.  The LDS additions of different wavefronts meet in an order
 // that varies from run to run, so F varies at the rounding level.
 template <typename T>
 __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const RowEnt* __restrict__ ent,
@@ -941,7 +932,7 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
     __syncthreads();
     auto fetch = [&](const RowEnt& E, int c, i32x4& gi, double (&v)[4]) {
       gi = *reinterpret_cast<const gmem<const i32x4>*>(idxg + E.idx_off + c);
-      const gmem<const T>* src = Tg + E.rowbase + (int64_t)((c >> 4) - (E.c0 >> 4)) * 256 + (c & 15);
+      const gmem<const T>* src = Tg + E.rowbase + c;
       if constexpr (sizeof(T) == 4) {
         const f32x4 f = *reinterpret_cast<const gmem<const f32x4>*>(src);
         v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
@@ -959,38 +950,56 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
         if (c + u >= E.c0 && c + u < E.c1 && g >= 0 && g < wl) atomicAdd(racc + g, v[u]);
       }
     };
-    int e = e0 + w;
-    if (e < e1) {
-      RowEnt E = load_global(entg + e);
-      int c = (E.c0 & ~3) + 4 * lane;
-      i32x4 gi = {-1, -1, -1, -1};
-      double v[4] = {0.0, 0.0, 0.0, 0.0};
-      if (c < E.c1) fetch(E, c, gi, v);
-      for (;;) {
-        const int en = e + kReduceWaves;
-        RowEnt En = E;
-        int cn = 0;
-        i32x4 gn = {-1, -1, -1, -1};
-        double vn[4] = {0.0, 0.0, 0.0, 0.0};
-        if (en < e1) {  // wavefront-uniform: the next entry's loads before this entry's additions
-          En = load_global(entg + en);
-          cn = (En.c0 & ~3) + 4 * lane;
-          if (cn < En.c1) fetch(En, cn, gn, vn);
-        }
-        if (c < E.c1) add(E, c, gi, v);
-        for (int c2 = c + 256; c2 < E.c1; c2 += 256) {
+    // this wavefront's entries e0 + w + kReduceWaves k, k < nk: their RowEnts are read 64 at a time,
+    // one per lane, and broadcast from the lanes (scalar registers), so that an entry's loads
+    // depend on no load of its own; the loads of the next kAhead entries are in flight during an
+    // entry's LDS additions
+    constexpr int kAhead = 1;
+    const int nk = e1 - e0 > w ? (e1 - e0 - w + kReduceWaves - 1) / kReduceWaves : 0;
+    for (int kb = 0; kb < nk; kb += 64) {
+      const int cnt = min(64, nk - kb);
+      RowEnt L = {0, 0, 0, 0, 0};
+      if (lane < cnt) L = load_global(entg + e0 + w + kReduceWaves * (kb + lane));
+      RowEnt Eq[kAhead + 1];
+      int cq[kAhead + 1];
+      i32x4 gq[kAhead + 1];
+      double vq[kAhead + 1][4];
+      auto issue = [&](int k, int d) {  // entry k into slot d
+        RowEnt& E = Eq[d];
+        const int lo = __builtin_amdgcn_readlane((int)(L.rowbase & 0xffffffff), k);
+        const int hi = __builtin_amdgcn_readlane((int)(L.rowbase >> 32), k);
+        E.rowbase = (int64_t)(((uint64_t)(unsigned)hi << 32) | (unsigned)lo);
+        E.c0 = __builtin_amdgcn_readlane(L.c0, k);
+        E.c1 = __builtin_amdgcn_readlane(L.c1, k);
+        E.idx_off = __builtin_amdgcn_readlane(L.idx_off, k);
+        E.block = 0;
+        cq[d] = (E.c0 & ~3) + 4 * lane;
+        gq[d] = i32x4{-1, -1, -1, -1};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) vq[d][u] = 0.0;
+        if (cq[d] < E.c1) fetch(E, cq[d], gq[d], vq[d]);
+      };
+#pragma unroll
+      for (int d = 0; d < kAhead; ++d)
+        if (d < cnt) issue(d, d);
+      for (int k = 0; k < cnt; ++k) {
+        if (k + kAhead < cnt) issue(k + kAhead, kAhead);  // wavefront-uniform
+        const RowEnt& E = Eq[0];
+        if (cq[0] < E.c1) add(E, cq[0], gq[0], vq[0]);
+        for (int c2 = cq[0] + 256; c2 < E.c1; c2 += 256) {
           i32x4 g2;
           double v2[4];
           fetch(E, c2, g2, v2);
           add(E, c2, g2, v2);
         }
-        if (en >= e1) break;
-        e = en;
-        E = En;
-        c = cn;
-        gi = gn;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = vn[u];
+        for (int d = 0; d < kAhead; ++d) {
+          Eq[d] = Eq[d + 1];
+          cq[d] = cq[d + 1];
+          gq[d] = gq[d + 1];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) vq[d][u] = vq[d + 1][u];
+        }
       }
     }
     __syncthreads();

@@ -1066,7 +1066,7 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
         for (int c = 0; c < kFockNumS; ++c) xcd_order(bf->fock_items[f][c]);
       bf->tile_elems = tile_off;
       // row entries of the Fock reduce: for every AO, the blocks holding it (in block order) with
-      // the position of its sub-tile row in the scratch
+      // the position of its row in the scratch
       const int nao = basis->nao;
       std::vector<int> rowptr(nao + 1, 0);
       for (int b = 0; b < nb; ++b)
@@ -1077,10 +1077,9 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
         std::vector<int> fill(rowptr.begin(), rowptr.end() - 1);
         for (int b = 0; b < nb; ++b) {
           const DevBlock& bk = bf->blocks[b];
-          const int nt = bk.npad / 16;
           for (int l = 0; l < bk.nb; ++l) {
             RowEnt e;
-            e.rowbase = bk.toff + fock_subtile_index(l / 16, l / 16, nt) * 256 + (l % 16) * 16;
+            e.rowbase = bk.toff + fock_row_start(l, bk.npad) - 16 * (l / 16);
             e.c0 = l;
             e.c1 = bk.nb;
             e.idx_off = bk.idx_off;
