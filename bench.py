@@ -385,7 +385,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the XC Fock build runs on one MI355X here "
-              "(multi-GPU block sharding is a design note, not part of this build).", file=sys.stderr)
+              "(multi-GPU block sharding is described in the design notes, not implemented).", file=sys.stderr)
         sys.exit(2)
 
     from serenity_amd import xc, workloads as W
