@@ -184,7 +184,8 @@ constexpr int kDensIdxLds = 2048;  // AO index lists up to this length are stage
 template <typename T>
 void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
                         const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
-                        int64_t npts, const PackSys* pack = nullptr, const int* nwork_dev = nullptr);
+                        int64_t npts, const PackSys* pack = nullptr, const int* nwork_dev = nullptr,
+                        const unsigned char* skip = nullptr);
 
 // Dense-block refinement of the mixed-precision density: the work lists (one per column-tile
 // class) and their device-side counts, filled by launch_mark_dense.
@@ -195,9 +196,15 @@ struct DenseLists {
 // Clears the counts, then appends the density items of every block whose largest density in the
 // partials part (one pass per item, 128-point blocks) exceeds rho_thr; by_block holds every block's
 // items contiguously from item0[b] (blocks[b].npass of them), cls[b] is their class.
+// With a mask (one byte per block), the blocks it marks were skipped by the first pass (they were
+// dense in the previous build) and are listed without a test; the mask is then set to this build's
+// listed blocks, and launch_refresh_dense, after the second pass, keeps only those whose density from
+// that pass still exceeds rho_thr.
 void launch_mark_dense(hipStream_t s, const DevBlock* blocks, int nblocks, int64_t npts, const double* part,
                        double rho_thr, const WorkItem* by_block, const int* item0, const int* cls,
-                       const DenseLists& lists);
+                       const DenseLists& lists, unsigned char* mask = nullptr);
+void launch_refresh_dense(hipStream_t s, const DevBlock* blocks, int nblocks, int64_t npts, const double* part,
+                          double rho_thr, unsigned char* mask);
 void launch_to_double(hipStream_t s, const float* src, double* dst, int64_t n);
 
 // rho[q] / grad[.][q] from the partials of the block of q (any block size).

@@ -209,7 +209,8 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
                                                           const T* __restrict__ tiles, const T* __restrict__ P,
                                                           int n, int want_grad, double* __restrict__ part,
                                                           int64_t npts, const PackSys* __restrict__ pack,
-                                                          const int* __restrict__ nwork_dev) {
+                                                          const int* __restrict__ nwork_dev,
+                                                          const unsigned char* __restrict__ skip) {
   // a launch over a list filled on the device runs only its first *nwork_dev items
   if (nwork_dev && (int)blockIdx.x >= *nwork_dev) return;
   using M = Mfma<T>;
@@ -227,6 +228,7 @@ __global__ __launch_bounds__(512) void density_sub_kernel(const WorkItem* __rest
   T(*sB)[RS] = reinterpret_cast<T(*)[RS]>(smem + kTile * RS * sizeof(T));
 
   const WorkItem wk = work[blockIdx.x];
+  if (skip && skip[wk.block]) return;  // a block the caller contracts in another pass
   const DevBlock bk = blocks[wk.block];
   if (pack) {
     const PackSys& ps = pack[bk.sys];
@@ -1179,13 +1181,13 @@ void launch_ao_eval(hipStream_t s, const DevBlock* blocks, int nblocks, const in
 template <typename T>
 void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, const DevBlock* blocks,
                         const int* ao_idx, const T* tiles, const T* P, int n, int want_grad, double* part,
-                        int64_t npts, const PackSys* pack, const int* nwork_dev) {
+                        int64_t npts, const PackSys* pack, const int* nwork_dev, const unsigned char* skip) {
   if (nwork <= 0) return;
   switch (S) {
 #define SXC_DENS_S(SV)                                                                                        \
   case SV:                                                                                                    \
     hipLaunchKernelGGL((density_sub_kernel<T, SV>), dim3(nwork), dim3(512), 0, s, work, blocks, ao_idx, tiles, \
-                       P, n, want_grad, part, npts, pack, nwork_dev);                                         \
+                       P, n, want_grad, part, npts, pack, nwork_dev, skip);                                   \
     return;
     static_assert(kDensMaxS == 6, "one case per column-tile class");
     SXC_DENS_S(1)
@@ -1200,21 +1202,14 @@ void launch_density_sub(hipStream_t s, int S, const WorkItem* work, int nwork, c
 }
 template void launch_density_sub<double>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
                                          const double*, const double*, int, int, double*, int64_t,
-                                         const PackSys*, const int*);
+                                         const PackSys*, const int*, const unsigned char*);
 template void launch_density_sub<float>(hipStream_t, int, const WorkItem*, int, const DevBlock*, const int*,
                                         const float*, const float*, int, int, double*, int64_t, const PackSys*,
-                                        const int*);
+                                        const int*, const unsigned char*);
 
-// Blocks whose largest density (from the partials of a first pass) exceeds rho_thr: their density
-// work items are appended to the list of their column-tile class, for a second pass.
-__global__ __launch_bounds__(128) void mark_dense_kernel(const DevBlock* __restrict__ blocks, int64_t npts,
-                                                         const double* __restrict__ part, double rho_thr,
-                                                         const WorkItem* __restrict__ by_block,
-                                                         const int* __restrict__ item0, const int* __restrict__ cls,
-                                                         DenseLists lists) {
-  __shared__ double red[128];
-  const int b = blockIdx.x, t = threadIdx.x;
-  const DevBlock bk = blocks[b];
+// Largest density of block b (128 threads), from the partials of its passes.
+__device__ double block_max_density(const DevBlock& bk, int64_t npts, const double* __restrict__ part, double* red) {
+  const int t = threadIdx.x;
   double r = 0.0;
   if (t < bk.npts)
     for (int s = 0; s < bk.npass; ++s) r += part[(int64_t)s * 4 * npts + bk.first + t];
@@ -1224,11 +1219,40 @@ __global__ __launch_bounds__(128) void mark_dense_kernel(const DevBlock* __restr
     if (t < k) red[t] = fmax(red[t], red[t + k]);
     __syncthreads();
   }
-  if (t == 0 && bk.npass > 0 && red[0] > rho_thr) {
+  return red[0];
+}
+
+// Blocks whose largest density (from the partials of a first pass) exceeds rho_thr, and the blocks
+// of the mask (skipped by the first pass), have their density work items appended to the list of
+// their column-tile class, for a second pass; the mask becomes the set of listed blocks.
+__global__ __launch_bounds__(128) void mark_dense_kernel(const DevBlock* __restrict__ blocks, int64_t npts,
+                                                         const double* __restrict__ part, double rho_thr,
+                                                         const WorkItem* __restrict__ by_block,
+                                                         const int* __restrict__ item0, const int* __restrict__ cls,
+                                                         DenseLists lists, unsigned char* __restrict__ mask) {
+  __shared__ double red[128];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const DevBlock bk = blocks[b];
+  const bool skipped = mask && mask[b];
+  const bool dense = bk.npass > 0 && (skipped || block_max_density(bk, npts, part, red) > rho_thr);
+  if (t == 0 && dense) {
     const int c = cls[b];
     const int pos = atomicAdd(lists.count + c, bk.npass);
     for (int k = 0; k < bk.npass; ++k) lists.item[c][pos + k] = by_block[item0[b] + k];
   }
+  if (t == 0 && mask) mask[b] = dense ? 1 : 0;
+}
+
+// After the second pass: a listed block stays in the mask while its density exceeds rho_thr.
+__global__ __launch_bounds__(128) void refresh_dense_kernel(const DevBlock* __restrict__ blocks, int64_t npts,
+                                                            const double* __restrict__ part, double rho_thr,
+                                                            unsigned char* __restrict__ mask) {
+  __shared__ double red[128];
+  const int b = blockIdx.x;
+  if (!mask[b]) return;  // uniform over the workgroup
+  const DevBlock bk = blocks[b];
+  const bool dense = block_max_density(bk, npts, part, red) > rho_thr;
+  if (threadIdx.x == 0) mask[b] = dense ? 1 : 0;
 }
 
 __global__ void clear_counts_kernel(int* __restrict__ c, int n) {
@@ -1242,11 +1266,17 @@ __global__ void to_double_kernel(const float* __restrict__ s, double* __restrict
 
 void launch_mark_dense(hipStream_t s, const DevBlock* blocks, int nblocks, int64_t npts, const double* part,
                        double rho_thr, const WorkItem* by_block, const int* item0, const int* cls,
-                       const DenseLists& lists) {
+                       const DenseLists& lists, unsigned char* mask) {
   if (nblocks <= 0) return;
   hipLaunchKernelGGL(clear_counts_kernel, dim3(1), dim3(64), 0, s, lists.count, kDensMaxS + 1);
   hipLaunchKernelGGL(mark_dense_kernel, dim3(nblocks), dim3(128), 0, s, blocks, npts, part, rho_thr, by_block, item0,
-                     cls, lists);
+                     cls, lists, mask);
+}
+
+void launch_refresh_dense(hipStream_t s, const DevBlock* blocks, int nblocks, int64_t npts, const double* part,
+                          double rho_thr, unsigned char* mask) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(refresh_dense_kernel, dim3(nblocks), dim3(128), 0, s, blocks, npts, part, rho_thr, mask);
 }
 
 void launch_to_double(hipStream_t s, const float* src, double* dst, int64_t n) {

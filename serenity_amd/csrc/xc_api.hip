@@ -383,6 +383,9 @@ struct xc_bfogc {
   std::vector<int> dens_item0, dens_cls;
   DBuf<WorkItem> d_dens_by_block, d_dense_list[kDensMaxS + 1];
   DBuf<int> d_dens_item0, d_dens_cls, d_dense_cnt;
+  // mixed precision: the blocks contracted in FP64 by the last mixed build that were still dense
+  // (one byte per block); the next mixed build's FP32 pass skips them
+  DBuf<unsigned char> d_dense_mask;
   DBuf<double> d_P32d;
   DBuf<FockItem> d_fock_items[2][kFockNumS];
   // per-build work buffers
@@ -435,30 +438,35 @@ void over_lanes(const Lanes& L, int nclass, const std::function<int(int)>& count
 }
 
 template <typename T>
-void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad, const Lanes& L) {
+void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad, const Lanes& L,
+                 const unsigned char* skip = nullptr) {
   over_lanes(L, kDensMaxS + 1, [&](int c) { return c >= 1 ? (int)bf->dens_work[c].size() : 0; },
              [&](int c, hipStream_t s) {
     launch_density_sub<T>(s, c, bf->d_dens_work[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
-                      bf->d_ao_idx.p, tiles, P, bf->basis->nao, want_grad, bf->d_part.p, bf->grid->npts);
+                      bf->d_ao_idx.p, tiles, P, bf->basis->nao, want_grad, bf->d_part.p, bf->grid->npts, nullptr,
+                      nullptr, skip);
   });
 }
 
 // Mixed precision: FP64 density items (FP64 tiles, the FP32-rounded P in FP64) for the blocks
-// whose largest density after the FP32 pass exceeds kMixedRhoThreshold; they overwrite those
-// blocks' partials.  The lists are filled on the device, so the launches cover their capacity
-// and run only the filled part.
+// whose largest density after the FP32 pass exceeds kMixedRhoThreshold, and for the blocks the
+// FP32 pass skipped because the last mixed build found them dense; they overwrite those blocks'
+// partials.  The lists are filled on the device, so the launches cover their capacity and run
+// only the filled part.  The mask then keeps the blocks whose FP64 density exceeds the threshold.
 void run_density_refine(xc_bfogc* bf, int want_grad, const Lanes& L) {
   DenseLists lists{};
   for (int c = 0; c <= kDensMaxS; ++c) lists.item[c] = bf->d_dense_list[c].p;
   lists.count = bf->d_dense_cnt.p;
   launch_mark_dense(L.main, bf->d_blocks.p, bf->nblocks, bf->grid->npts, bf->d_part.p, kMixedRhoThreshold,
-                    bf->d_dens_by_block.p, bf->d_dens_item0.p, bf->d_dens_cls.p, lists);
+                    bf->d_dens_by_block.p, bf->d_dens_item0.p, bf->d_dens_cls.p, lists, bf->d_dense_mask.p);
   over_lanes(L, kDensMaxS + 1, [&](int c) { return c >= 1 ? (int)bf->dens_work[c].size() : 0; },
              [&](int c, hipStream_t s) {
     launch_density_sub<double>(s, c, bf->d_dense_list[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
                                bf->d_ao_idx.p, bf->d_t64.p, bf->d_P32d.p, bf->basis->nao, want_grad, bf->d_part.p,
                                bf->grid->npts, nullptr, bf->d_dense_cnt.p + c);
   });
+  launch_refresh_dense(L.main, bf->d_blocks.p, bf->nblocks, bf->grid->npts, bf->d_part.p, kMixedRhoThreshold,
+                       bf->d_dense_mask.p);
 }
 
 // A controller whose server build is still running owns its work buffers (P, partial
@@ -1114,6 +1122,8 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
         for (int c = 1; c <= kDensMaxS; ++c)
           bf->d_dense_list[c].alloc(ctx, std::max<size_t>(bf->dens_work[c].size(), 1));
         bf->d_dense_cnt.alloc(ctx, kDensMaxS + 1);
+        bf->d_dense_mask.alloc(ctx, std::max(bf->nblocks, 1));
+        HIPX(hipMemsetAsync(bf->d_dense_mask.p, 0, std::max(bf->nblocks, 1), ctx->stream));
       }
       for (int f = 0; f < 2; ++f)
         for (int c = 0; c < kFockNumS; ++c)
@@ -1403,7 +1413,7 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   if (!lowp) {
     run_density<double>(bf, bf->d_t64.p, dP, gga ? 1 : 0, L);
   } else {
-    run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0, L);
+    run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0, L, mixed ? bf->d_dense_mask.p : nullptr);
     if (mixed) run_density_refine(bf, gga ? 1 : 0, L);
   }
   if (s == ctx->stream) ctx->mark();

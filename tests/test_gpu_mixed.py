@@ -43,10 +43,27 @@ def test_mixed_build_removes_the_fp32_bias(ctx):
     say("(H2O)8/TZVP guess density: |dE| FP32 %.2e mixed %.2e; max|dF| FP32 %.2e mixed %.2e" % (d32, dmx, f32, fmx))
     assert dmx < 1e-7 and dmx < d32
     assert fmx < 1e-5
-    # the same mixed build again: the device-side dense-block lists start from zero every build
+    # the same mixed build again: the blocks found dense by the first build are now skipped by the
+    # FP32 pass and contracted in FP64 directly; the result is the first build's
     pot = xc.FuncPotential(bf, "PBE", "MIXED")
-    pot.getMatrix(P)
+    F2 = pot.getMatrix(P)
     assert abs(pot.getEnergy() - out["MIXED"][0]) < 1e-12
+    assert np.abs(F2 - out["MIXED"][1]).max() < 1e-12
+    # a different density next (half the charge: fewer dense blocks): the carried-over blocks are
+    # contracted in FP64 whatever their density, so the build stays within the mixed bar of FP64
+    # for that density; back to P, the dense blocks are found again and the first result returns
+    e64h = xc.FuncPotential(bf, "PBE", "FP64")
+    e64h.getMatrix(0.5 * P)
+    pot = xc.FuncPotential(bf, "PBE", "MIXED")
+    pot.getMatrix(0.5 * P)
+    dh = abs(pot.getEnergy() - e64h.getEnergy())
+    pot = xc.FuncPotential(bf, "PBE", "MIXED")
+    F3 = pot.getMatrix(P)
+    say("mixed builds carrying dense blocks over: P/2 |dE| vs FP64 %.2e; back to P: |dE| vs the first %.2e" %
+        (dh, abs(pot.getEnergy() - out["MIXED"][0])))
+    assert dh < 1e-7
+    assert abs(pot.getEnergy() - out["MIXED"][0]) < 1e-12
+    assert np.abs(F3 - out["MIXED"][1]).max() < 1e-12
     only32 = xc.BasisFunctionOnGridController(ctx, b, g, 128, 1e-9, 1, keep_fp32="only")
     with pytest.raises(xc.XCError, match="both tile sets"):
         xc.FuncPotential(only32, "PBE", "MIXED").getMatrix(P)
