@@ -330,7 +330,8 @@ def scf_leg(ctx, check_oracle, config=1, cycle_log=None):
         task.run("FP64")  # first run builds the resident tiles of both grids
     r64 = task.run("FP64")
     rmx = task.run("MIXED")
-    out["fp64"] = dict(energy=r64["energy"], cycles=r64["cycles"], wall_s=r64["wall_s"], xc_s=r64["t_xc_s"])
+    out["fp64"] = dict(energy=r64["energy"], cycles=r64["cycles"], wall_s=r64["wall_s"], xc_s=r64["t_xc_s"],
+                       j_s=r64["t_j_s"], host_algebra_s=r64["t_alg_s"])
     out["mixed"] = dict(energy_before_fp64_finish=rmx["energy_before_fp64_finish"], energy=rmx["energy"],
                         cycles_fp32=rmx["cycles_fp32"], cycles_fp64_finish=rmx["cycles_fp64_finish"],
                         wall_s=rmx["wall_s"])
@@ -543,9 +544,10 @@ def main():
     if not args.no_scf3 and not args.no_scf:
         sc3 = scf_leg(ctx, args.scf3_oracle and not args.no_cpu_baseline, config=3,
                       cycle_log=lambda *a: log("  scf3", *a))
-        log("SCF (H2O)64/TZVP/PBE: FP64 E %.10f (%d cycles, %.1f s); mixed before finish dE %.2e, after %.2e "
-            "(%d+%d cycles, %.1f s)" %
-            (sc3["fp64"]["energy"], sc3["fp64"]["cycles"], sc3["fp64"]["wall_s"],
+        log("SCF (H2O)64/TZVP/PBE: FP64 E %.10f (%d cycles, %.1f s; XC %.2f s, J %.2f s, host algebra %.2f s); "
+            "mixed before finish dE %.2e, after %.2e (%d+%d cycles, %.1f s)" %
+            (sc3["fp64"]["energy"], sc3["fp64"]["cycles"], sc3["fp64"]["wall_s"], sc3["fp64"]["xc_s"],
+             sc3["fp64"]["j_s"], sc3["fp64"]["host_algebra_s"],
              sc3["mixed"]["dE_before_finish_vs_fp64"], sc3["mixed"]["dE_after_finish_vs_fp64"],
              sc3["mixed"]["cycles_fp32"], sc3["mixed"]["cycles_fp64_finish"], sc3["mixed"]["wall_s"]))
         out_extra["scf_config3"] = sc3
