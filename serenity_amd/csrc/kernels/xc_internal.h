@@ -196,13 +196,16 @@ struct DenseLists {
 // Clears the counts, then appends the density items of every block whose largest density in the
 // partials part (one pass per item, 128-point blocks) exceeds rho_thr; by_block holds every block's
 // items contiguously from item0[b] (blocks[b].npass of them), cls[b] is their class.
-// With a mask (one byte per block), the blocks it marks were skipped by the first pass (they were
-// dense in the previous build) and are listed without a test; the mask is then set to this build's
-// listed blocks, and launch_refresh_dense, after the second pass, keeps only those whose density from
-// that pass still exceeds rho_thr.
+// With a mask (one byte per block: the blocks the first pass skipped because the previous build
+// found them dense), only the blocks outside it are tested and the listed ones join it;
+// launch_list_masked lists the mask's own blocks (before the first pass, so that their second pass
+// can run beside it), and launch_refresh_dense, after the second passes, keeps in the mask only the
+// blocks whose density from them still exceeds rho_thr.
 void launch_mark_dense(hipStream_t s, const DevBlock* blocks, int nblocks, int64_t npts, const double* part,
                        double rho_thr, const WorkItem* by_block, const int* item0, const int* cls,
                        const DenseLists& lists, unsigned char* mask = nullptr);
+void launch_list_masked(hipStream_t s, const DevBlock* blocks, int nblocks, const WorkItem* by_block,
+                        const int* item0, const int* cls, const DenseLists& lists, const unsigned char* mask);
 void launch_refresh_dense(hipStream_t s, const DevBlock* blocks, int nblocks, int64_t npts, const double* part,
                           double rho_thr, unsigned char* mask);
 void launch_to_double(hipStream_t s, const float* src, double* dst, int64_t n);

@@ -1222,9 +1222,10 @@ __device__ double block_max_density(const DevBlock& bk, int64_t npts, const doub
   return red[0];
 }
 
-// Blocks whose largest density (from the partials of a first pass) exceeds rho_thr, and the blocks
-// of the mask (skipped by the first pass), have their density work items appended to the list of
-// their column-tile class, for a second pass; the mask becomes the set of listed blocks.
+// Blocks whose largest density (from the partials of a first pass) exceeds rho_thr have their
+// density work items appended to the list of their column-tile class, for a second pass.  With a
+// mask, only the blocks outside it are tested (the first pass skipped the others) and the listed
+// ones join it.
 __global__ __launch_bounds__(128) void mark_dense_kernel(const DevBlock* __restrict__ blocks, int64_t npts,
                                                          const double* __restrict__ part, double rho_thr,
                                                          const WorkItem* __restrict__ by_block,
@@ -1232,15 +1233,29 @@ __global__ __launch_bounds__(128) void mark_dense_kernel(const DevBlock* __restr
                                                          DenseLists lists, unsigned char* __restrict__ mask) {
   __shared__ double red[128];
   const int b = blockIdx.x, t = threadIdx.x;
+  if (mask && mask[b]) return;  // uniform over the workgroup
   const DevBlock bk = blocks[b];
-  const bool skipped = mask && mask[b];
-  const bool dense = bk.npass > 0 && (skipped || block_max_density(bk, npts, part, red) > rho_thr);
+  const bool dense = bk.npass > 0 && block_max_density(bk, npts, part, red) > rho_thr;
   if (t == 0 && dense) {
     const int c = cls[b];
     const int pos = atomicAdd(lists.count + c, bk.npass);
     for (int k = 0; k < bk.npass; ++k) lists.item[c][pos + k] = by_block[item0[b] + k];
+    if (mask) mask[b] = 1;
   }
-  if (t == 0 && mask) mask[b] = dense ? 1 : 0;
+}
+
+// The density work items of the blocks in the mask, appended to the list of their class.
+__global__ __launch_bounds__(64) void list_masked_kernel(const DevBlock* __restrict__ blocks, int nblocks,
+                                                         const WorkItem* __restrict__ by_block,
+                                                         const int* __restrict__ item0, const int* __restrict__ cls,
+                                                         DenseLists lists, const unsigned char* __restrict__ mask) {
+  const int b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= nblocks || !mask[b]) return;
+  const int np = blocks[b].npass;
+  if (np <= 0) return;
+  const int c = cls[b];
+  const int pos = atomicAdd(lists.count + c, np);
+  for (int k = 0; k < np; ++k) lists.item[c][pos + k] = by_block[item0[b] + k];
 }
 
 // After the second pass: a listed block stays in the mask while its density exceeds rho_thr.
@@ -1270,6 +1285,14 @@ void launch_mark_dense(hipStream_t s, const DevBlock* blocks, int nblocks, int64
   if (nblocks <= 0) return;
   hipLaunchKernelGGL(clear_counts_kernel, dim3(1), dim3(64), 0, s, lists.count, kDensMaxS + 1);
   hipLaunchKernelGGL(mark_dense_kernel, dim3(nblocks), dim3(128), 0, s, blocks, npts, part, rho_thr, by_block, item0,
+                     cls, lists, mask);
+}
+
+void launch_list_masked(hipStream_t s, const DevBlock* blocks, int nblocks, const WorkItem* by_block,
+                        const int* item0, const int* cls, const DenseLists& lists, const unsigned char* mask) {
+  hipLaunchKernelGGL(clear_counts_kernel, dim3(1), dim3(64), 0, s, lists.count, kDensMaxS + 1);
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(list_masked_kernel, dim3((nblocks + 63) / 64), dim3(64), 0, s, blocks, nblocks, by_block, item0,
                      cls, lists, mask);
 }
 

@@ -381,8 +381,8 @@ struct xc_bfogc {
   // FP32-rounded density matrix in FP64
   std::vector<WorkItem> dens_by_block;
   std::vector<int> dens_item0, dens_cls;
-  DBuf<WorkItem> d_dens_by_block, d_dense_list[kDensMaxS + 1];
-  DBuf<int> d_dens_item0, d_dens_cls, d_dense_cnt;
+  DBuf<WorkItem> d_dens_by_block, d_dense_list[kDensMaxS + 1], d_carry_list[kDensMaxS + 1];
+  DBuf<int> d_dens_item0, d_dens_cls, d_dense_cnt, d_carry_cnt;
   // mixed precision: the blocks contracted in FP64 by the last mixed build that were still dense
   // (one byte per block); the next mixed build's FP32 pass skips them
   DBuf<unsigned char> d_dense_mask;
@@ -438,34 +438,53 @@ void over_lanes(const Lanes& L, int nclass, const std::function<int(int)>& count
 }
 
 template <typename T>
-void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad, const Lanes& L,
-                 const unsigned char* skip = nullptr) {
+void run_density(xc_bfogc* bf, const T* tiles, const T* P, int want_grad, const Lanes& L) {
   over_lanes(L, kDensMaxS + 1, [&](int c) { return c >= 1 ? (int)bf->dens_work[c].size() : 0; },
              [&](int c, hipStream_t s) {
     launch_density_sub<T>(s, c, bf->d_dens_work[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
-                      bf->d_ao_idx.p, tiles, P, bf->basis->nao, want_grad, bf->d_part.p, bf->grid->npts, nullptr,
-                      nullptr, skip);
+                      bf->d_ao_idx.p, tiles, P, bf->basis->nao, want_grad, bf->d_part.p, bf->grid->npts);
   });
 }
 
-// Mixed precision: FP64 density items (FP64 tiles, the FP32-rounded P in FP64) for the blocks
-// whose largest density after the FP32 pass exceeds kMixedRhoThreshold, and for the blocks the
-// FP32 pass skipped because the last mixed build found them dense; they overwrite those blocks'
-// partials.  The lists are filled on the device, so the launches cover their capacity and run
-// only the filled part.  The mask then keeps the blocks whose FP64 density exceeds the threshold.
-void run_density_refine(xc_bfogc* bf, int want_grad, const Lanes& L) {
-  DenseLists lists{};
-  for (int c = 0; c <= kDensMaxS; ++c) lists.item[c] = bf->d_dense_list[c].p;
-  lists.count = bf->d_dense_cnt.p;
-  launch_mark_dense(L.main, bf->d_blocks.p, bf->nblocks, bf->grid->npts, bf->d_part.p, kMixedRhoThreshold,
-                    bf->d_dens_by_block.p, bf->d_dens_item0.p, bf->d_dens_cls.p, lists, bf->d_dense_mask.p);
+// Mixed precision: the FP32 density pass, and FP64 density items (FP64 tiles, the FP32-rounded P
+// in FP64) for the blocks whose largest density exceeds kMixedRhoThreshold.  The controller's mask
+// holds the blocks the last mixed build found dense: their FP64 items run beside the FP32 pass over
+// the other blocks (on the helper lanes its smaller classes leave free), then the blocks outside
+// the mask that the FP32 pass finds dense get theirs, and the mask keeps the blocks whose FP64
+// density exceeds the threshold.  The lists are filled on the device, so those launches cover the
+// lists' capacity and run only the filled part.
+void run_density_mixed(xc_bfogc* bf, int want_grad, const Lanes& L) {
+  DenseLists carry{}, late{};
+  for (int c = 0; c <= kDensMaxS; ++c) {
+    carry.item[c] = bf->d_carry_list[c].p;
+    late.item[c] = bf->d_dense_list[c].p;
+  }
+  carry.count = bf->d_carry_cnt.p;
+  late.count = bf->d_dense_cnt.p;
+  const int n = bf->basis->nao;
+  const int64_t N = bf->grid->npts;
+  auto fp64_pass = [&](int c, hipStream_t s, const DenseLists& lists) {
+    launch_density_sub<double>(s, c, lists.item[c], (int)bf->dens_work[c].size(), bf->d_blocks.p, bf->d_ao_idx.p,
+                               bf->d_t64.p, bf->d_P32d.p, n, want_grad, bf->d_part.p, N, nullptr, lists.count + c);
+  };
+  launch_list_masked(L.main, bf->d_blocks.p, bf->nblocks, bf->d_dens_by_block.p, bf->d_dens_item0.p,
+                     bf->d_dens_cls.p, carry, bf->d_dense_mask.p);
+  const bool par = L.n > 1;
+  if (par) L.fork_all();
+  int k = 0, kb = L.n - 1;
+  for (int c = kDensMaxS; c >= 1; --c)
+    if (!bf->dens_work[c].empty())
+      launch_density_sub<float>(par ? L.lane(k++) : L.main, c, bf->d_dens_work[c].p, (int)bf->dens_work[c].size(),
+                                bf->d_blocks.p, bf->d_ao_idx.p, bf->d_t32.p, bf->d_P32.p, n, want_grad,
+                                bf->d_part.p, N, nullptr, nullptr, bf->d_dense_mask.p);
+  for (int c = kDensMaxS; c >= 1; --c)
+    if (!bf->dens_work[c].empty()) fp64_pass(c, par ? L.lane(kb-- + 4 * L.n) : L.main, carry);
+  if (par) L.join_all();
+  launch_mark_dense(L.main, bf->d_blocks.p, bf->nblocks, N, bf->d_part.p, kMixedRhoThreshold, bf->d_dens_by_block.p,
+                    bf->d_dens_item0.p, bf->d_dens_cls.p, late, bf->d_dense_mask.p);
   over_lanes(L, kDensMaxS + 1, [&](int c) { return c >= 1 ? (int)bf->dens_work[c].size() : 0; },
-             [&](int c, hipStream_t s) {
-    launch_density_sub<double>(s, c, bf->d_dense_list[c].p, (int)bf->dens_work[c].size(), bf->d_blocks.p,
-                               bf->d_ao_idx.p, bf->d_t64.p, bf->d_P32d.p, bf->basis->nao, want_grad, bf->d_part.p,
-                               bf->grid->npts, nullptr, bf->d_dense_cnt.p + c);
-  });
-  launch_refresh_dense(L.main, bf->d_blocks.p, bf->nblocks, bf->grid->npts, bf->d_part.p, kMixedRhoThreshold,
+             [&](int c, hipStream_t s) { fp64_pass(c, s, late); });
+  launch_refresh_dense(L.main, bf->d_blocks.p, bf->nblocks, N, bf->d_part.p, kMixedRhoThreshold,
                        bf->d_dense_mask.p);
 }
 
@@ -1119,9 +1138,12 @@ int xc_bfogc_create(xc_basis* basis, xc_grid* grid, int max_block_size, double r
         bf->d_dens_by_block.upload(ctx, bf->dens_by_block.data(), bf->dens_by_block.size());
         bf->d_dens_item0.upload(ctx, bf->dens_item0.data(), bf->dens_item0.size());
         bf->d_dens_cls.upload(ctx, bf->dens_cls.data(), bf->dens_cls.size());
-        for (int c = 1; c <= kDensMaxS; ++c)
+        for (int c = 1; c <= kDensMaxS; ++c) {
           bf->d_dense_list[c].alloc(ctx, std::max<size_t>(bf->dens_work[c].size(), 1));
+          bf->d_carry_list[c].alloc(ctx, std::max<size_t>(bf->dens_work[c].size(), 1));
+        }
         bf->d_dense_cnt.alloc(ctx, kDensMaxS + 1);
+        bf->d_carry_cnt.alloc(ctx, kDensMaxS + 1);
         bf->d_dense_mask.alloc(ctx, std::max(bf->nblocks, 1));
         HIPX(hipMemsetAsync(bf->d_dense_mask.p, 0, std::max(bf->nblocks, 1), ctx->stream));
       }
@@ -1413,8 +1435,10 @@ static void enqueue_build(xc_bfogc* bf, int functional, int precision, double th
   if (!lowp) {
     run_density<double>(bf, bf->d_t64.p, dP, gga ? 1 : 0, L);
   } else {
-    run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0, L, mixed ? bf->d_dense_mask.p : nullptr);
-    if (mixed) run_density_refine(bf, gga ? 1 : 0, L);
+    if (mixed)
+      run_density_mixed(bf, gga ? 1 : 0, L);
+    else
+      run_density<float>(bf, bf->d_t32.p, bf->d_P32.p, gga ? 1 : 0, L);
   }
   if (s == ctx->stream) ctx->mark();
   if (bf->bs == kTile) {
