@@ -471,14 +471,16 @@ void run_density_mixed(xc_bfogc* bf, int want_grad, const Lanes& L) {
                      bf->d_dens_cls.p, carry, bf->d_dense_mask.p);
   const bool par = L.n > 1;
   if (par) L.fork_all();
-  int k = 0, kb = L.n - 1;
+  // FP32 classes from the widest on lanes 0, 1, ...; FP64 classes from the widest on the last
+  // lane backwards, behind the FP32 pass's smallest classes
+  int k = 0, kb = 0;
   for (int c = kDensMaxS; c >= 1; --c)
     if (!bf->dens_work[c].empty())
       launch_density_sub<float>(par ? L.lane(k++) : L.main, c, bf->d_dens_work[c].p, (int)bf->dens_work[c].size(),
                                 bf->d_blocks.p, bf->d_ao_idx.p, bf->d_t32.p, bf->d_P32.p, n, want_grad,
                                 bf->d_part.p, N, nullptr, nullptr, bf->d_dense_mask.p);
   for (int c = kDensMaxS; c >= 1; --c)
-    if (!bf->dens_work[c].empty()) fp64_pass(c, par ? L.lane(kb-- + 4 * L.n) : L.main, carry);
+    if (!bf->dens_work[c].empty()) fp64_pass(c, par ? L.lane(L.n - 1 - kb++ % L.n) : L.main, carry);
   if (par) L.join_all();
   launch_mark_dense(L.main, bf->d_blocks.p, bf->nblocks, N, bf->d_part.p, kMixedRhoThreshold, bf->d_dens_by_block.p,
                     bf->d_dens_item0.p, bf->d_dens_cls.p, late, bf->d_dense_mask.p);
