@@ -36,15 +36,16 @@ def test_server_same_alone_and_in_mix(ctx):
         s["bf"] = xc.BasisFunctionOnGridController(ctx, s["basis"], s["grid"], 128, 1e-9, 1, keep_fp32=True)
     alone = {}
     for i, s in enumerate(systems):
-        for prec in ("FP64", "FP32"):
+        for prec in ("FP64", "FP32", "MIXED"):
             pot = xc.FuncPotential(s["bf"], s["func"], prec)
             alone[(i, prec)] = (pot.getMatrix(s["P"]), pot.getEnergy())
         Fo, Eo, _, _ = O.fock_build(s["basis"], s["grid"].points, s["grid"].weights, s["P"], s["func"])
         s["oracle"] = (Fo, Eo)
     srv = xc.Server(ctx, nslots=3, use_graphs=True)
     # three rounds over every (system, precision), interleaved so that builds of different
-    # systems overlap; a controller with a build in flight waits for its turn
-    queue = [(r, i, prec) for r in range(3) for prec in ("FP64", "FP32") for i in range(len(systems))]
+    # systems overlap; a controller with a build in flight waits for its turn (the mixed builds
+    # run on the slots' single stream, with the dense blocks carried from the build alone)
+    queue = [(r, i, prec) for r in range(3) for prec in ("FP64", "FP32", "MIXED") for i in range(len(systems))]
     tickets, results = {}, {}
     t0 = time.time()
     while queue or tickets:
@@ -62,7 +63,7 @@ def test_server_same_alone_and_in_mix(ctx):
         assert time.time() - t0 < 120, "server builds did not finish"
     st = srv.stats()
     say("server stats", st)
-    assert st["submitted"] == len(results) == 3 * 2 * len(systems)
+    assert st["submitted"] == len(results) == 3 * 3 * len(systems)
     assert st["graph_launches"] == st["submitted"]
     for (r, i, prec), (F, E) in results.items():
         Fa, Ea = alone[(i, prec)]
