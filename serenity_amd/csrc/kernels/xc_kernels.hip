@@ -658,13 +658,12 @@ __global__ __launch_bounds__(128) void functional_pol_kernel(int func, XcParams 
 // S per wavefront, each accumulating S_ij = phi_i^T G_j + G_i^T phi_j over the
 // block's 128 points in registers (dummy slots of a short item compute on row
 // 0 and are dropped).  The points stream in chunks of KC (32 FP32 / 16 FP64,
-// one 128-byte line of each AO row): the global loads of chunk c + 1 (along
-// the point axis of the AO-major tiles) are issued before the MFMAs of chunk c, and a chunk is
-// turned into phi and G = (w v / 2) phi + sum_k (w g_k) d_k phi rows of LDS
-// on its way in.  Inside a chunk, k-step t of lane group g takes point
-// KV g + t (KV = KC / 4), so each MFMA operand of a chunk is one 32-byte LDS
-// read of KV consecutive points of one AO row.  MFMAs are issued in groups of
-// GS independent accumulators.
+// one 128-byte line of each AO row): the global loads of a chunk (along the
+// point axis of the AO-major tiles) are issued a chunk or two before its MFMAs,
+// and a chunk is turned into phi and G = (w v / 2) phi + sum_k (w g_k) d_k phi
+// rows of LDS on its way in.  Inside a chunk, k-step t of lane group g takes
+// point KV g + t (KV = KC / 4), so each MFMA operand of a chunk is one 32-byte
+// LDS read of KV consecutive points of one AO row.
 template <typename T, int S>
 __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockItem* __restrict__ work,
                                                        const DevBlock* __restrict__ blocks,
@@ -679,11 +678,12 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   constexpr int NCH = kTile / KC;
   constexpr int NT = 64 * kFockWaves;            // threads
   constexpr int NTASK = kFockRows * 4 / NT;
-  constexpr int GS = kFockWaves == 16 ? 1 : 2;   // independent accumulators per MFMA group
   typedef T vec_t __attribute__((ext_vector_type(KV)));
-  // FP32: two LDS buffers, chunk c + 1 staged after the MFMAs of chunk c (one barrier per chunk;
-  // 4.25 -> 4.09 ms at (H2O)64/def2-TZVP).  FP64 keeps one buffer (the second measured 4 % slower).
-  constexpr int NBUF = sizeof(T) == 4 ? 2 : 1;
+  // Two LDS buffers, one barrier per chunk.  FP32 stages chunk c + 1 after the MFMAs of chunk c
+  // (4.25 -> 4.09 ms at (H2O)64/def2-TZVP against one buffer); FP64 stages it in the instruction
+  // stream of the MFMAs of slot 0 of chunk c, from registers loaded one chunk earlier (6.22 -> 5.88
+  // ms; the same order measured no faster in FP32).
+  constexpr int NBUF = 2;
   __shared__ __attribute__((aligned(16))) T sPhi2[NBUF][kFockRows][RS];
   __shared__ __attribute__((aligned(16))) T sG2[NBUF][kFockRows][RS];
   // w v / 2, w g_x, w g_y, w g_z of each point in the contraction type, one plane per quantity:
@@ -709,6 +709,12 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   const gmem<const T>* tb = to_global(tiles + bk.tile_off);
   const int64_t cstride = (int64_t)npad * kTile;
   vec_t raw[NTASK][4];
+  if constexpr (sizeof(T) == 8) {  // LDA leaves the gradient components zero (the FP64 staging uses them)
+#pragma unroll
+    for (int m = 0; m < NTASK; ++m)
+#pragma unroll
+      for (int k = 1; k < 4; ++k) raw[m][k] = vec_t(T(0));
+  }
   auto grow_of = [&](int lr) { return lr < it.na ? it.a0 + lr : it.b0 + (lr - it.na); };
   // staging task -> (row, quarter of the chunk): 16 consecutive lanes take 16 rows of one
   // quarter, so their 16-byte LDS stores (144-byte row stride) cover distinct banks
@@ -833,47 +839,76 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
   const int r = lane & 15, g = lane >> 4;
   // slots of this wavefront that hold a sub-tile (an item smaller than its class leaves the rest idle)
   const int slive = it.nsub > w ? (it.nsub - w + kFockWaves - 1) / kFockWaves : 0;
-  if constexpr (NBUF == 2) {
-    store(0);
+  auto slot = [&](int s, T(*sPhi)[RS], T(*sG)[RS]) {
+    const vec_t aP = *reinterpret_cast<const vec_t*>(&sPhi[li[s] + r][KV * g]);
+    const vec_t aG = *reinterpret_cast<const vec_t*>(&sG[li[s] + r][KV * g]);
+    const vec_t bP = *reinterpret_cast<const vec_t*>(&sPhi[lj[s] + r][KV * g]);
+    const vec_t bG = *reinterpret_cast<const vec_t*>(&sG[lj[s] + r][KV * g]);
+#pragma unroll
+    for (int t = 0; t < KV; ++t) acc[s] = M::step(aP[t], bG[t], acc[s]);
+#pragma unroll
+    for (int t = 0; t < KV; ++t) acc[s] = M::step(aG[t], bP[t], acc[s]);
+  };
+  if constexpr (sizeof(T) == 8) {
+    static_assert((NCH & (NCH - 1)) == 0, "chunks per item: a power of two");
+    // Staging without branches, so that the compiler can spread it between the MFMAs of slot 0:
+    // G takes all four terms (LDA's gradient potentials and components are zero), tasks beyond
+    // the item's rows write rows that no MFMA reads, and chunk NCH re-stages chunk 0 into the
+    // buffer no later chunk reads.
+    auto store_flat = [&](int c) {
+      T(*sPhi)[RS] = sPhi2[c & 1];
+      T(*sG)[RS] = sG2[c & 1];
+      const int p0 = (c & (NCH - 1)) * KC + KV * q;
+      const vec_t w0 = *reinterpret_cast<const vec_t*>(&spot[0][p0]);
+      const vec_t w1 = *reinterpret_cast<const vec_t*>(&spot[1][p0]);
+      const vec_t w2 = *reinterpret_cast<const vec_t*>(&spot[2][p0]);
+      const vec_t w3 = *reinterpret_cast<const vec_t*>(&spot[3][p0]);
+#pragma unroll
+      for (int m = 0; m < NTASK; ++m) {
+        vec_t gv;
+#pragma unroll
+        for (int e = 0; e < KV; ++e)
+          gv[e] = w0[e] * raw[m][0][e] + w1[e] * raw[m][1][e] + w2[e] * raw[m][2][e] + w3[e] * raw[m][3][e];
+        const int lr = row_of(tid + NT * m);
+        *reinterpret_cast<vec_t*>(&sPhi[lr][KV * q]) = raw[m][0];
+        *reinterpret_cast<vec_t*>(&sG[lr][KV * q]) = gv;
+      }
+    };
+    store_flat(0);
+    if (NCH > 1) load(1);
     __syncthreads();
-  }
-  for (int c = 0; c < NCH; ++c) {
-    T(*sPhi)[RS] = sPhi2[NBUF == 2 ? (c & 1) : 0];
-    T(*sG)[RS] = sG2[NBUF == 2 ? (c & 1) : 0];
-    if constexpr (NBUF == 1) {
-      store(c);
+    for (int c = 0; c < NCH; ++c) {
+      T(*sPhi)[RS] = sPhi2[c & 1];
+      T(*sG)[RS] = sG2[c & 1];
+      slot(0, sPhi, sG);  // every wavefront (an idle one computes on row 0 and drops the result)
+      store_flat(c + 1);
+      if (c + 2 < NCH) load(c + 2);
+#pragma unroll
+      for (int s = 1; s < S; ++s) {
+        if (s >= slive) break;  // wavefront-uniform
+        slot(s, sPhi, sG);
+        // keep one slot's operands live at a time (bounds the register file)
+        __builtin_amdgcn_sched_barrier(0);
+      }
       __syncthreads();
     }
-    if (c + 1 < NCH) load(c + 1);
-#pragma unroll
-    for (int s0 = 0; s0 < S; s0 += GS) {
-      if (s0 >= slive) break;  // wavefront-uniform
-      vec_t aP[GS], aG[GS], bP[GS], bG[GS];
-#pragma unroll
-      for (int u = 0; u < GS; ++u)
-        if (s0 + u < S) {
-          aP[u] = *reinterpret_cast<const vec_t*>(&sPhi[li[s0 + u] + r][KV * g]);
-          aG[u] = *reinterpret_cast<const vec_t*>(&sG[li[s0 + u] + r][KV * g]);
-          bP[u] = *reinterpret_cast<const vec_t*>(&sPhi[lj[s0 + u] + r][KV * g]);
-          bG[u] = *reinterpret_cast<const vec_t*>(&sG[lj[s0 + u] + r][KV * g]);
-        }
-#pragma unroll
-      for (int t = 0; t < KV; ++t)
-#pragma unroll
-        for (int u = 0; u < GS; ++u)
-          if (s0 + u < S) acc[s0 + u] = M::step(aP[u][t], bG[u][t], acc[s0 + u]);
-#pragma unroll
-      for (int t = 0; t < KV; ++t)
-#pragma unroll
-        for (int u = 0; u < GS; ++u)
-          if (s0 + u < S) acc[s0 + u] = M::step(aG[u][t], bP[u][t], acc[s0 + u]);
-      // keep one group's operands live at a time (bounds the register file)
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    if constexpr (NBUF == 2) {
-      if (c + 1 < NCH) store(c + 1);
-    }
+  } else {
+    store(0);
     __syncthreads();
+    for (int c = 0; c < NCH; ++c) {
+      T(*sPhi)[RS] = sPhi2[c & 1];
+      T(*sG)[RS] = sG2[c & 1];
+      if (c + 1 < NCH) load(c + 1);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        if (s >= slive) break;  // wavefront-uniform
+        slot(s, sPhi, sG);
+        // keep one slot's operands live at a time (bounds the register file)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      if (c + 1 < NCH) store(c + 1);
+      __syncthreads();
+    }
   }
 
   // store the sub-tiles (block sub-tile (I, J), I <= J; 16 lanes write 16 consecutive values of a row)
