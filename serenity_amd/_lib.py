@@ -56,6 +56,7 @@ def lib():
     _sig("xc_ctx_synchronize", C.c_int, vp)
     _sig("xc_ctx_stream", vp, vp)
     _sig("xc_ctx_memory", C.c_int, vp, C.POINTER(i64))
+    _sig("xc_ctx_set_deterministic", C.c_int, vp, C.c_int)
     _sig("xc_normalize_contraction", C.c_int, C.c_int, C.c_int, dp, dp)
     _sig("xc_load_basis", C.c_int, C.c_char_p, C.c_int, ip, dp, C.c_int, ip,
          C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), C.POINTER(dp), C.POINTER(dp), C.POINTER(dp))

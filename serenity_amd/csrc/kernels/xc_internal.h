@@ -248,7 +248,7 @@ constexpr int kReduceWaves = 16;
 constexpr int kReduceWindow = 8192;  // row columns accumulated in LDS per pass
 template <typename T>
 void launch_fock_reduce(hipStream_t s, const RowEnt* ent, const int* row_ptr, const T* Tscr, const int* ao_idx,
-                        double* U, int n, const PackSys* pack = nullptr, int nsys = 1);
+                        double* U, int n, const PackSys* pack = nullptr, int nsys = 1, bool fixed = false);
 // Packed elementwise stages over nsys systems (maxnn = largest n^2): P -> FP32,
 // F = mirror of U (+ FP32 rounding partials), E = sum of block energies (+ rounding term).
 constexpr int kPackRoundBlocks = 512;  // finalize workgroups per system of a packed build

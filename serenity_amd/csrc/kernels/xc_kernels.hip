@@ -936,16 +936,23 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
 // a multiple of 32 with -1 and its stored row a runs from column 16 (a / 16) to the padded end,
 // so an aligned group never leaves either; the columns of the group before the segment start are
 // masked).  The loads of a wavefront's next entry are issued before the current entry's LDS
-// additions.  Rows longer than the LDS window are done in several passes over the entries.  The LDS additions of different wavefronts meet in an order
-// that varies from run to run, so F varies at the rounding level.
-template <typename T>
+// additions.  Rows longer than the LDS window are done in several passes over the entries.
+// Fixed = false: FP64 additions in LDS, which the wavefronts make in an order that varies from run
+// to run, so F varies at the rounding level.  Fixed = true: each term x is split exactly into
+// integers, x = h 2^-32 + l 2^-72 + e with h = floor(x 2^32), 0 <= l <= 2^40 and |e| <= 2^-73, and
+// h and l are added into two 64-bit integer sums per column; integer sums do not depend on the
+// order, so F is the same bit for bit on every run (and the sum is exact up to the e terms).  The
+// sums stay in range while a column has fewer than 2^23 terms and the terms' absolute values sum
+// to less than 2^31.
+template <typename T, bool Fixed>
 __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const RowEnt* __restrict__ ent,
                                                                         const int* __restrict__ row_ptr,
                                                                         const T* __restrict__ Tscr,
                                                                         const int* __restrict__ ao_idx,
                                                                         double* __restrict__ U, int n, int window,
                                                                         const PackSys* __restrict__ pack) {
-  extern __shared__ double racc[];  // window
+  extern __shared__ unsigned long long racc_u[];  // window (FP64 sums) or 2 window (Fixed: h, then l sums)
+  double* racc = reinterpret_cast<double*>(racc_u);
   if (pack) {
     const PackSys& ps = pack[blockIdx.y];
     ent = ps.ent;
@@ -965,7 +972,14 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
   const int e0 = to_global(row_ptr)[i], e1 = to_global(row_ptr)[i + 1];
   for (int w0 = 0; w0 < n - i; w0 += window) {
     const int wl = min(window, n - i - w0);
-    for (int k = tid; k < wl; k += 64 * kReduceWaves) racc[k] = 0.0;
+    for (int k = tid; k < wl; k += 64 * kReduceWaves) {
+      if constexpr (Fixed) {
+        racc_u[k] = 0;
+        racc_u[window + k] = 0;
+      } else {
+        racc[k] = 0.0;
+      }
+    }
     __syncthreads();
     auto fetch = [&](const RowEnt& E, int c, i32x4& gi, double (&v)[4]) {
       gi = *reinterpret_cast<const gmem<const i32x4>*>(idxg + E.idx_off + c);
@@ -984,7 +998,16 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int g = gg[u] - i - w0;
-        if (c + u >= E.c0 && c + u < E.c1 && g >= 0 && g < wl) atomicAdd(racc + g, v[u]);
+        if (c + u >= E.c0 && c + u < E.c1 && g >= 0 && g < wl) {
+          if constexpr (Fixed) {
+            const double xs = v[u] * 0x1p32;
+            const double fl = floor(xs);
+            atomicAdd(racc_u + g, (unsigned long long)(long long)fl);
+            atomicAdd(racc_u + window + g, (unsigned long long)(long long)rint((xs - fl) * 0x1p40));
+          } else {
+            atomicAdd(racc + g, v[u]);
+          }
+        }
       }
     };
     // this wavefront's entries e0 + w + kReduceWaves k, k < nk: their RowEnts are read 64 at a time,
@@ -1040,7 +1063,14 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
       }
     }
     __syncthreads();
-    for (int j = tid; j < wl; j += 64 * kReduceWaves) to_global(U)[(int64_t)(i + w0 + j) + (int64_t)n * i] = racc[j];
+    for (int j = tid; j < wl; j += 64 * kReduceWaves) {
+      double f;
+      if constexpr (Fixed)
+        f = (double)(long long)racc_u[j] * 0x1p-32 + (double)(long long)racc_u[window + j] * 0x1p-72;
+      else
+        f = racc[j];
+      to_global(U)[(int64_t)(i + w0 + j) + (int64_t)n * i] = f;
+    }
     __syncthreads();
   }
 }
@@ -1438,35 +1468,39 @@ template void launch_fock_sub<float>(hipStream_t, int, const FockItem*, int, con
                                      const PackSys*);
 
 // Row columns per LDS pass of the reduce: kReduceWindow, or less where the device's LDS per
-// workgroup cannot hold kReduceWaves copies of it.
-static int reduce_window_limit() {
-  static int cached[64] = {};
+// workgroup cannot hold that many columns of 8 bytes (fixed = false) or 16 bytes (fixed = true).
+static int reduce_window_limit(bool fixed) {
+  static int cached[2][64] = {};
   int dev = 0;
   (void)hipGetDevice(&dev);
   if (dev < 0 || dev >= 64) dev = 0;
-  if (!cached[dev]) {
+  if (!cached[fixed][dev]) {
     int lds = 65536;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lds = 65536;
-    const int w = lds / (int)sizeof(double);
-    cached[dev] = (w < kReduceWindow ? w : kReduceWindow) / 64 * 64;
+    const int w = lds / (fixed ? 16 : 8);
+    cached[fixed][dev] = (w < kReduceWindow ? w : kReduceWindow) / 64 * 64;
   }
-  return cached[dev];
+  return cached[fixed][dev];
 }
 
 template <typename T>
 void launch_fock_reduce(hipStream_t s, const RowEnt* ent, const int* row_ptr, const T* Tscr, const int* ao_idx,
-                        double* U, int n, const PackSys* pack, int nsys) {
+                        double* U, int n, const PackSys* pack, int nsys, bool fixed) {
   if (n <= 0 || nsys <= 0) return;
-  const int lim = reduce_window_limit();
+  const int lim = reduce_window_limit(fixed);
   const int window = n < lim ? n : lim;
-  const size_t lds = sizeof(double) * (size_t)window;
-  hipLaunchKernelGGL((fock_reduce_kernel<T>), dim3(n, nsys), dim3(64 * kReduceWaves), lds, s, ent, row_ptr, Tscr,
-                     ao_idx, U, n, window, pack);
+  const size_t lds = (fixed ? 16 : 8) * (size_t)window;
+  if (fixed)
+    hipLaunchKernelGGL((fock_reduce_kernel<T, true>), dim3(n, nsys), dim3(64 * kReduceWaves), lds, s, ent, row_ptr,
+                       Tscr, ao_idx, U, n, window, pack);
+  else
+    hipLaunchKernelGGL((fock_reduce_kernel<T, false>), dim3(n, nsys), dim3(64 * kReduceWaves), lds, s, ent, row_ptr,
+                       Tscr, ao_idx, U, n, window, pack);
 }
 template void launch_fock_reduce<double>(hipStream_t, const RowEnt*, const int*, const double*, const int*, double*,
-                                         int, const PackSys*, int);
+                                         int, const PackSys*, int, bool);
 template void launch_fock_reduce<float>(hipStream_t, const RowEnt*, const int*, const float*, const int*, double*,
-                                        int, const PackSys*, int);
+                                        int, const PackSys*, int, bool);
 
 void launch_weight_potential(hipStream_t s, int64_t npts, const double* w, const double* v, const double* g,
                              double* vs, double* gw) {

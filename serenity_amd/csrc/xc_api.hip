@@ -182,6 +182,9 @@ struct xc_ctx {
   int live_handles = 0;                 // bases, grids and controllers not yet destroyed
   // stage timing: XC_NSTAGES + 1 events per profiled build
   bool profile = false;
+  // every row reduce sums in 64-bit integers (F the same bit for bit on every run); set while no
+  // basis, grid or controller exists
+  bool deterministic = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   hipEvent_t next_event() {
@@ -521,7 +524,7 @@ void run_fock_tiles(xc_bfogc* bf, const T* tiles, int gga, double thr, const Lan
 template <typename T>
 void run_fock_reduce(xc_bfogc* bf, const Lanes& L) {
   launch_fock_reduce<T>(L.main, bf->d_rowent.p, bf->d_rowptr.p, reinterpret_cast<const T*>(bf->d_T.p),
-                        bf->d_ao_idx.p, bf->d_U.p, bf->basis->nao);
+                        bf->d_ao_idx.p, bf->d_U.p, bf->basis->nao, nullptr, 1, bf->ctx->deterministic);
 }
 
 template <typename T>
@@ -2101,9 +2104,11 @@ int xc_fock_build_packed(int nsys, xc_bfogc* const* bfs, int functional, int pre
                                     nullptr, nullptr, g, thr, nullptr, 0, dsys);
         });
         if (fp32)
-          launch_fock_reduce<float>(s, nullptr, nullptr, nullptr, nullptr, nullptr, maxn, dsys, nsys);
+          launch_fock_reduce<float>(s, nullptr, nullptr, nullptr, nullptr, nullptr, maxn, dsys, nsys,
+                                    ctx->deterministic);
         else
-          launch_fock_reduce<double>(s, nullptr, nullptr, nullptr, nullptr, nullptr, maxn, dsys, nsys);
+          launch_fock_reduce<double>(s, nullptr, nullptr, nullptr, nullptr, nullptr, maxn, dsys, nsys,
+                                     ctx->deterministic);
         launch_pack_finalize(s, dsys, nsys, fp32 ? 1 : 0);
         launch_pack_sum(s, dsys, nsys, fp32 ? 1 : 0);
       };
@@ -2442,6 +2447,16 @@ int xc_fock_build(xc_bfogc* bf, int functional, int precision, double thr, const
     HIPX(hipMemcpyAsync(&ctx->last_energy, ctx->d_energy, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPX(hipStreamSynchronize(ctx->stream));
     if (energy) *energy = ctx->last_energy;
+  });
+}
+
+int xc_ctx_set_deterministic(xc_ctx* ctx, int enable) {
+  return guarded([&] {
+    require(ctx, XC_ERR_ARG, "null context");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    require(ctx->live_handles == 0, XC_ERR_STATE,
+            "xc_ctx_set_deterministic: set before any basis, grid or controller of the context exists");
+    ctx->deterministic = enable != 0;
   });
 }
 

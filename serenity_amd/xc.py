@@ -96,6 +96,11 @@ class Context:
         """Device-to-device copy by the library's streaming copy kernel."""
         check(_lib.lib().xc_device_copy(self.handle, dst, src, C.c_size_t(nbytes)))
 
+    def set_deterministic(self, enable=True):
+        """Sum every Fock element's terms in exact integer parts, so that F is the same bit for bit
+        on every run; only before any basis, grid or controller of the context exists."""
+        check(_lib.lib().xc_ctx_set_deterministic(self.handle, int(bool(enable))))
+
     def profile(self, enable=True):
         """Record HIP events between the build stages of every following build."""
         check(_lib.lib().xc_profile(self.handle, int(bool(enable))))
