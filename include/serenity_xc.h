@@ -88,9 +88,10 @@ int xc_ctx_memory(xc_ctx* ctx, int64_t* bytes_in_use);
  * src/data/grid/ScalarOperatorToMatrixAdder.cpp:66-110).  enable != 0:
  * every row reduce of the context adds each element's terms as exact 64-bit integer parts
  * (2^-32 and 2^-72 units) instead of FP64 additions in a varying order, so F is the same on
- * every run, packed or alone; the element is exact up to 2^-73 per term.  Costs ~0.37 ms of
- * the 7.8 ms mixed build of (H2O)64/def2-TZVP.  Valid while a column has fewer than 2^23
- * terms whose absolute values sum to less than 2^31.  Fails with XC_ERR_STATE once a basis,
+ * every run, packed or alone; the element is exact up to 2^-73 per term.  Costs ~0.14 ms of
+ * the 7.8 ms mixed build of (H2O)64/def2-TZVP.  Valid for terms below 2^19 in magnitude in
+ * columns of fewer than 2^23 terms whose absolute values sum to less than 2^30 (a Fock
+ * element's terms are far smaller).  Fails with XC_ERR_STATE once a basis,
  * grid or controller of the context exists: set it right after xc_ctx_create. */
 int xc_ctx_set_deterministic(xc_ctx* ctx, int enable);
 

@@ -938,12 +938,14 @@ __global__ __launch_bounds__(64 * kFockWaves) void fock_sub_kernel(const FockIte
 // masked).  The loads of a wavefront's next entry are issued before the current entry's LDS
 // additions.  Rows longer than the LDS window are done in several passes over the entries.
 // Fixed = false: FP64 additions in LDS, which the wavefronts make in an order that varies from run
-// to run, so F varies at the rounding level.  Fixed = true: each term x is split exactly into
-// integers, x = h 2^-32 + l 2^-72 + e with h = floor(x 2^32), 0 <= l <= 2^40 and |e| <= 2^-73, and
-// h and l are added into two 64-bit integer sums per column; integer sums do not depend on the
-// order, so F is the same bit for bit on every run (and the sum is exact up to the e terms).  The
-// sums stay in range while a column has fewer than 2^23 terms and the terms' absolute values sum
-// to less than 2^31.
+// to run, so F varies at the rounding level.  Fixed = true: each term x is split into integers,
+// x = h 2^-32 + l 2^-72 + e with h = x 2^32 and l = (x - h 2^-32) 2^72 rounded to the nearest
+// integer, |l| <= 2^39 and |e| <= 2^-73, and h and l are added into two 64-bit integer sums per
+// column; integer sums do not depend on the order, so F is the same bit for bit on every run (and
+// the sum is exact up to the e terms).  The rounding adds 1.5 2^20 (1.5 2^52) in FP64, whose unit
+// in the last place is then 2^-32 (1), and reads the integer from the bit pattern.  Valid for
+// terms of |x| < 2^19 in columns of fewer than 2^23 terms whose absolute values sum to less than
+// 2^30.
 template <typename T, bool Fixed>
 __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const RowEnt* __restrict__ ent,
                                                                         const int* __restrict__ row_ptr,
@@ -1000,10 +1002,12 @@ __global__ __launch_bounds__(64 * kReduceWaves) void fock_reduce_kernel(const Ro
         const int g = gg[u] - i - w0;
         if (c + u >= E.c0 && c + u < E.c1 && g >= 0 && g < wl) {
           if constexpr (Fixed) {
-            const double xs = v[u] * 0x1p32;
-            const double fl = floor(xs);
-            atomicAdd(racc_u + g, (unsigned long long)(long long)fl);
-            atomicAdd(racc_u + window + g, (unsigned long long)(long long)rint((xs - fl) * 0x1p40));
+            const double t = v[u] + 0x1.8p20;
+            const long long h = __double_as_longlong(t) - __double_as_longlong(0x1.8p20);
+            const double rem = v[u] - (t - 0x1.8p20);  // exact, |rem| <= 2^-33
+            const long long l = __double_as_longlong(rem * 0x1p72 + 0x1.8p52) - __double_as_longlong(0x1.8p52);
+            atomicAdd(racc_u + g, (unsigned long long)h);
+            atomicAdd(racc_u + window + g, (unsigned long long)l);
           } else {
             atomicAdd(racc + g, v[u]);
           }
